@@ -1,0 +1,14 @@
+"""MI355X-native numeric engine for the multifrontal path of CliqueTrees.jl.
+
+The directory name is not a Python identifier; load it with
+``importlib.util.spec_from_file_location`` (see ``load_package`` in
+``__graft_entry__.py``) under the module name ``cliquetrees_amd``.
+"""
+from .symbolic import ChordalSymbolic, Permutation, symbolic, flatindices, grid_nd, mmd  # noqa: F401
+from .factorization import (  # noqa: F401
+    ChordalCholesky, ChordalLDLt, ChordalFactorization, DynamicRegularization,
+    PosDefException, SingularException, MFError, cholesky_, ldlt_, ldiv_, solve, issuccess,
+    chol_impl, div_impl, device_plan, DevicePlan, DeviceFactor, checkinfo,
+    MF_FP64, MF_FP32, MF_CHOLESKY, MF_LDLT, STAT_NAMES, KC_NAMES,
+)
+from . import problems  # noqa: F401

@@ -1,0 +1,43 @@
+"""ctypes signatures of libmfhip.so (include/mfhip.h)."""
+import ctypes
+
+i64 = ctypes.c_int64
+i64p = ctypes.POINTER(ctypes.c_int64)
+f64p = ctypes.POINTER(ctypes.c_double)
+vp = ctypes.c_void_p
+ci = ctypes.c_int
+cd = ctypes.c_double
+
+SIGNATURES = {
+    "mf_last_error": ([], ctypes.c_char_p),
+    "mf_plan_create": ([i64, i64] + [i64p] * 8 + [i64, ctypes.POINTER(vp)], ci),
+    "mf_plan_destroy": ([vp], ci),
+    "mf_plan_stats": ([vp, i64p, i64], ci),
+    "mf_factor_create": ([vp, ci, ctypes.POINTER(vp)], ci),
+    "mf_factor_destroy": ([vp], ci),
+    "mf_factor_set_stream": ([vp, vp], ci),
+    "mf_factor_sync": ([vp], ci),
+    "mf_factor_set_values": ([vp, vp, vp, ci], ci),
+    "mf_factor_set_pattern": ([vp, i64, i64p], ci),
+    "mf_factor_refresh": ([vp, vp, ci], ci),
+    "mf_factorize": ([vp, ci, vp, ci, cd, cd, i64p], ci),
+    "mf_factorize_async": ([vp, ci, vp, ci, cd, cd], ci),
+    "mf_factor_info": ([vp, i64p], ci),
+    "mf_factor_get": ([vp, vp, vp, vp, ci], ci),
+    "mf_solve_natural": ([vp, vp, i64, i64, ci], ci),
+    "mf_solve": ([vp, i64p, vp, i64, i64, ci], ci),
+    "mf_chol_impl": ([vp, f64p, f64p, f64p, vp, ci, ci, cd, cd, i64p], ci),
+    "mf_div_impl": ([vp, f64p, f64p, vp, f64p, i64, i64, ci], ci),
+    "mf_timing": ([vp, ci], ci),
+    "mf_timing_read": ([vp, f64p, i64p, f64p, ci], ci),
+}
+
+# Names of the C-ABI entry points declared in include/mfhip.h.
+EXPORTS = tuple(SIGNATURES)
+
+
+def declare(lib):
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res

@@ -1,0 +1,733 @@
+// mf_api.hip — the extern "C" boundary of libmfhip.so (include/mfhip.h):
+// plan upload, factor storage, the launch sequences of the factorization
+// and of the solves, kernel timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mfhip.h"
+#include "mf_kernels.h"
+#include "mf_plan.h"
+
+using namespace mf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                    \
+    do {                                                                             \
+        hipError_t e_ = (x);                                                         \
+        if (e_ != hipSuccess)                                                        \
+            return fail(MF_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+template <typename V>
+int upload(const std::vector<V>& h, V** d, std::vector<void*>& allocs) {
+    size_t bytes = std::max<size_t>(h.size() * sizeof(V), 16);
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return fail(MF_ENOMEM, "hipMalloc failed (plan)");
+    allocs.push_back(p);
+    if (!h.empty() && hipMemcpy(p, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(MF_EHIP, "hipMemcpy failed (plan)");
+    *d = (V*)p;
+    return MF_OK;
+}
+
+}  // namespace
+
+struct mf_plan {
+    HostPlan H;
+    DevPlan D{};
+    std::vector<void*> allocs;
+    std::vector<uint8_t> in_small;
+    double ea_bytes = 0, small_flops = 0, diag_flops = 0;
+    ~mf_plan() {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
+struct TimedLaunch {
+    int kc;
+    hipEvent_t a, b;
+};
+
+struct mf_factor {
+    mf_plan* plan = nullptr;
+    int precision = MF_FP64;
+    hipStream_t own = nullptr, stream = nullptr;
+    void* panel = nullptr;
+    void* U = nullptr;
+    void* W = nullptr;
+    void* d = nullptr;          // pivots in the front precision
+    double* Dref = nullptr;     // reference-layout Dval / Lval / d (double)
+    double* Lref = nullptr;
+    double* dref = nullptr;
+    double* signs = nullptr;
+    unsigned long long* info = nullptr;
+    int32_t* flags = nullptr;
+    uint32_t* ticket = nullptr;
+    int32_t* hang = nullptr;
+    int64_t* tgt_panel = nullptr;
+    int64_t* tgt_ref = nullptr;
+    int64_t nnz = -1;
+    double* stage = nullptr;    // staging for host nzval / values
+    size_t stage_bytes = 0;
+    void* Mv = nullptr;
+    size_t Mv_bytes = 0;
+    void* Bws = nullptr;
+    size_t Bws_bytes = 0;
+    int64_t* perm_dev = nullptr;
+    int diag = MF_CHOLESKY;
+    int reg_kind = 0;
+    double delta = 0, eps = 0;
+    bool factored = false;
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> evpool;
+    double t_ms[MF_KC_COUNT] = {0};
+    int64_t t_cnt[MF_KC_COUNT] = {0};
+    double t_work[MF_KC_COUNT] = {0};
+    std::vector<void*> allocs;
+    ~mf_factor() {
+        for (auto& t : pending) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        for (auto e : evpool) (void)hipEventDestroy(e);
+        for (void* p : allocs) (void)hipFree(p);
+        if (Mv) (void)hipFree(Mv);
+        if (Bws) (void)hipFree(Bws);
+        if (stage) (void)hipFree(stage);
+        if (own) (void)hipStreamDestroy(own);
+    }
+};
+
+namespace {
+
+hipEvent_t get_event(mf_factor* f) {
+    if (!f->evpool.empty()) {
+        hipEvent_t e = f->evpool.back();
+        f->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Times one launch (class kc) when timing is on; nothing is launched or
+// recorded for an empty task range.
+template <typename Fn>
+void timed(mf_factor* f, int kc, double work, int64_t count, Fn&& fn) {
+    if (count <= 0) return;
+    if (!f->timing) {
+        fn();
+        return;
+    }
+    TimedLaunch t{kc, get_event(f), get_event(f)};
+    (void)hipEventRecord(t.a, f->stream);
+    fn();
+    (void)hipEventRecord(t.b, f->stream);
+    f->pending.push_back(t);
+    f->t_work[kc] += work;
+}
+
+void drain_timing(mf_factor* f) {
+    if (f->pending.empty()) return;
+    (void)hipStreamSynchronize(f->stream);
+    for (auto& t : f->pending) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
+        f->t_ms[t.kc] += ms;
+        f->t_cnt[t.kc] += 1;
+        f->evpool.push_back(t.a);
+        f->evpool.push_back(t.b);
+    }
+    f->pending.clear();
+}
+
+template <typename T>
+DevFactor<T> dev_factor(mf_factor* f) {
+    DevFactor<T> F{};
+    F.panel = (T*)f->panel;
+    F.U = (T*)f->U;
+    F.W = (T*)f->W;
+    F.d = (T*)f->d;
+    F.signs = f->signs;
+    F.info = f->info;
+    F.flags = f->flags;
+    F.ticket = f->ticket;
+    F.hang = f->hang;
+    F.diag = f->diag;
+    F.reg_kind = f->reg_kind;
+    F.delta = f->delta;
+    F.eps = f->eps;
+    return F;
+}
+
+int ensure(void** p, size_t* have, size_t need) {
+    if (*have >= need && *p) return MF_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    if (hipMalloc(p, std::max<size_t>(need, 16)) != hipSuccess) return fail(MF_ENOMEM, "hipMalloc failed (workspace)");
+    *have = std::max<size_t>(need, 16);
+    return MF_OK;
+}
+
+// Launch sequence of one numeric factorization (values already in the panels).
+template <typename T>
+void enqueue_factor(mf_factor* f) {
+    mf_plan* pl = f->plan;
+    const HostPlan& H = pl->H;
+    const DevPlan& P = pl->D;
+    hipStream_t s = f->stream;
+    DevFactor<T> F = dev_factor<T>(f);
+    launch_fill<unsigned long long>(f->info, 1, ~0ull, s);
+    launch_fill<int32_t>(f->flags, std::max<int>(P.nJ, 1), 0, s);
+    launch_fill<uint32_t>(f->ticket, 1, 0u, s);
+    timed(f, MF_KC_SMALL, pl->small_flops, H.n_small_region, [&] {
+        launch_small<T>(P, F, P.small_order, (int)H.n_small_region, true, s);
+    });
+    for (const Level& L : H.levels) {
+        timed(f, MF_KC_SMALL, 0, L.smallL1 - L.smallL0, [&] {
+            launch_small<T>(P, F, P.level_small + L.smallL0, L.smallL1 - L.smallL0, false, s);
+        });
+        timed(f, MF_KC_EA, 0, L.ea1 - L.ea0, [&] { launch_ea<T>(P, F, L.ea0, L.ea1, s); });
+        for (const Step& st : L.steps) {
+            timed(f, MF_KC_DIAG, 0, st.diag1 - st.diag0,
+                  [&] { launch_diag<T>(P, F, st.diag0, st.diag1, false, s); });
+            timed(f, MF_KC_GEMM, 0, st.trsm1 - st.trsm0,
+                  [&] { launch_gemm<T>(P, F, st.trsm0, st.trsm1, s); });
+            timed(f, MF_KC_GEMM, 0, st.trail1 - st.trail0,
+                  [&] { launch_gemm<T>(P, F, st.trail0, st.trail1, s); });
+        }
+        timed(f, MF_KC_GEMM, 0, L.syrk1 - L.syrk0, [&] { launch_gemm<T>(P, F, L.syrk0, L.syrk1, s); });
+    }
+    if (f->timing) {
+        f->t_work[MF_KC_GEMM] += H.gemm_flops;
+        f->t_work[MF_KC_EA] += pl->ea_bytes;
+        f->t_work[MF_KC_DIAG] += pl->diag_flops;
+    }
+    timed(f, MF_KC_COPYOUT, 8.0 * 2.0 * (double)(H.ndz + H.nlz), H.nfr, [&] {
+        launch_copyout<T>(P, F, f->Dref, f->Lref, f->dref, (int)H.nfr, (int)H.n, s);
+    });
+}
+
+template <typename T>
+void enqueue_solve(mf_factor* f, T* B, int64_t ldb, int nrhs) {
+    mf_plan* pl = f->plan;
+    const HostPlan& H = pl->H;
+    const DevPlan& P = pl->D;
+    hipStream_t s = f->stream;
+    DevFactor<T> F = dev_factor<T>(f);
+    DevSolve<T> S{B, ldb, (T*)f->Mv, nrhs};
+    double bytes = (double)sizeof(T) * (2.0 * (double)(H.ndz + H.nlz) + 4.0 * H.n * nrhs);
+    timed(f, MF_KC_SOLVE, bytes, 1, [&] {
+        launch_fill<int32_t>(f->flags, std::max<int>(P.nJ, 1), 0, s);
+        launch_fill<uint32_t>(f->ticket, 1, 0u, s);
+        launch_solve_small_fwd<T>(P, F, S, P.small_order, (int)H.n_small_region, true, s);
+        for (const Level& L : H.levels) {
+            launch_solve_small_fwd<T>(P, F, S, P.level_small + L.smallL0, L.smallL1 - L.smallL0,
+                                      false, s);
+            launch_sgather<T>(P, F, S, L.sea0, L.sea1, s);
+            for (const SolveStep& st : L.fsteps) {
+                launch_sdiag<T>(P, F, S, st.t0, st.t1, false, s);
+                launch_supd_fwd<T>(P, F, S, st.u0, st.u1, s);
+            }
+        }
+        if (f->diag == MF_LDLT) launch_dscale<T>(F, S, (int)H.n, s);
+        for (auto it = H.levels.rbegin(); it != H.levels.rend(); ++it) {
+            const Level& L = *it;
+            launch_solve_small_bwd<T>(P, F, S, P.level_small + L.smallL0, L.smallL1 - L.smallL0,
+                                      false, s);
+            launch_sbgather<T>(P, F, S, L.bg0, L.bg1, s);
+            for (const SolveStep& st : L.bsteps) {
+                launch_sdiag<T>(P, F, S, st.t0, st.t1, true, s);
+                launch_supd_bwd<T>(P, F, S, st.u0, st.u1, s);
+            }
+        }
+        launch_fill<int32_t>(f->flags, std::max<int>(P.nJ, 1), 0, s);
+        launch_fill<uint32_t>(f->ticket, 1, 0u, s);
+        launch_solve_small_bwd<T>(P, F, S, P.small_order, (int)H.n_small_region, true, s);
+    });
+}
+
+size_t elt(const mf_factor* f) { return f->precision == MF_FP32 ? 4 : 8; }
+
+int read_info(mf_factor* f, int64_t* info) {
+    unsigned long long v = 0;
+    int32_t hang = 0;
+    HIPCHK(hipMemcpyAsync(&v, f->info, sizeof(v), hipMemcpyDeviceToHost, f->stream));
+    HIPCHK(hipMemcpyAsync(&hang, f->hang, sizeof(hang), hipMemcpyDeviceToHost, f->stream));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    if (hang) return fail(MF_EHIP, "a bounded device wait gave up (dependency flag never set)");
+    if (info) *info = (v == ~0ull) ? 0 : (int64_t)v;
+    return MF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mf_last_error(void) { return g_err.c_str(); }
+
+int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                   const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                   const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
+                   int64_t flags, mf_plan** out) {
+    (void)flags;
+    if (!out || !resptr || !sepptr || !chdptr || !Dptr || !Lptr || (nfr > 0 && !chdtgt))
+        return fail(MF_EARG, "null argument");
+    *out = nullptr;
+    mf_plan* pl = new mf_plan();
+    std::string err = build_plan(pl->H, n, nfr, resptr, sepptr, septgt, reltgt, chdptr, chdtgt,
+                                 Dptr, Lptr, 0.05);
+    if (!err.empty()) {
+        delete pl;
+        return fail(MF_EARG, "plan: " + err);
+    }
+    HostPlan& H = pl->H;
+    int rc = MF_OK;
+    DevPlan& D = pl->D;
+    pl->in_small.assign(H.fronts.size(), 0);
+    for (int32_t J : H.small_order) pl->in_small[J] = 1;
+    std::vector<int32_t> ea_src = H.ea_src;
+    if ((rc = upload(H.fronts, (Front**)&D.fronts, pl->allocs)) ||
+        (rc = upload(H.rows, (int32_t**)&D.rows, pl->allocs)) ||
+        (rc = upload(H.chd, (int32_t**)&D.chd, pl->allocs)) ||
+        (rc = upload(H.rel, (int32_t**)&D.rel, pl->allocs)) ||
+        (rc = upload(H.small_order, (int32_t**)&D.small_order, pl->allocs)) ||
+        (rc = upload(H.level_small, (int32_t**)&D.level_small, pl->allocs)) ||
+        (rc = upload(H.level_big, (int32_t**)&D.level_big, pl->allocs)) ||
+        (rc = upload(H.ea, (EaTask**)&D.ea, pl->allocs)) ||
+        (rc = upload(ea_src, (int32_t**)&D.ea_src, pl->allocs)) ||
+        (rc = upload(H.gemm, (GemmTask**)&D.gemm, pl->allocs)) ||
+        (rc = upload(H.diag, (DiagTask**)&D.diag, pl->allocs)) ||
+        (rc = upload(H.stask, (SolveTask**)&D.stask, pl->allocs)) ||
+        (rc = upload(H.out, (OutFront**)&D.out, pl->allocs)) ||
+        (rc = upload(H.septpos, (int32_t**)&D.septpos, pl->allocs)) ||
+        (rc = upload(H.mvoff1, (int64_t**)&D.mvoff1, pl->allocs)) ||
+        (rc = upload(pl->in_small, (uint8_t**)&D.in_small, pl->allocs))) {
+        delete pl;
+        return rc;
+    }
+    D.nJ = (int32_t)H.fronts.size();
+    D.n = (int32_t)H.n;
+    D.nfr = (int32_t)H.nfr;
+    D.n_small_region = (int32_t)H.n_small_region;
+    // algorithmic work per kernel class
+    for (size_t J = 0; J < H.fronts.size(); ++J) {
+        const Front& F = H.fronts[J];
+        double a = F.nn, b = F.na;
+        if (H.is_small[J]) pl->small_flops += a * a * a / 3.0 + b * a * a + b * b * a;
+        else
+            for (int32_t q = F.chd0; q < F.chd1; ++q) {
+                double m = H.fronts[H.chd[q]].na;
+                pl->ea_bytes += 3.0 * 8.0 * m * (m + 1) / 2.0 + 4.0 * m;
+            }
+    }
+    for (const DiagTask& t : H.diag) pl->diag_flops += (double)t.kb * t.kb * t.kb / 3.0;
+    *out = pl;
+    return MF_OK;
+}
+
+int mf_plan_destroy(mf_plan* plan) {
+    delete plan;
+    return MF_OK;
+}
+
+int mf_plan_stats(const mf_plan* pl, int64_t* st, int64_t nst) {
+    if (!pl || !st) return fail(MF_EARG, "null argument");
+    const HostPlan& H = pl->H;
+    int64_t v[] = {H.n, H.nfr, (int64_t)H.fronts.size(), H.n_small_region, H.n_big,
+                   (int64_t)H.levels.size(), H.small_levels, H.max_nj, H.panel_elems, H.u_elems,
+                   H.w_elems, (int64_t)H.gemm.size(), (int64_t)H.ea.size(),
+                   (int64_t)H.diag.size(), (int64_t)llround(H.ref_flops),
+                   (int64_t)llround(H.engine_flops), (int64_t)llround(H.gemm_flops), H.ndz,
+                   H.nlz};
+    int64_t m = (int64_t)(sizeof(v) / sizeof(v[0]));
+    for (int64_t i = 0; i < nst && i < m; ++i) st[i] = v[i];
+    return MF_OK;
+}
+
+int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
+    if (!pl || !out || (precision != MF_FP64 && precision != MF_FP32))
+        return fail(MF_EARG, "bad argument");
+    *out = nullptr;
+    mf_factor* f = new mf_factor();
+    f->plan = pl;
+    f->precision = precision;
+    const HostPlan& H = pl->H;
+    size_t es = precision == MF_FP32 ? 4 : 8;
+    auto alloc = [&](void** p, size_t bytes) -> bool {
+        if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
+        f->allocs.push_back(*p);
+        return true;
+    };
+    bool ok = alloc(&f->panel, es * (size_t)H.panel_elems) && alloc(&f->U, es * (size_t)H.u_elems) &&
+              alloc(&f->W, es * (size_t)H.w_elems) && alloc(&f->d, es * (size_t)(H.n + 1)) &&
+              alloc((void**)&f->Dref, 8 * (size_t)H.ndz) && alloc((void**)&f->Lref, 8 * (size_t)H.nlz) &&
+              alloc((void**)&f->dref, 8 * (size_t)(H.n + 1)) &&
+              alloc((void**)&f->signs, 8 * (size_t)(H.n + 1)) &&
+              alloc((void**)&f->info, 64) && alloc((void**)&f->flags, 4 * (H.fronts.size() + 1)) &&
+              alloc((void**)&f->ticket, 64) && alloc((void**)&f->hang, 64) &&
+              alloc((void**)&f->perm_dev, 8 * (size_t)(H.n + 1));
+    if (!ok) {
+        delete f;
+        return fail(MF_ENOMEM, "hipMalloc failed (factor)");
+    }
+    if (hipStreamCreateWithFlags(&f->own, hipStreamNonBlocking) != hipSuccess) {
+        delete f;
+        return fail(MF_EHIP, "hipStreamCreate failed");
+    }
+    f->stream = f->own;
+    launch_fill<int32_t>(f->hang, 1, 0, f->stream);
+    if (H.n > 0) {
+        if (precision == MF_FP32) launch_fill<float>((float*)f->d, H.n, 1.0f, f->stream);
+        else launch_fill<double>((double*)f->d, H.n, 1.0, f->stream);
+        launch_fill<double>(f->dref, H.n, 1.0, f->stream);
+    }
+    if (hipStreamSynchronize(f->stream) != hipSuccess) {
+        delete f;
+        return fail(MF_EHIP, "initialization failed");
+    }
+    *out = f;
+    return MF_OK;
+}
+
+int mf_factor_destroy(mf_factor* f) {
+    if (f) (void)hipStreamSynchronize(f->stream);
+    delete f;
+    return MF_OK;
+}
+
+int mf_factor_set_stream(mf_factor* f, void* stream) {
+    if (!f) return fail(MF_EARG, "null factor");
+    f->stream = stream ? (hipStream_t)stream : f->own;
+    return MF_OK;
+}
+
+int mf_factor_sync(mf_factor* f) {
+    if (!f) return fail(MF_EARG, "null factor");
+    HIPCHK(hipStreamSynchronize(f->stream));
+    return MF_OK;
+}
+
+int mf_factor_set_values(mf_factor* f, const double* Dval, const double* Lval, int device) {
+    if (!f || (!Dval && f->plan->H.ndz) || (!Lval && f->plan->H.nlz))
+        return fail(MF_EARG, "null argument");
+    const HostPlan& H = f->plan->H;
+    hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (H.ndz) HIPCHK(hipMemcpyAsync(f->Dref, Dval, 8 * H.ndz, k, f->stream));
+    if (H.nlz) HIPCHK(hipMemcpyAsync(f->Lref, Lval, 8 * H.nlz, k, f->stream));
+    timed(f, MF_KC_ASSEMBLE, 8.0 * (double)(H.ndz + H.nlz), 1, [&] {
+        if (f->precision == MF_FP32) {
+            launch_fill<float>((float*)f->panel, H.panel_elems, 0.0f, f->stream);
+            launch_assemble_ref<float>(f->plan->D, dev_factor<float>(f), f->Dref, f->Lref,
+                                       (int)H.nfr, f->stream);
+        } else {
+            launch_fill<double>((double*)f->panel, H.panel_elems, 0.0, f->stream);
+            launch_assemble_ref<double>(f->plan->D, dev_factor<double>(f), f->Dref, f->Lref,
+                                        (int)H.nfr, f->stream);
+        }
+    });
+    HIPCHK(hipGetLastError());
+    if (!device) HIPCHK(hipStreamSynchronize(f->stream));
+    f->factored = false;
+    return MF_OK;
+}
+
+int mf_factor_set_pattern(mf_factor* f, int64_t nnz, const int64_t* flat) {
+    if (!f || nnz < 0 || (nnz > 0 && !flat)) return fail(MF_EARG, "bad argument");
+    const HostPlan& H = f->plan->H;
+    std::vector<int64_t> tp(nnz, -1), tr(nnz, -1);
+    for (int64_t k = 0; k < nnz; ++k) {
+        int64_t p = flat[k];
+        if (p == 0) continue;
+        if (p < 0 || p > H.ndz + H.nlz) return fail(MF_EARG, "flat index out of range");
+        int64_t q = p - 1;
+        tr[k] = q;
+        int64_t j, e, r, c;
+        const OutFront* o;
+        if (q < H.ndz) {
+            j = std::upper_bound(H.Dptr.begin(), H.Dptr.end(), q) - H.Dptr.begin() - 1;
+            o = &H.out[j];
+            e = q - H.Dptr[j];
+            r = e % o->nn;
+            c = e / o->nn;
+            if (r < c) return fail(MF_EARG, "flat index in the upper triangle");
+            const Front& F = H.fronts[o->J];
+            int64_t cl = o->r0 - F.c0, nj = F.nn + F.na;
+            tp[k] = F.poff + (cl + r) + (cl + c) * nj;
+        } else {
+            q -= H.ndz;
+            j = std::upper_bound(H.Lptr.begin(), H.Lptr.end(), q) - H.Lptr.begin() - 1;
+            // skip fronts with empty L blocks (Lptr repeats)
+            while (j > 0 && H.Lptr[j] == H.Lptr[j + 1]) --j;
+            o = &H.out[j];
+            e = q - H.Lptr[j];
+            r = e % o->na;
+            c = e / o->na;
+            const Front& F = H.fronts[o->J];
+            int64_t cl = o->r0 - F.c0, nj = F.nn + F.na;
+            tp[k] = F.poff + H.septpos[o->sepoff + r] + (cl + c) * nj;
+        }
+    }
+    if (f->tgt_panel) (void)hipFree(f->tgt_panel);
+    if (f->tgt_ref) (void)hipFree(f->tgt_ref);
+    f->tgt_panel = f->tgt_ref = nullptr;
+    HIPCHK(hipMalloc(&f->tgt_panel, 8 * std::max<int64_t>(nnz, 2)));
+    HIPCHK(hipMalloc(&f->tgt_ref, 8 * std::max<int64_t>(nnz, 2)));
+    if (nnz) {
+        HIPCHK(hipMemcpy(f->tgt_panel, tp.data(), 8 * nnz, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(f->tgt_ref, tr.data(), 8 * nnz, hipMemcpyHostToDevice));
+    }
+    f->nnz = nnz;
+    return MF_OK;
+}
+
+int mf_factor_refresh(mf_factor* f, const double* nzval, int device) {
+    if (!f || f->nnz < 0) return fail(MF_ESTATE, "no pattern registered");
+    if (f->nnz > 0 && !nzval) return fail(MF_EARG, "null values");
+    const HostPlan& H = f->plan->H;
+    const double* src = nzval;
+    if (!device && f->nnz > 0) {
+        if (ensure((void**)&f->stage, &f->stage_bytes, 8 * (size_t)f->nnz)) return MF_ENOMEM;
+        HIPCHK(hipMemcpyAsync(f->stage, nzval, 8 * f->nnz, hipMemcpyHostToDevice, f->stream));
+        src = f->stage;
+    }
+    timed(f, MF_KC_ASSEMBLE, 8.0 * (double)(H.panel_elems + H.ndz + H.nlz) + 24.0 * f->nnz, 1, [&] {
+        launch_fill<double>(f->Dref, H.ndz, 0.0, f->stream);
+        launch_fill<double>(f->Lref, H.nlz, 0.0, f->stream);
+        if (f->precision == MF_FP32) {
+            launch_fill<float>((float*)f->panel, H.panel_elems, 0.0f, f->stream);
+            launch_refresh<float>(dev_factor<float>(f), f->tgt_panel, f->tgt_ref, src, f->nnz,
+                                  f->Dref, H.ndz, f->Lref, f->stream);
+        } else {
+            launch_fill<double>((double*)f->panel, H.panel_elems, 0.0, f->stream);
+            launch_refresh<double>(dev_factor<double>(f), f->tgt_panel, f->tgt_ref, src, f->nnz,
+                                   f->Dref, H.ndz, f->Lref, f->stream);
+        }
+    });
+    HIPCHK(hipGetLastError());
+    if (!device) HIPCHK(hipStreamSynchronize(f->stream));
+    f->factored = false;
+    return MF_OK;
+}
+
+int mf_factorize_async(mf_factor* f, int diag, const double* signs_dev, int reg_kind,
+                       double delta, double eps) {
+    if (!f || (diag != MF_CHOLESKY && diag != MF_LDLT) ||
+        (reg_kind != MF_REG_NONE && reg_kind != MF_REG_DYNAMIC))
+        return fail(MF_EARG, "bad argument");
+    const HostPlan& H = f->plan->H;
+    f->diag = diag;
+    f->reg_kind = reg_kind;
+    f->delta = delta;
+    f->eps = eps;
+    double* keep = f->signs;
+    if (signs_dev) {
+        if (signs_dev != f->signs)
+            HIPCHK(hipMemcpyAsync(f->signs, signs_dev, 8 * H.n, hipMemcpyDeviceToDevice, f->stream));
+    } else {
+        f->signs = nullptr;  // default signs: Ones (Cholesky) / Zeros (LDLt)
+    }
+    if (f->precision == MF_FP32) enqueue_factor<float>(f);
+    else enqueue_factor<double>(f);
+    f->signs = keep;
+    HIPCHK(hipGetLastError());
+    f->factored = true;
+    return MF_OK;
+}
+
+int mf_factor_info(mf_factor* f, int64_t* info) {
+    if (!f) return fail(MF_EARG, "null factor");
+    return read_info(f, info);
+}
+
+int mf_factorize(mf_factor* f, int diag, const double* signs, int reg_kind, double delta,
+                 double eps, int64_t* info) {
+    if (!f) return fail(MF_EARG, "null factor");
+    int rc;
+    if (signs) {
+        HIPCHK(hipMemcpyAsync(f->signs, signs, 8 * f->plan->H.n, hipMemcpyHostToDevice, f->stream));
+        rc = mf_factorize_async(f, diag, f->signs, reg_kind, delta, eps);
+    } else {
+        rc = mf_factorize_async(f, diag, nullptr, reg_kind, delta, eps);
+    }
+    if (rc) return rc;
+    return read_info(f, info);
+}
+
+int mf_factor_get(mf_factor* f, double* Dval, double* Lval, double* d, int device) {
+    if (!f) return fail(MF_EARG, "null factor");
+    const HostPlan& H = f->plan->H;
+    hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (Dval && H.ndz) HIPCHK(hipMemcpyAsync(Dval, f->Dref, 8 * H.ndz, k, f->stream));
+    if (Lval && H.nlz) HIPCHK(hipMemcpyAsync(Lval, f->Lref, 8 * H.nlz, k, f->stream));
+    if (d && H.n) HIPCHK(hipMemcpyAsync(d, f->dref, 8 * H.n, k, f->stream));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    return MF_OK;
+}
+
+int mf_solve_natural(mf_factor* f, double* C, int64_t nrhs, int64_t ldc, int device) {
+    if (!f || nrhs < 0 || (nrhs > 0 && !C)) return fail(MF_EARG, "bad argument");
+    const HostPlan& H = f->plan->H;
+    if (ldc < H.n) return fail(MF_EARG, "ldc < n");
+    if (nrhs == 0 || H.n == 0) return MF_OK;
+    size_t es = elt(f);
+    if (ensure(&f->Mv, &f->Mv_bytes, es * (size_t)H.mv_elems1 * nrhs)) return MF_ENOMEM;
+    if (f->precision == MF_FP64 && device) {
+        enqueue_solve<double>(f, C, ldc, (int)nrhs);
+    } else {
+        if (ensure(&f->Bws, &f->Bws_bytes, es * (size_t)H.n * nrhs)) return MF_ENOMEM;
+        // identity permutation copies through the workspace
+        std::vector<int64_t> id;
+        const double* src = C;
+        double* stage = nullptr;
+        if (!device) {
+            if (ensure((void**)&f->stage, &f->stage_bytes, 8 * (size_t)H.n * nrhs)) return MF_ENOMEM;
+            for (int64_t c = 0; c < nrhs; ++c)
+                HIPCHK(hipMemcpyAsync(f->stage + c * H.n, C + c * ldc, 8 * H.n, hipMemcpyHostToDevice,
+                                      f->stream));
+            src = f->stage;
+            stage = f->stage;
+        }
+        int64_t lds = device ? ldc : H.n;
+        if (f->precision == MF_FP32) {
+            // permutation by the identity = conversion
+            id.resize(H.n);
+            for (int64_t i = 0; i < H.n; ++i) id[i] = i + 1;
+            HIPCHK(hipMemcpyAsync(f->perm_dev, id.data(), 8 * H.n, hipMemcpyHostToDevice, f->stream));
+            launch_permute<float>(src, lds, (float*)f->Bws, H.n, f->perm_dev, (int)H.n, (int)nrhs,
+                                  false, f->stream);
+            enqueue_solve<float>(f, (float*)f->Bws, H.n, (int)nrhs);
+            double* dst = device ? C : stage;
+            launch_permute_back(f->Bws, H.n, true, dst, device ? ldc : H.n, f->perm_dev, (int)H.n,
+                                (int)nrhs, f->stream);
+        } else {
+            HIPCHK(hipMemcpyAsync(f->Bws, src, 8 * H.n * nrhs, hipMemcpyDeviceToDevice, f->stream));
+            enqueue_solve<double>(f, (double*)f->Bws, H.n, (int)nrhs);
+            HIPCHK(hipMemcpyAsync(stage, f->Bws, 8 * H.n * nrhs, hipMemcpyDeviceToDevice, f->stream));
+        }
+        if (!device) {
+            for (int64_t c = 0; c < nrhs; ++c)
+                HIPCHK(hipMemcpyAsync(C + c * ldc, f->stage + c * H.n, 8 * H.n, hipMemcpyDeviceToHost,
+                                      f->stream));
+        }
+    }
+    HIPCHK(hipGetLastError());
+    if (!device) HIPCHK(hipStreamSynchronize(f->stream));
+    return MF_OK;
+}
+
+int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t ldb, int device) {
+    if (!f || !perm || nrhs < 0 || (nrhs > 0 && !B)) return fail(MF_EARG, "bad argument");
+    const HostPlan& H = f->plan->H;
+    if (ldb < H.n) return fail(MF_EARG, "ldb < n");
+    if (nrhs == 0 || H.n == 0) return MF_OK;
+    size_t es = elt(f);
+    if (ensure(&f->Mv, &f->Mv_bytes, es * (size_t)H.mv_elems1 * nrhs)) return MF_ENOMEM;
+    if (ensure(&f->Bws, &f->Bws_bytes, es * (size_t)H.n * nrhs)) return MF_ENOMEM;
+    HIPCHK(hipMemcpyAsync(f->perm_dev, perm, 8 * H.n, hipMemcpyHostToDevice, f->stream));
+    double* src = B;
+    int64_t lds = ldb;
+    if (!device) {
+        if (ensure((void**)&f->stage, &f->stage_bytes, 8 * (size_t)H.n * nrhs)) return MF_ENOMEM;
+        for (int64_t c = 0; c < nrhs; ++c)
+            HIPCHK(hipMemcpyAsync(f->stage + c * H.n, B + c * ldb, 8 * H.n, hipMemcpyHostToDevice,
+                                  f->stream));
+        src = f->stage;
+        lds = H.n;
+    }
+    if (f->precision == MF_FP32) {
+        launch_permute<float>(src, lds, (float*)f->Bws, H.n, f->perm_dev, (int)H.n, (int)nrhs, false,
+                              f->stream);
+        enqueue_solve<float>(f, (float*)f->Bws, H.n, (int)nrhs);
+        launch_permute_back(f->Bws, H.n, true, src, lds, f->perm_dev, (int)H.n, (int)nrhs, f->stream);
+    } else {
+        launch_permute<double>(src, lds, (double*)f->Bws, H.n, f->perm_dev, (int)H.n, (int)nrhs,
+                               false, f->stream);
+        enqueue_solve<double>(f, (double*)f->Bws, H.n, (int)nrhs);
+        launch_permute_back(f->Bws, H.n, false, src, lds, f->perm_dev, (int)H.n, (int)nrhs,
+                            f->stream);
+    }
+    if (!device) {
+        for (int64_t c = 0; c < nrhs; ++c)
+            HIPCHK(hipMemcpyAsync(B + c * ldb, f->stage + c * H.n, 8 * H.n, hipMemcpyDeviceToHost,
+                                  f->stream));
+    }
+    HIPCHK(hipGetLastError());
+    if (!device) HIPCHK(hipStreamSynchronize(f->stream));
+    return MF_OK;
+}
+
+int mf_chol_impl(mf_plan* plan, double* Dval, double* Lval, double* d, const double* signs,
+                 int diag, int reg_kind, double delta, double eps, int64_t* info) {
+    mf_factor* f = nullptr;
+    int rc = mf_factor_create(plan, MF_FP64, &f);
+    if (rc) return rc;
+    if (!(rc = mf_factor_set_values(f, Dval, Lval, 0)) &&
+        !(rc = mf_factorize(f, diag, signs, reg_kind, delta, eps, info)))
+        rc = mf_factor_get(f, Dval, Lval, diag == MF_LDLT ? d : nullptr, 0);
+    mf_factor_destroy(f);
+    return rc;
+}
+
+int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const double* d,
+                double* C, int64_t nrhs, int64_t ldc, int diag) {
+    mf_factor* f = nullptr;
+    int rc = mf_factor_create(plan, MF_FP64, &f);
+    if (rc) return rc;
+    f->diag = diag;
+    if (!(rc = mf_factor_set_values(f, Dval, Lval, 0))) {
+        if (diag == MF_LDLT && d) {
+            if (hipMemcpy(f->d, d, 8 * plan->H.n, hipMemcpyHostToDevice) != hipSuccess)
+                rc = fail(MF_EHIP, "hipMemcpy d");
+        }
+        // the big-front solves use the inverted 64x64 diagonal blocks
+        DevFactor<double> F = dev_factor<double>(f);
+        for (const Level& L : plan->H.levels)
+            for (const Step& st : L.steps)
+                launch_diag<double>(plan->D, F, st.diag0, st.diag1, true, f->stream);
+        if (!rc) rc = mf_solve_natural(f, C, nrhs, ldc, 0);
+    }
+    mf_factor_destroy(f);
+    return rc;
+}
+
+int mf_timing(mf_factor* f, int enable) {
+    if (!f) return fail(MF_EARG, "null factor");
+    f->timing = enable != 0;
+    return MF_OK;
+}
+
+int mf_timing_read(mf_factor* f, double* times_ms, int64_t* counts, double* work, int reset) {
+    if (!f) return fail(MF_EARG, "null factor");
+    drain_timing(f);
+    for (int k = 0; k < MF_KC_COUNT; ++k) {
+        if (times_ms) times_ms[k] = f->t_ms[k];
+        if (counts) counts[k] = f->t_cnt[k];
+        if (work) work[k] = f->t_work[k];
+        if (reset) {
+            f->t_ms[k] = 0;
+            f->t_cnt[k] = 0;
+            f->t_work[k] = 0;
+        }
+    }
+    return MF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// mf_internal.h — data structures shared by the planner (host) and the
+// kernels of libmfhip.so.
+//
+// Layout in HBM (one factor object):
+//   panel pool  For every internal front J, the stacked panel [D_JJ; L_J]
+//               (nJ x nnJ, column-major, ld = nJ) — the F11/F21 part of the
+//               reference's frontal matrix (cholesky.jl:467-500).
+//   U pool      Update matrix U_J = F22 - L_J L_J' (naJ x naJ, ld = naJ),
+//               lower triangle meaningful (cholesky.jl:531-553).
+//   W pool      Inverses of the 64x64 diagonal blocks of big fronts
+//               (nnJ x 64, ld = nnJ), reused by the solves.
+//   Dval/Lval/d The reference's own factor arrays, written by the copy-out
+//               kernel from the panels.
+// An internal front is a chain of reference fronts j_1 -> j_2 -> ... (each
+// the last child of the next, hence contiguous in postorder and in columns)
+// merged when the explicit zeros it introduces stay below a bound.  Entries
+// that are structurally zero in the reference factor stay exactly zero.
+#pragma once
+
+#include <cstdint>
+
+namespace mf {
+
+constexpr int NB = 64;          // panel width and GEMM tile edge of big fronts
+constexpr int SMALL_NJ = 64;    // fronts with nJ <= SMALL_NJ use the fused LDS kernel
+constexpr int EA_CHUNK = 256;   // rows per extend-add task
+
+struct Front {
+    int64_t poff;    // panel offset (elements) in the panel pool
+    int64_t uoff;    // U offset (elements) in the U pool, -1 if na == 0
+    int64_t woff;    // W offset (elements), -1 for small fronts
+    int32_t c0;      // first global (permuted) column, 0-based
+    int32_t nn;      // eliminated columns
+    int32_t na;      // update rows
+    int32_t rowoff;  // offset of the row list (nn + na entries) in rows[]
+    int32_t chd0;    // children of J in chd[] (in accumulation order)
+    int32_t chd1;
+    int32_t reloff;  // offset of J's rel map (na entries) in rel[]: positions of
+                     // J's update rows inside its parent's row list
+    int32_t parent;  // internal parent, -1 for roots
+};
+
+// Extend-add task: parent front, parent column, parent row range.
+struct EaTask {
+    int32_t front;
+    int32_t col;
+    int32_t row0;
+    int32_t row1;
+    int32_t src0;   // sources [src0, src1) in ea_src (child, child column)
+    int32_t src1;
+};
+
+enum : int8_t { ARR_PANEL = 0, ARR_U = 1, ARR_W = 2 };
+enum : int8_t {
+    G_SUB = 0,      // C -= A diag(s) B'            (trailing / SYRK updates)
+    G_TRSM = 1,     // C  = A W' [diag(1/s)]        (panel solve, A aliases C)
+};
+
+// One 64x64 output tile of a batched GEMM.
+struct GemmTask {
+    int64_t a_off, b_off, c_off;
+    int32_t lda, ldb, ldc;
+    int32_t m, n, k;
+    int32_t s_col;  // global column of s[0] (LDLt pivots), -1: none
+    int8_t arrA, arrB, arrC, mode;
+    int8_t lower;   // store only r >= c (diagonal tile)
+    int8_t pad[3];
+};
+
+// Factor one diagonal block of a big front and invert it.
+struct DiagTask {
+    int32_t front;
+    int32_t k0;     // first local column of the block
+    int32_t kb;     // block width (<= NB)
+    int32_t pad;
+};
+
+// Copy-out of one reference front from its internal panel.
+struct OutFront {
+    int64_t Dp, Lp;      // 0-based offsets in Dval / Lval
+    int32_t J;           // internal front
+    int32_t r0;          // first global column of the reference front
+    int32_t nn, na;
+    int32_t sepoff;      // offset in septpos[] of this front's separator rows
+    int32_t pad;
+};
+
+}  // namespace mf

@@ -1,0 +1,117 @@
+// mf_kernels.h — device argument blocks and host launch wrappers of the
+// factorization and solve kernels (mf_factor.hip, mf_solve.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mf_internal.h"
+#include "mf_plan.h"
+
+namespace mf {
+
+// Device view of a plan (all arrays resident in HBM).
+struct DevPlan {
+    const Front* fronts;
+    const int32_t* rows;
+    const int32_t* chd;
+    const int32_t* rel;
+    const int32_t* small_order;
+    const int32_t* level_small;
+    const int32_t* level_big;
+    const EaTask* ea;
+    const int32_t* ea_src;
+    const GemmTask* gemm;
+    const DiagTask* diag;
+    const SolveTask* stask;
+    const OutFront* out;
+    const int32_t* septpos;
+    const int64_t* mvoff1;
+    const uint8_t* in_small;     // 1: front of the small region
+    int32_t nJ;
+    int32_t n;
+    int32_t nfr;
+    int32_t n_small_region;
+};
+
+// Device view of one factor object's storage.
+template <typename T>
+struct DevFactor {
+    T* panel;
+    T* U;
+    T* W;
+    T* d;                    // pivots (permuted order), LDLt
+    const double* signs;     // NULL: default signs
+    unsigned long long* info;  // min failing global column (1-based), ~0ull: none
+    int32_t* flags;          // per internal front, persistent kernels
+    uint32_t* ticket;
+    int32_t* hang;           // set when a bounded wait gives up
+    int diag;                // 0 Cholesky, 1 LDLt
+    int reg_kind;
+    double delta, eps;
+};
+
+// ---- factorization (mf_factor.hip) ----
+template <typename T>
+void launch_fill(T* p, int64_t count, T value, hipStream_t s);
+template <typename T>
+void launch_assemble_ref(const DevPlan& P, const DevFactor<T>& F, const double* Dval,
+                         const double* Lval, int nfr, hipStream_t s);
+template <typename T>
+void launch_refresh(const DevFactor<T>& F, const int64_t* tgt_panel, const int64_t* tgt_ref,
+                    const double* nzval, int64_t nnz, double* Dref, int64_t ndz, double* Lref,
+                    hipStream_t s);
+template <typename T>
+void launch_small(const DevPlan& P, const DevFactor<T>& F, const int32_t* list, int count,
+                  bool persistent, hipStream_t s);
+template <typename T>
+void launch_ea(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, hipStream_t s);
+// invert_only: the block already holds the factor; only W is produced.
+template <typename T>
+void launch_diag(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, bool invert_only,
+                 hipStream_t s);
+template <typename T>
+void launch_gemm(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, hipStream_t s);
+template <typename T>
+void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, double* Lval,
+                    double* dout, int nfr, int n, hipStream_t s);
+
+// ---- solves (mf_solve.hip) ----
+template <typename T>
+struct DevSolve {
+    T* B;        // n x nrhs, permuted order
+    int64_t ldb;
+    T* Mv;       // update vectors, na_J x nrhs at mvoff1[J]*nrhs
+    int nrhs;
+};
+template <typename T>
+void launch_solve_small_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
+                            const int32_t* list, int count, bool persistent, hipStream_t s);
+template <typename T>
+void launch_solve_small_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
+                            const int32_t* list, int count, bool persistent, hipStream_t s);
+template <typename T>
+void launch_sgather(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
+                    int t1, hipStream_t s);
+template <typename T>
+void launch_sdiag(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
+                  int t1, bool backward, hipStream_t s);
+template <typename T>
+void launch_supd_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
+                     int t1, hipStream_t s);
+template <typename T>
+void launch_sbgather(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
+                     int t1, hipStream_t s);
+template <typename T>
+void launch_supd_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
+                     int t1, hipStream_t s);
+template <typename T>
+void launch_dscale(const DevFactor<T>& F, const DevSolve<T>& S, int n, hipStream_t s);
+template <typename TB>
+void launch_permute(const double* src, int64_t lds, TB* dst, int64_t ldd, const int64_t* perm,
+                    int n, int nrhs, bool inverse, hipStream_t s);
+void launch_permute_back(const void* src, int64_t lds, bool src_f32, double* dst, int64_t ldd,
+                         const int64_t* perm, int n, int nrhs, hipStream_t s);
+
+}  // namespace mf

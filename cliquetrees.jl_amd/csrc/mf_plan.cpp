@@ -1,0 +1,618 @@
+// mf_plan.cpp — host planner of libmfhip.so.
+//
+// Input: the reference's ChordalSymbolic arrays (1-based Int64,
+// src/Multifrontal.jl/src/chordal_symbolic.jl:15-28).  Output: HostPlan.
+//   1. internal fronts: chains j -> pnt(j) with pnt(j) = j + 1 (j is the last
+//      child, so columns are contiguous) merged while the explicit zeros stay
+//      under `zero_frac` of the merged panel (or the panel is tiny);
+//   2. small region (whole subtree with nJ <= SMALL_NJ): one persistent
+//      launch in ticket order; big region: levels by height;
+//   3. memory: panel pool (all fronts), U pool (small region: private slots;
+//      big region: slots reused after the parent's extend-add), W pool;
+//   4. static task lists of every launch of the factorization and the solves.
+#include "mf_plan.h"
+
+#include <algorithm>
+#include <map>
+#include <numeric>
+
+namespace mf {
+
+namespace {
+
+// First-fit allocator over a linear address space with coalescing.
+struct Allocator {
+    std::map<int64_t, int64_t> free_;  // offset -> size
+    int64_t top = 0;
+    int64_t peak = 0;
+    int64_t alloc(int64_t size) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= size) {
+                int64_t off = it->first;
+                int64_t rest = it->second - size;
+                free_.erase(it);
+                if (rest > 0) free_[off + size] = rest;
+                return off;
+            }
+        }
+        int64_t off = top;
+        top += size;
+        peak = std::max(peak, top);
+        return off;
+    }
+    void release(int64_t off, int64_t size) {
+        auto it = free_.emplace(off, size).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+                it = pv;
+            }
+        }
+        // give back a free block that ends at the top
+        auto last = std::prev(free_.end());
+        if (last->first + last->second == top) {
+            top = last->first;
+            free_.erase(last);
+        }
+    }
+};
+
+inline int64_t rup(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* resptr,
+                       const int64_t* sepptr, const int64_t* septgt, const int64_t* reltgt,
+                       const int64_t* chdptr, const int64_t* chdtgt, const int64_t* Dptr,
+                       const int64_t* Lptr, double zero_frac) {
+    (void)reltgt;
+    if (n < 0 || nfr < 0) return "negative size";
+    if (n > (int64_t)INT32_MAX / 2) return "n too large";
+    P.n = n;
+    P.nfr = nfr;
+    P.resptr.resize(nfr + 1);
+    P.sepptr.resize(nfr + 1);
+    P.Dptr.resize(nfr + 1);
+    P.Lptr.resize(nfr + 1);
+    for (int64_t j = 0; j <= nfr; ++j) {
+        P.resptr[j] = resptr[j] - 1;
+        P.sepptr[j] = sepptr[j] - 1;
+        P.Dptr[j] = Dptr[j] - 1;
+        P.Lptr[j] = Lptr[j] - 1;
+    }
+    if (P.resptr[0] != 0 || P.resptr[nfr] != n) return "resptr does not partition 1..n";
+    int64_t nsep = P.sepptr[nfr];
+    P.septgt.resize(nsep);
+    for (int64_t p = 0; p < nsep; ++p) {
+        P.septgt[p] = septgt[p] - 1;
+        if (P.septgt[p] < 0 || P.septgt[p] >= n) return "septgt out of range";
+    }
+    P.ndz = P.Dptr[nfr];
+    P.nlz = P.Lptr[nfr];
+    // reference parents from the children lists (chd has nfr+2 pointers)
+    P.pnt.assign(nfr, -1);
+    for (int64_t j = 0; j < nfr; ++j) {
+        for (int64_t q = chdptr[j] - 1; q < chdptr[j + 1] - 1; ++q) {
+            int64_t c = chdtgt[q] - 1;
+            if (c < 0 || c >= nfr || c >= j) return "children must precede parents";
+            P.pnt[c] = (int32_t)j;
+        }
+    }
+    auto nnref = [&](int64_t j) { return P.resptr[j + 1] - P.resptr[j]; };
+    auto naref = [&](int64_t j) { return P.sepptr[j + 1] - P.sepptr[j]; };
+    for (int64_t j = 0; j < nfr; ++j) {
+        if (nnref(j) <= 0) return "empty front";
+        if (Dptr[j + 1] - Dptr[j] != nnref(j) * nnref(j)) return "Dptr inconsistent";
+        if (Lptr[j + 1] - Lptr[j] != nnref(j) * naref(j)) return "Lptr inconsistent";
+        for (int64_t p = P.sepptr[j]; p < P.sepptr[j + 1]; ++p) {
+            if (P.septgt[p] < P.resptr[j + 1]) return "separator row inside its front";
+            if (p > P.sepptr[j] && P.septgt[p] <= P.septgt[p - 1]) return "separator not sorted";
+        }
+    }
+
+    // ---- 1. amalgamation --------------------------------------------------
+    std::vector<int32_t> gowner(nfr);
+    std::vector<int64_t> gcols(nfr), gzeros(nfr), gc0(nfr);
+    std::vector<int32_t> gtop(nfr);
+    for (int64_t j = 0; j < nfr; ++j) {
+        gowner[j] = (int32_t)j;
+        gcols[j] = nnref(j);
+        gzeros[j] = 0;
+        gc0[j] = P.resptr[j];
+        gtop[j] = (int32_t)j;
+    }
+    for (int64_t j = 0; j < nfr; ++j) {
+        int64_t p = P.pnt[j];
+        if (p != j + 1) continue;
+        int32_t G = gowner[j];
+        int64_t c1 = gcols[G], c2 = nnref(p), naj = naref(j), nap = naref(p);
+        int64_t extra = c1 * (c2 + nap - naj);
+        int64_t C = c1 + c2;
+        int64_t total = C * (C + 1) / 2 + C * nap;
+        int64_t z = gzeros[G] + extra;
+        if ((double)z <= zero_frac * (double)total || C <= 8) {
+            gowner[p] = G;
+            gcols[G] = C;
+            gzeros[G] = z;
+            gtop[G] = (int32_t)p;
+        }
+    }
+    // internal fronts, ordered by their top member (a postorder)
+    std::vector<int32_t> groups;
+    for (int64_t j = 0; j < nfr; ++j)
+        if (gowner[j] == j) groups.push_back((int32_t)j);
+    std::sort(groups.begin(), groups.end(), [&](int32_t a, int32_t b) { return gtop[a] < gtop[b]; });
+    int32_t NJ = (int32_t)groups.size();
+    std::vector<int32_t> gid(nfr, -1);
+    for (int32_t J = 0; J < NJ; ++J) gid[groups[J]] = J;
+    P.owner.resize(nfr);
+    for (int64_t j = 0; j < nfr; ++j) P.owner[j] = gid[gowner[j]];
+    P.fronts.resize(NJ);
+    P.rows.clear();
+    for (int32_t J = 0; J < NJ; ++J) {
+        int32_t G = groups[J], top = gtop[G];
+        Front& F = P.fronts[J];
+        F.c0 = (int32_t)gc0[G];
+        F.nn = (int32_t)gcols[G];
+        F.na = (int32_t)naref(top);
+        F.rowoff = (int32_t)P.rows.size();
+        for (int32_t c = 0; c < F.nn; ++c) P.rows.push_back(F.c0 + c);
+        for (int64_t p = P.sepptr[top]; p < P.sepptr[top + 1]; ++p)
+            P.rows.push_back((int32_t)P.septgt[p]);
+        F.parent = P.pnt[top] < 0 ? -1 : P.owner[P.pnt[top]];
+        if (F.parent >= 0 && F.parent <= J) return "internal tree not topological";
+        P.max_nj = std::max<int64_t>(P.max_nj, F.nn + F.na);
+    }
+    // children (accumulation order: decreasing index, like the reference's
+    // reverse traversal of the children list) and rel maps
+    {
+        std::vector<std::vector<int32_t>> kids(NJ);
+        for (int32_t J = 0; J < NJ; ++J)
+            if (P.fronts[J].parent >= 0) kids[P.fronts[J].parent].push_back(J);
+        P.chd.clear();
+        for (int32_t J = 0; J < NJ; ++J) {
+            P.fronts[J].chd0 = (int32_t)P.chd.size();
+            for (auto it = kids[J].rbegin(); it != kids[J].rend(); ++it) P.chd.push_back(*it);
+            P.fronts[J].chd1 = (int32_t)P.chd.size();
+        }
+        P.rel.clear();
+        for (int32_t K = 0; K < NJ; ++K) {
+            Front& F = P.fronts[K];
+            F.reloff = (int32_t)P.rel.size();
+            if (F.parent < 0) {
+                if (F.na != 0) return "root front with update rows";
+                continue;
+            }
+            const Front& Pf = P.fronts[F.parent];
+            const int32_t* prow = P.rows.data() + Pf.rowoff;
+            int32_t pn = Pf.nn + Pf.na;
+            for (int32_t t = 0; t < F.na; ++t) {
+                int32_t v = P.rows[F.rowoff + F.nn + t];
+                const int32_t* it = std::lower_bound(prow, prow + pn, v);
+                if (it == prow + pn || *it != v) return "separator row missing in parent";
+                P.rel.push_back((int32_t)(it - prow));
+            }
+        }
+    }
+    // copy-out map
+    P.out.resize(nfr);
+    P.septpos.resize(nsep);
+    for (int64_t j = 0; j < nfr; ++j) {
+        OutFront& o = P.out[j];
+        o.Dp = P.Dptr[j];
+        o.Lp = P.Lptr[j];
+        o.J = P.owner[j];
+        o.r0 = (int32_t)P.resptr[j];
+        o.nn = (int32_t)nnref(j);
+        o.na = (int32_t)naref(j);
+        o.sepoff = (int32_t)P.sepptr[j];
+        const Front& F = P.fronts[o.J];
+        const int32_t* frow = P.rows.data() + F.rowoff;
+        int32_t fn = F.nn + F.na;
+        for (int64_t p = P.sepptr[j]; p < P.sepptr[j + 1]; ++p) {
+            int32_t v = (int32_t)P.septgt[p];
+            const int32_t* it = std::lower_bound(frow, frow + fn, v);
+            if (it == frow + fn || *it != v) return "reference separator row missing";
+            P.septpos[p] = (int32_t)(it - frow);
+        }
+    }
+
+    // ---- 2. classification and levels --------------------------------------
+    P.is_small.assign(NJ, 0);
+    std::vector<uint8_t> in_small_region(NJ, 0);
+    std::vector<int32_t> slev(NJ, 0), blev(NJ, 0);
+    for (int32_t J = 0; J < NJ; ++J) {
+        const Front& F = P.fronts[J];
+        P.is_small[J] = (F.nn + F.na) <= SMALL_NJ;
+        bool reg = P.is_small[J];
+        int32_t sl = 0, bl = 0;
+        for (int32_t q = F.chd0; q < F.chd1; ++q) {
+            int32_t K = P.chd[q];
+            if (!in_small_region[K]) reg = false;
+            sl = std::max(sl, slev[K] + 1);
+            if (!in_small_region[K]) bl = std::max(bl, blev[K] + 1);
+        }
+        in_small_region[J] = reg;
+        slev[J] = sl;
+        blev[J] = bl;
+    }
+    P.small_order.clear();
+    int32_t nlev = 0;
+    for (int32_t J = 0; J < NJ; ++J) {
+        if (in_small_region[J]) {
+            P.small_order.push_back(J);
+            P.small_levels = std::max<int64_t>(P.small_levels, slev[J] + 1);
+        } else {
+            nlev = std::max(nlev, blev[J] + 1);
+        }
+    }
+    std::stable_sort(P.small_order.begin(), P.small_order.end(),
+                     [&](int32_t a, int32_t b) { return slev[a] < slev[b]; });
+    P.n_small_region = (int64_t)P.small_order.size();
+    P.levels.assign(nlev, Level());
+    for (int32_t J = 0; J < NJ; ++J) {
+        if (in_small_region[J]) continue;
+        if (P.is_small[J]) P.levels[blev[J]].small_list.push_back(J);
+        else {
+            P.levels[blev[J]].big_list.push_back(J);
+            P.n_big++;
+        }
+    }
+
+    // ---- 3. memory -----------------------------------------------------------
+    int64_t poff = 0;
+    for (int32_t J = 0; J < NJ; ++J) {
+        Front& F = P.fronts[J];
+        F.poff = poff;
+        poff += rup((int64_t)(F.nn + F.na) * F.nn, 32);
+        F.uoff = -1;
+        F.woff = -1;
+    }
+    P.panel_elems = poff;
+    int64_t uoff = 0;
+    for (int32_t J : P.small_order) {
+        Front& F = P.fronts[J];
+        if (F.na > 0) {
+            F.uoff = uoff;
+            uoff += rup((int64_t)F.na * F.na, 32);
+        }
+    }
+    int64_t small_u = uoff;
+    Allocator A;
+    for (int32_t l = 0; l < nlev; ++l) {
+        Level& L = P.levels[l];
+        std::vector<int32_t> here(L.small_list);
+        here.insert(here.end(), L.big_list.begin(), L.big_list.end());
+        for (int32_t J : here) {
+            Front& F = P.fronts[J];
+            if (F.na > 0) F.uoff = small_u + A.alloc(rup((int64_t)F.na * F.na, 32));
+        }
+        for (int32_t J : here) {
+            const Front& F = P.fronts[J];
+            for (int32_t q = F.chd0; q < F.chd1; ++q) {
+                int32_t K = P.chd[q];
+                const Front& Kf = P.fronts[K];
+                if (!in_small_region[K] && Kf.na > 0)
+                    A.release(Kf.uoff - small_u, rup((int64_t)Kf.na * Kf.na, 32));
+            }
+        }
+    }
+    P.u_elems = small_u + A.peak + 32;
+    int64_t woff = 0;
+    for (int32_t J = 0; J < NJ; ++J) {
+        Front& F = P.fronts[J];
+        if (!P.is_small[J] && !in_small_region[J]) {
+            F.woff = woff;
+            woff += rup((int64_t)F.nn * NB, 32);
+        }
+    }
+    P.w_elems = woff + 32;
+    P.mvoff1.resize(NJ);
+    int64_t mv = 0;
+    for (int32_t J = 0; J < NJ; ++J) {
+        P.mvoff1[J] = mv;
+        mv += P.fronts[J].na;
+    }
+    P.mv_elems1 = mv + 1;
+
+    // ---- 4. task lists ---------------------------------------------------------
+    P.level_small.clear();
+    P.level_big.clear();
+    P.ea.clear();
+    P.ea_src.clear();
+    P.gemm.clear();
+    P.diag.clear();
+    P.stask.clear();
+    auto add_gemm = [&](int8_t mode, int8_t aA, int64_t ao, int32_t lda, int8_t aB, int64_t bo,
+                        int32_t ldb, int8_t aC, int64_t co, int32_t ldc, int32_t m, int32_t nn_,
+                        int32_t k, int32_t scol, int8_t lower) {
+        GemmTask t{};
+        t.a_off = ao;
+        t.b_off = bo;
+        t.c_off = co;
+        t.lda = lda;
+        t.ldb = ldb;
+        t.ldc = ldc;
+        t.m = m;
+        t.n = nn_;
+        t.k = k;
+        t.s_col = scol;
+        t.arrA = aA;
+        t.arrB = aB;
+        t.arrC = aC;
+        t.mode = mode;
+        t.lower = lower;
+        P.gemm.push_back(t);
+        P.gemm_flops += lower ? (double)m * (m + 1) * k : 2.0 * m * nn_ * k;
+    };
+    for (int32_t l = 0; l < nlev; ++l) {
+        Level& L = P.levels[l];
+        L.smallL0 = (int32_t)P.level_small.size();
+        P.level_small.insert(P.level_small.end(), L.small_list.begin(), L.small_list.end());
+        L.smallL1 = (int32_t)P.level_small.size();
+        L.bigL0 = (int32_t)P.level_big.size();
+        P.level_big.insert(P.level_big.end(), L.big_list.begin(), L.big_list.end());
+        L.bigL1 = (int32_t)P.level_big.size();
+        // extend-add tasks
+        L.ea0 = (int32_t)P.ea.size();
+        for (int32_t J : L.big_list) {
+            const Front& F = P.fronts[J];
+            int32_t nj = F.nn + F.na;
+            std::vector<std::vector<int32_t>> src(nj);
+            for (int32_t q = F.chd0; q < F.chd1; ++q) {
+                int32_t K = P.chd[q];
+                const Front& Kf = P.fronts[K];
+                for (int32_t t = 0; t < Kf.na; ++t) {
+                    int32_t pc = P.rel[Kf.reloff + t];
+                    src[pc].push_back(K);
+                    src[pc].push_back(t);
+                }
+            }
+            for (int32_t pc = 0; pc < nj; ++pc) {
+                if (pc < F.nn && src[pc].empty()) continue;
+                EaTask e{};
+                e.front = J;
+                e.col = pc;
+                e.row0 = pc;
+                e.row1 = nj;
+                e.src0 = (int32_t)(P.ea_src.size() / 2);
+                P.ea_src.insert(P.ea_src.end(), src[pc].begin(), src[pc].end());
+                e.src1 = (int32_t)(P.ea_src.size() / 2);
+                P.ea.push_back(e);
+            }
+        }
+        L.ea1 = (int32_t)P.ea.size();
+        // panel steps
+        int32_t maxp = 0;
+        for (int32_t J : L.big_list) maxp = std::max(maxp, (P.fronts[J].nn + NB - 1) / NB);
+        L.steps.assign(maxp, Step());
+        for (int32_t s = 0; s < maxp; ++s) {
+            Step& st = L.steps[s];
+            st.diag0 = (int32_t)P.diag.size();
+            for (int32_t J : L.big_list) {
+                const Front& F = P.fronts[J];
+                int32_t k0 = s * NB;
+                if (k0 >= F.nn) continue;
+                DiagTask d{};
+                d.front = J;
+                d.k0 = k0;
+                d.kb = std::min(NB, F.nn - k0);
+                P.diag.push_back(d);
+            }
+            st.diag1 = (int32_t)P.diag.size();
+            st.trsm0 = (int32_t)P.gemm.size();
+            for (int32_t J : L.big_list) {
+                const Front& F = P.fronts[J];
+                int32_t nj = F.nn + F.na, k0 = s * NB;
+                if (k0 >= F.nn) continue;
+                int32_t kb = std::min(NB, F.nn - k0);
+                for (int32_t r = k0 + kb; r < nj; r += NB) {
+                    int32_t m = std::min(NB, nj - r);
+                    int64_t ao = F.poff + r + (int64_t)k0 * nj;
+                    add_gemm(G_TRSM, ARR_PANEL, ao, nj, ARR_W, F.woff + k0, F.nn, ARR_PANEL, ao, nj,
+                             m, kb, kb, F.c0 + k0, 0);
+                }
+            }
+            st.trsm1 = (int32_t)P.gemm.size();
+            st.trail0 = (int32_t)P.gemm.size();
+            for (int32_t J : L.big_list) {
+                const Front& F = P.fronts[J];
+                int32_t nj = F.nn + F.na, k0 = s * NB;
+                if (k0 >= F.nn) continue;
+                int32_t kb = std::min(NB, F.nn - k0);
+                for (int32_t c = k0 + kb; c < F.nn; c += NB) {
+                    int32_t nc = std::min(NB, F.nn - c);
+                    for (int32_t r = c; r < nj; r += NB) {
+                        int32_t m = std::min(NB, nj - r);
+                        add_gemm(G_SUB, ARR_PANEL, F.poff + r + (int64_t)k0 * nj, nj, ARR_PANEL,
+                                 F.poff + c + (int64_t)k0 * nj, nj, ARR_PANEL,
+                                 F.poff + r + (int64_t)c * nj, nj, m, nc, kb, F.c0 + k0,
+                                 r == c ? 1 : 0);
+                    }
+                }
+            }
+            st.trail1 = (int32_t)P.gemm.size();
+        }
+        // update matrices
+        L.syrk0 = (int32_t)P.gemm.size();
+        for (int32_t J : L.big_list) {
+            const Front& F = P.fronts[J];
+            int32_t nj = F.nn + F.na;
+            if (F.na == 0) continue;
+            for (int32_t c = 0; c < F.na; c += NB) {
+                int32_t nc = std::min(NB, F.na - c);
+                for (int32_t r = c; r < F.na; r += NB) {
+                    int32_t m = std::min(NB, F.na - r);
+                    add_gemm(G_SUB, ARR_PANEL, F.poff + F.nn + r, nj, ARR_PANEL, F.poff + F.nn + c,
+                             nj, ARR_U, F.uoff + r + (int64_t)c * F.na, F.na, m, nc, F.nn, F.c0,
+                             r == c ? 1 : 0);
+                }
+            }
+        }
+        L.syrk1 = (int32_t)P.gemm.size();
+
+        // solve tasks: forward gather (row chunks of EA_CHUNK)
+        L.sea0 = (int32_t)P.stask.size();
+        for (int32_t J : L.big_list) {
+            const Front& F = P.fronts[J];
+            int32_t nj = F.nn + F.na;
+            for (int32_t r = 0; r < nj; r += EA_CHUNK)
+                P.stask.push_back(SolveTask{J, 0, 0, r, std::min(nj, r + EA_CHUNK)});
+        }
+        L.sea1 = (int32_t)P.stask.size();
+        L.fsteps.assign(maxp, SolveStep());
+        for (int32_t s = 0; s < maxp; ++s) {
+            SolveStep& st = L.fsteps[s];
+            st.t0 = (int32_t)P.stask.size();
+            for (int32_t J : L.big_list) {
+                const Front& F = P.fronts[J];
+                int32_t k0 = s * NB;
+                if (k0 >= F.nn) continue;
+                int32_t kb = std::min(NB, F.nn - k0);
+                P.stask.push_back(SolveTask{J, k0, kb, k0, k0 + kb});
+            }
+            st.t1 = (int32_t)P.stask.size();
+            st.u0 = (int32_t)P.stask.size();
+            for (int32_t J : L.big_list) {
+                const Front& F = P.fronts[J];
+                int32_t nj = F.nn + F.na, k0 = s * NB;
+                if (k0 >= F.nn) continue;
+                int32_t kb = std::min(NB, F.nn - k0);
+                for (int32_t r = k0 + kb; r < nj; r += NB)
+                    P.stask.push_back(SolveTask{J, k0, kb, r, std::min(nj, r + NB)});
+            }
+            st.u1 = (int32_t)P.stask.size();
+        }
+        // backward: L' x_sep gather over column chunks, then panels in reverse
+        L.bg0 = (int32_t)P.stask.size();
+        for (int32_t J : L.big_list) {
+            const Front& F = P.fronts[J];
+            for (int32_t c = 0; c < F.nn; c += NB)
+                P.stask.push_back(SolveTask{J, 0, 0, c, std::min(F.nn, c + NB)});
+        }
+        L.bg1 = (int32_t)P.stask.size();
+        L.bsteps.clear();
+        for (int32_t s = maxp - 1; s >= 0; --s) {
+            SolveStep st{};
+            st.t0 = (int32_t)P.stask.size();
+            for (int32_t J : L.big_list) {
+                const Front& F = P.fronts[J];
+                int32_t k0 = s * NB;
+                if (k0 >= F.nn) continue;
+                int32_t kb = std::min(NB, F.nn - k0);
+                P.stask.push_back(SolveTask{J, k0, kb, k0, k0 + kb});
+            }
+            st.t1 = (int32_t)P.stask.size();
+            st.u0 = (int32_t)P.stask.size();
+            for (int32_t J : L.big_list) {
+                const Front& F = P.fronts[J];
+                int32_t k0 = s * NB;
+                if (k0 >= F.nn) continue;
+                int32_t kb = std::min(NB, F.nn - k0);
+                for (int32_t c = 0; c < k0; c += NB)
+                    P.stask.push_back(SolveTask{J, k0, kb, c, std::min(k0, c + NB)});
+            }
+            st.u1 = (int32_t)P.stask.size();
+            L.bsteps.push_back(st);
+        }
+    }
+
+    // ---- bounds validation of every task against the pools ----------------------
+    {
+        auto pool = [&](int8_t a) {
+            return a == ARR_PANEL ? P.panel_elems : (a == ARR_U ? P.u_elems : P.w_elems);
+        };
+        auto inside = [&](int8_t a, int64_t off, int32_t rows, int32_t cols, int32_t ld) {
+            if (rows <= 0 || cols <= 0) return true;
+            if (off < 0 || ld < rows) return false;
+            return off + (rows - 1) + (int64_t)(cols - 1) * ld < pool(a);
+        };
+        for (const GemmTask& t : P.gemm) {
+            if (t.m <= 0 || t.n <= 0 || t.k <= 0 || t.m > NB || t.n > NB) return "gemm tile shape";
+            if (!inside(t.arrA, t.a_off, t.m, t.k, t.lda) || !inside(t.arrB, t.b_off, t.n, t.k, t.ldb) ||
+                !inside(t.arrC, t.c_off, t.m, t.n, t.ldc))
+                return "gemm task out of bounds";
+            if (t.s_col < 0 || t.s_col + t.k > n || (t.mode == G_TRSM && t.s_col + t.n > n))
+                return "gemm scaling out of range";
+        }
+        for (const DiagTask& t : P.diag) {
+            const Front& F = P.fronts[t.front];
+            if (t.kb <= 0 || t.kb > NB || t.k0 + t.kb > F.nn || F.woff < 0) return "diag task";
+            if (!inside(ARR_W, F.woff + t.k0, t.kb, t.kb, F.nn)) return "diag W out of bounds";
+        }
+        for (const EaTask& e : P.ea) {
+            const Front& F = P.fronts[e.front];
+            int32_t nj = F.nn + F.na;
+            if (e.col < 0 || e.col >= nj || e.row0 != e.col || e.row1 != nj) return "ea task";
+            if (e.col >= F.nn && (F.uoff < 0 || F.uoff + (int64_t)F.na * F.na > P.u_elems))
+                return "ea U out of bounds";
+            for (int32_t s = e.src0; s < e.src1; ++s) {
+                int32_t K = P.ea_src[2 * s], tc = P.ea_src[2 * s + 1];
+                const Front& Kf = P.fronts[K];
+                if (Kf.parent != e.front || tc < 0 || tc >= Kf.na) return "ea source";
+                if (Kf.uoff < 0 || Kf.uoff + (int64_t)Kf.na * Kf.na > P.u_elems) return "ea child U";
+            }
+        }
+        for (size_t J = 0; J < P.fronts.size(); ++J) {
+            const Front& F = P.fronts[J];
+            int32_t nj = F.nn + F.na;
+            if (F.poff + (int64_t)nj * F.nn > P.panel_elems) return "panel out of bounds";
+            if (F.na > 0 && (F.uoff < 0 || F.uoff + (int64_t)F.na * F.na > P.u_elems)) return "U slot";
+            if (P.is_small[J] && nj > SMALL_NJ) return "small front too big";
+            for (int32_t t = 0; t < F.na && F.parent >= 0; ++t) {
+                int32_t r = P.rel[F.reloff + t];
+                const Front& Pf = P.fronts[F.parent];
+                if (r < 0 || r >= Pf.nn + Pf.na) return "rel out of range";
+            }
+        }
+        for (const SolveTask& t : P.stask) {
+            const Front& F = P.fronts[t.front];
+            int32_t nj = F.nn + F.na;
+            if (t.r0 < 0 || t.r1 > nj || t.r1 <= t.r0 || t.r1 - t.r0 > EA_CHUNK) return "solve task";
+            if (t.kb > 0 && (t.k0 + t.kb > F.nn || F.woff < 0 || t.r1 - t.r0 > NB)) return "solve panel";
+        }
+    }
+
+    // ---- statistics ------------------------------------------------------------
+    for (int64_t j = 0; j < nfr; ++j) {
+        double a = (double)nnref(j), b = (double)naref(j);
+        P.ref_flops += a * a * a / 3.0 + b * a * a + b * b * a;
+    }
+    for (int32_t J = 0; J < NJ; ++J) {
+        double a = P.fronts[J].nn, b = P.fronts[J].na;
+        P.engine_flops += a * a * a / 3.0 + b * a * a + b * b * a;
+    }
+    return std::string();
+}
+
+}  // namespace mf
+
+// Host-only entry (built into libmfsym.so): runs the planner with its bounds
+// validation, without any device, and reports a few sizes.
+extern "C" int mf_plan_check(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                             const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                             const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
+                             int64_t* stats, int64_t nstats, char* msg, int64_t msglen) {
+    mf::HostPlan P;
+    std::string err = mf::build_plan(P, n, nfr, resptr, sepptr, septgt, reltgt, chdptr, chdtgt,
+                                     Dptr, Lptr, 0.05);
+    if (msg && msglen > 0) {
+        size_t k = std::min<size_t>(err.size(), (size_t)msglen - 1);
+        std::copy(err.begin(), err.begin() + k, msg);
+        msg[k] = 0;
+    }
+    int64_t v[] = {(int64_t)P.fronts.size(), P.n_small_region, P.n_big, (int64_t)P.levels.size(),
+                   P.panel_elems, P.u_elems, P.w_elems, (int64_t)P.gemm.size(),
+                   (int64_t)P.ea.size(), (int64_t)P.stask.size(), (int64_t)P.ref_flops,
+                   (int64_t)P.engine_flops, P.max_nj};
+    for (int64_t i = 0; stats && i < nstats && i < (int64_t)(sizeof(v) / sizeof(v[0])); ++i)
+        stats[i] = v[i];
+    return err.empty() ? 0 : -1;
+}

@@ -1,0 +1,81 @@
+// mf_plan.h — host-side plan of libmfhip.so: internal fronts, level
+// schedule, memory layout and the static task lists of every launch.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mf_internal.h"
+
+namespace mf {
+
+struct Step {            // one panel step of a big level
+    int32_t diag0, diag1;    // DiagTask range
+    int32_t trsm0, trsm1;    // GemmTask range (panel solve)
+    int32_t trail0, trail1;  // GemmTask range (trailing update inside the panel)
+};
+
+struct SolveStep {       // one panel step of the big-front solves
+    int32_t t0, t1;          // SolveTask range (diag block rows)
+    int32_t u0, u1;          // SolveTask range (update rows / columns)
+};
+
+// Row-block task of the big-front solves.
+struct SolveTask {
+    int32_t front;
+    int32_t k0, kb;      // panel (local columns)
+    int32_t r0, r1;      // local rows (forward) or local columns (backward)
+};
+
+struct Level {
+    std::vector<int32_t> small_list;   // small fronts living in the big region
+    std::vector<int32_t> big_list;
+    int32_t smallL0 = 0, smallL1 = 0;  // range in level_small (device)
+    int32_t bigL0 = 0, bigL1 = 0;      // range in level_big (device)
+    int32_t ea0 = 0, ea1 = 0;          // EaTask range
+    std::vector<Step> steps;
+    int32_t syrk0 = 0, syrk1 = 0;
+    // solves
+    int32_t sea0 = 0, sea1 = 0;        // forward gather tasks (SolveTask, r-chunks)
+    std::vector<SolveStep> fsteps;     // forward panel steps
+    int32_t bg0 = 0, bg1 = 0;          // backward L' x_sep tasks
+    std::vector<SolveStep> bsteps;     // backward panel steps (already reversed)
+};
+
+struct HostPlan {
+    int64_t n = 0, nfr = 0, ndz = 0, nlz = 0;
+    // reference arrays, 0-based
+    std::vector<int64_t> resptr, sepptr, septgt, Dptr, Lptr;
+    std::vector<int32_t> pnt;          // reference front parent, -1 root
+    // internal fronts
+    std::vector<Front> fronts;
+    std::vector<int32_t> rows, chd, rel;
+    std::vector<int32_t> owner;        // reference front -> internal front
+    std::vector<uint8_t> is_small;     // nJ <= SMALL_NJ
+    int64_t panel_elems = 0, u_elems = 0, w_elems = 0, mv_elems1 = 0;
+    // schedule
+    std::vector<int32_t> small_order;  // small region, ticket order (children first)
+    std::vector<int32_t> level_small, level_big;
+    std::vector<Level> levels;
+    std::vector<EaTask> ea;
+    std::vector<int32_t> ea_src;       // pairs (child, child column)
+    std::vector<GemmTask> gemm;
+    std::vector<DiagTask> diag;
+    std::vector<SolveTask> stask;
+    std::vector<int64_t> mvoff1;       // per internal front: offset of Mv (na x 1)
+    // copy-out
+    std::vector<OutFront> out;
+    std::vector<int32_t> septpos;
+    // statistics
+    double ref_flops = 0, engine_flops = 0, gemm_flops = 0;
+    int64_t n_small_region = 0, n_big = 0, max_nj = 0, small_levels = 0;
+};
+
+// Builds the plan; returns an empty string on success, else an error message.
+std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* resptr,
+                       const int64_t* sepptr, const int64_t* septgt, const int64_t* reltgt,
+                       const int64_t* chdptr, const int64_t* chdtgt, const int64_t* Dptr,
+                       const int64_t* Lptr, double zero_frac);
+
+}  // namespace mf

@@ -1,0 +1,326 @@
+"""Host mirror of the reference's factorization API for the multifrontal path.
+
+Names follow src/Multifrontal.jl (Python has no ``!``: the in-place verbs
+carry a trailing underscore):
+
+  ``ChordalCholesky(A[, P, S])`` / ``ChordalLDLt(A[, P, S])``
+        chordal_factorization.jl:45-121,197-244: symbolic analysis, then the
+        values of A are assembled (``copyto!``, chordal_triangular.jl:852-893)
+        on the device from ``flatindices`` (chordal_symbolic.jl:595-752).
+  ``cholesky_(F; check, reg)``   cholesky!  cholesky.jl:80-101
+  ``ldlt_(F; signs, reg, check)`` ldlt!     cholesky.jl:181-203
+  ``ldiv_(F, B)`` / ``solve(F, B)`` (= ``F \\ B``)  ldiv! divide.jl:201-206
+  ``issuccess(F)``               abstract_factorization.jl:100-102
+  ``chol_impl(S, Dval, Lval, ...)`` / ``div_impl(...)``: the flat-array seam
+        (cholesky.jl:361-392, divide.jl:417-450) on host arrays.
+
+Every numeric call goes to libmfhip.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _native
+from .symbolic import ChordalSymbolic, Permutation, flatindices, symbolic
+
+MF_FP64, MF_FP32 = 0, 1
+MF_CHOLESKY, MF_LDLT = 0, 1
+MF_REG_NONE, MF_REG_DYNAMIC = 0, 1
+
+STAT_NAMES = ["n", "nfr", "internal_fronts", "small_region", "big_fronts", "big_levels",
+              "small_levels", "max_nj", "panel_elems", "u_elems", "w_elems", "gemm_tasks",
+              "ea_tasks", "diag_tasks", "ref_flops", "engine_flops", "gemm_flops", "ndz", "nlz"]
+KC_NAMES = ["small", "ea", "diag", "gemm", "copyout", "solve", "assemble"]
+
+
+class PosDefException(Exception):
+    """Mirror of LinearAlgebra.PosDefException (utils.jl:778-779)."""
+
+    def __init__(self, info):
+        super().__init__(f"matrix is not positive definite; factorization failed at column {info}")
+        self.info = info
+
+
+class SingularException(Exception):
+    """Mirror of LinearAlgebra.SingularException (utils.jl:780-781)."""
+
+    def __init__(self, info):
+        super().__init__(f"matrix is singular; zero pivot at column {info}")
+        self.info = info
+
+
+class MFError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        msg = _native.hip().mf_last_error()
+        raise MFError(f"libmfhip error {rc}: {msg.decode() if msg else ''}")
+
+
+def checkinfo(info, diag):
+    """checkinfo (utils.jl:775-785)."""
+    if info < 0:
+        raise ValueError(info)
+    if info > 0:
+        raise (PosDefException if diag == MF_CHOLESKY else SingularException)(info)
+
+
+class DynamicRegularization:
+    """DynamicRegularization (dynamic_regularization.jl:80-115): pivots with
+    sign*Djj < epsilon are replaced by delta*sign; negative parameters take the
+    defaults cbrt(eps(T)) and sqrt(eps(T))."""
+
+    def __init__(self, delta=-1.0, epsilon=-1.0):
+        self.delta = delta
+        self.epsilon = epsilon
+
+    def initialize(self, eps):
+        d = self.delta if self.delta >= 0 else eps ** (1.0 / 3.0)
+        e = self.epsilon if self.epsilon >= 0 else math.sqrt(eps)
+        return d, e
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DevicePlan:
+    """A ChordalSymbolic uploaded once (mf_plan_create)."""
+
+    def __init__(self, S: ChordalSymbolic):
+        lib = _native.hip()
+        h = ctypes.c_void_p()
+        arrs = [S.resptr, S.sepptr, S.septgt, S.reltgt, S.chdptr, S.chdtgt, S.Dptr, S.Lptr]
+        self._keep = [np.ascontiguousarray(a, dtype=np.int64) for a in arrs]
+        _check(lib.mf_plan_create(S.n, S.nfr, *[a.ctypes.data_as(_native.i64p) for a in self._keep],
+                                  0, ctypes.byref(h)))
+        self.handle = h
+        self.S = S
+
+    def stats(self):
+        out = np.zeros(len(STAT_NAMES), dtype=np.int64)
+        _check(_native.hip().mf_plan_stats(self.handle, out.ctypes.data_as(_native.i64p), len(out)))
+        return dict(zip(STAT_NAMES, out.tolist()))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                _native.hip().mf_plan_destroy(self.handle)
+            except Exception:
+                pass
+            self.handle = None
+
+
+def device_plan(S: ChordalSymbolic) -> DevicePlan:
+    dp = getattr(S, "_devplan", None)
+    if dp is None:
+        dp = DevicePlan(S)
+        S._devplan = dp
+    return dp
+
+
+class DeviceFactor:
+    """Device storage of one numeric factorization (mf_factor_create)."""
+
+    def __init__(self, plan: DevicePlan, precision=MF_FP64):
+        h = ctypes.c_void_p()
+        _check(_native.hip().mf_factor_create(plan.handle, precision, ctypes.byref(h)))
+        self.handle = h
+        self.plan = plan
+        self.precision = precision
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                _native.hip().mf_factor_destroy(self.handle)
+            except Exception:
+                pass
+            self.handle = None
+
+
+class ChordalFactorization:
+    """ChordalFactorization{DIAG, :L} (chordal_factorization.jl:1-21) whose
+    values live on the GPU."""
+
+    DIAG = MF_CHOLESKY
+
+    def __init__(self, A, P: Permutation | None = None, S: ChordalSymbolic | None = None, *,
+                 alg=None, snd="maximal", precision=MF_FP64):
+        A = sp.csc_matrix(A, dtype=np.float64)
+        A.sort_indices()
+        if P is None or S is None:
+            P, S = symbolic(A, alg=alg, snd=snd)
+        self.P, self.S = P, S
+        self.perm, self.invp = P.perm, P.invp
+        self.n = S.n
+        self.info = 0
+        self._plan = device_plan(S)
+        self._f = DeviceFactor(self._plan, precision)
+        self._pattern = None
+        self.copyto_(A)
+
+    # copyto!(F, A) with a pattern reuse fast path (flatindices + setflatindex!)
+    def copyto_(self, A):
+        A = sp.csc_matrix(A, dtype=np.float64)
+        A.sort_indices()
+        key = (A.shape, A.indptr.tobytes(), A.indices.tobytes())
+        lib = _native.hip()
+        if self._pattern != key:
+            flat = flatindices(self.S, A)
+            _check(lib.mf_factor_set_pattern(self._f.handle, len(flat),
+                                             flat.ctypes.data_as(_native.i64p)))
+            self._pattern = key
+        vals = np.ascontiguousarray(A.data, dtype=np.float64)
+        _check(lib.mf_factor_refresh(self._f.handle, _ptr(vals), 0))
+        self.info = 0
+        return self
+
+    def set_values(self, Dval, Lval):
+        """Load Dval/Lval in the reference layout (e.g. an assembled copyto! result)."""
+        Dval = np.ascontiguousarray(Dval, dtype=np.float64)
+        Lval = np.ascontiguousarray(Lval, dtype=np.float64)
+        _check(_native.hip().mf_factor_set_values(self._f.handle, _ptr(Dval), _ptr(Lval), 0))
+        return self
+
+    @property
+    def Dval(self):
+        out = np.empty(self.S.ndz)
+        _check(_native.hip().mf_factor_get(self._f.handle, _ptr(out), None, None, 0))
+        return out
+
+    @property
+    def Lval(self):
+        out = np.empty(self.S.nlz)
+        _check(_native.hip().mf_factor_get(self._f.handle, None, _ptr(out), None, 0))
+        return out
+
+    @property
+    def d(self):
+        out = np.empty(self.n)
+        _check(_native.hip().mf_factor_get(self._f.handle, None, None, _ptr(out), 0))
+        return out
+
+    def stats(self):
+        return self._plan.stats()
+
+
+class ChordalCholesky(ChordalFactorization):
+    DIAG = MF_CHOLESKY
+
+
+class ChordalLDLt(ChordalFactorization):
+    DIAG = MF_LDLT
+
+
+def _factorize(F, diag, signs, reg, check):
+    """factorize! levels 1-3 (cholesky.jl:233-315)."""
+    if signs is not None:
+        s = np.asarray(signs, dtype=np.float64)
+        if s.shape != (F.n,):
+            raise ValueError("signs has the wrong length")
+        s = np.ascontiguousarray(s[F.perm - 1])  # permuteto (utils.jl:1-9)
+        bad = ~((s == 0) | (s == 1) | (s == -1)) if reg is None else ~((s == 1) | (s == -1))
+        if bad.any():  # checksigns (no_regularization.jl:7-25)
+            raise ValueError("invalid signs")
+        sp_ = _ptr(s)
+    else:
+        sp_ = None
+    kind, delta, eps = MF_REG_NONE, 0.0, 0.0
+    if reg is not None:
+        kind = MF_REG_DYNAMIC
+        delta, eps = reg.initialize(np.finfo(np.float64).eps)
+    info = ctypes.c_int64(0)
+    _check(_native.hip().mf_factorize(F._f.handle, diag, sp_, kind, delta, eps, ctypes.byref(info)))
+    info = int(info.value)
+    if check:
+        checkinfo(info, diag)
+    F.info = 0 if info == 0 else int(F.perm[info - 1])
+    return F
+
+
+def cholesky_(F: ChordalFactorization, check=True, reg=None):
+    """cholesky!(F; check, reg) with signs = Ones."""
+    if F.DIAG != MF_CHOLESKY:
+        raise TypeError("cholesky_ needs a ChordalCholesky")
+    return _factorize(F, MF_CHOLESKY, None, reg, check)
+
+
+def ldlt_(F: ChordalFactorization, signs=None, reg=None, check=True):
+    """ldlt!(F; signs, reg, check); default signs = Zeros (no sign check)."""
+    if F.DIAG != MF_LDLT:
+        raise TypeError("ldlt_ needs a ChordalLDLt")
+    return _factorize(F, MF_LDLT, signs, reg, check)
+
+
+def issuccess(F):
+    return F.info == 0
+
+
+def ldiv_(F: ChordalFactorization, B):
+    """ldiv!(F, B): B <- A \\ B in place (B float64, n or n x nrhs)."""
+    if not isinstance(B, np.ndarray) or B.dtype != np.float64:
+        raise TypeError("B must be a float64 numpy array")
+    vec = B.ndim == 1
+    M = B.reshape(-1, 1) if vec else B
+    if M.shape[0] != F.n:
+        raise ValueError("dimension mismatch")
+    W = np.asfortranarray(M)
+    perm = np.ascontiguousarray(F.perm, dtype=np.int64)
+    _check(_native.hip().mf_solve(F._f.handle, perm.ctypes.data_as(_native.i64p), _ptr(W),
+                                  W.shape[1], F.n, 0))
+    if W is not M:
+        M[...] = W
+    return B
+
+
+def solve(F: ChordalFactorization, B):
+    """F \\ B."""
+    X = np.array(B, dtype=np.float64, copy=True)
+    return ldiv_(F, X)
+
+
+def chol_impl(S: ChordalSymbolic, Dval, Lval, diag=MF_CHOLESKY, signs=None, reg=None):
+    """chol_impl! on host arrays through mf_chol_impl: returns (Dval, Lval, d, info)."""
+    Dval = np.array(Dval, dtype=np.float64, copy=True)
+    Lval = np.array(Lval, dtype=np.float64, copy=True)
+    d = np.ones(S.n) if diag == MF_CHOLESKY else np.zeros(S.n)
+    sp_ = None
+    if signs is not None:
+        signs = np.ascontiguousarray(signs, dtype=np.float64)
+        sp_ = _ptr(signs)
+    kind, delta, eps = MF_REG_NONE, 0.0, 0.0
+    if reg is not None:
+        kind = MF_REG_DYNAMIC
+        delta, eps = reg.initialize(np.finfo(np.float64).eps)
+    info = ctypes.c_int64(0)
+    plan = device_plan(S)
+    _check(_native.hip().mf_chol_impl(plan.handle, Dval.ctypes.data_as(_native.f64p),
+                                      Lval.ctypes.data_as(_native.f64p),
+                                      d.ctypes.data_as(_native.f64p), sp_, diag, kind, delta, eps,
+                                      ctypes.byref(info)))
+    return Dval, Lval, d, int(info.value)
+
+
+def div_impl(S: ChordalSymbolic, Dval, Lval, d, C, diag=MF_CHOLESKY):
+    """Natural solve C <- L^-T [D^-1] L^-1 C on host arrays (mf_div_impl)."""
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    vec = C.ndim == 1
+    if vec:
+        C = C.reshape(-1, 1, order="F")
+    Dval = np.ascontiguousarray(Dval, dtype=np.float64)
+    Lval = np.ascontiguousarray(Lval, dtype=np.float64)
+    dp = None
+    if d is not None:
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        dp = _ptr(d)
+    plan = device_plan(S)
+    _check(_native.hip().mf_div_impl(plan.handle, Dval.ctypes.data_as(_native.f64p),
+                                     Lval.ctypes.data_as(_native.f64p), dp,
+                                     C.ctypes.data_as(_native.f64p), C.shape[1], S.n, diag))
+    return C[:, 0] if vec else C

@@ -1,0 +1,155 @@
+/*
+ * mfhip.h — C-ABI of libmfhip.so, the MI355X (gfx950) numeric engine for the
+ * multifrontal Cholesky / LDLt path of CliqueTrees.jl.
+ *
+ * The boundary is the reference's own flat-array seam: `chol_impl!(Mptr,
+ * Mval, Dptr, Dval, Lptr, Lval, d, Fval, res, rel, chd, uplo, S, R, diag)`
+ * (src/Multifrontal.jl/src/cholesky.jl:361-377) and `div_impl!(B, Mptr, Mval,
+ * Dptr, Dval, Lptr, Lval, Fval, res, rel, chd, tA, tB, uplo, side, diag)`
+ * (src/Multifrontal.jl/src/divide.jl:417-435).  Every index array is passed
+ * exactly as the reference stores it (Int64, 1-based, CSR pointers starting
+ * at 1) and every value array in the reference's layout (Dval: per front an
+ * nn x nn column-major block at Dptr[j]; Lval: an na x nn block at Lptr[j];
+ * uplo = :L).  Paths in this header are relative to the CliqueTrees.jl tree.
+ *
+ * Error convention (cholesky.jl:384-388, utils.jl:775-785): functions return
+ * MF_OK (0) or a negative status and never throw or abort; a numeric failure
+ * is reported in *info as the first failing column in permuted order
+ * (1-based), 0 on success — the value chol_impl! returns.  A plan and its
+ * factors are not re-entrant: use one host thread per factor object.
+ */
+#ifndef MFHIP_H
+#define MFHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mf_plan mf_plan;
+typedef struct mf_factor mf_factor;
+
+enum {
+    MF_OK = 0,
+    MF_EARG = -1,      /* invalid argument                                  */
+    MF_EHIP = -2,      /* HIP runtime error (message in mf_last_error())    */
+    MF_ENOMEM = -3,    /* device allocation failed                          */
+    MF_ESTATE = -4     /* call out of order (e.g. solve before factorize)   */
+};
+
+enum { MF_FP64 = 0, MF_FP32 = 1 };          /* precision of the fronts       */
+enum { MF_CHOLESKY = 0, MF_LDLT = 1 };      /* diag = :N / :U                */
+enum { MF_REG_NONE = 0, MF_REG_DYNAMIC = 1 };
+
+/* Message of the last failing call on this thread. */
+const char* mf_last_error(void);
+
+/*
+ * Upload a ChordalSymbolic (src/Multifrontal.jl/src/chordal_symbolic.jl:15-28)
+ * once: res = (resptr), sep = (sepptr, septgt), rel = (sepptr, reltgt),
+ * chd = (chdptr [nfr+2], chdtgt [nfr]), Dptr, Lptr.  Builds the internal
+ * front tree, the level schedule and all task lists; replaces the workspace
+ * sizing of FactorizationWorkspace (cholesky.jl:15-24) and DivisionWorkspace
+ * (divide.jl:13-18).  flags: reserved, pass 0.
+ */
+int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                   const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                   const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
+                   int64_t flags, mf_plan** out);
+int mf_plan_destroy(mf_plan* plan);
+
+/* Plan statistics (see MF_STAT_* in the package's Python mirror). */
+int mf_plan_stats(const mf_plan* plan, int64_t* stats, int64_t nstats);
+
+/*
+ * Device storage for one numeric factorization of a plan: the fields d,
+ * Dval, Lval of ChordalFactorization (chordal_factorization.jl:1-21) plus the
+ * engine's internal panels and update matrices.  precision: MF_FP64 or
+ * MF_FP32 (fronts computed in FP32 with FP32 accumulation).
+ */
+int mf_factor_create(mf_plan* plan, int precision, mf_factor** out);
+int mf_factor_destroy(mf_factor* f);
+
+/* All work of a factor object is ordered on this stream (a hipStream_t;
+ * NULL restores the factor's own stream). */
+int mf_factor_set_stream(mf_factor* f, void* stream);
+int mf_factor_sync(mf_factor* f);
+
+/*
+ * Values of P A P' in the reference layout, as left by copyto!(F, A)
+ * (chordal_triangular.jl:852-893).  device != 0: the pointers are device
+ * pointers (no PCIe transfer).
+ */
+int mf_factor_set_values(mf_factor* f, const double* Dval, const double* Lval, int device);
+
+/*
+ * Register flatindices(F, A) (chordal_factorization.jl:322-328) for a
+ * pattern: flat[k] is the 1-based slot of nonzero k in [Dval; Lval] or 0.
+ * mf_factor_refresh then does fill!(F, 0) followed by setflatindex! of every
+ * nonzero (chordal_triangular.jl:837-849) on the device.
+ */
+int mf_factor_set_pattern(mf_factor* f, int64_t nnz, const int64_t* flat);
+int mf_factor_refresh(mf_factor* f, const double* nzval, int device);
+
+/*
+ * Numeric factorization: cholesky!/ldlt! with NoPivot (cholesky.jl:80-101,
+ * 181-203, 304-315), replacing chol_impl! (cholesky.jl:361-392).
+ * diag: MF_CHOLESKY (:N) or MF_LDLT (:U).  signs: n pivot signs in permuted
+ * order (NULL = Ones for Cholesky, Zeros for LDLt), host pointer.
+ * reg_kind: MF_REG_NONE (NoRegularization, no_regularization.jl:27-42) or
+ * MF_REG_DYNAMIC (DynamicRegularization(delta, eps),
+ * dynamic_regularization.jl:117-132).  *info as chol_impl!.  The call
+ * returns once the work is enqueued and *info is known (it synchronizes).
+ */
+int mf_factorize(mf_factor* f, int diag, const double* signs, int reg_kind, double delta,
+                 double eps, int64_t* info);
+
+/* Same, without synchronizing: *info is not produced; read it later with
+ * mf_factor_info.  Used by the serving loop and by graph capture. */
+int mf_factorize_async(mf_factor* f, int diag, const double* signs_dev, int reg_kind,
+                       double delta, double eps);
+int mf_factor_info(mf_factor* f, int64_t* info);
+
+/* Copy the factor back in the reference layout (Dval, Lval, and for LDLt d).
+ * Any pointer may be NULL.  device != 0: device pointers. */
+int mf_factor_get(mf_factor* f, double* Dval, double* Lval, double* d, int device);
+
+/*
+ * Natural solve C <- L^-T [D^-1] L^-1 C (ldiv!(W, F::NaturalFactorization, C),
+ * divide.jl:191-199, i.e. div_impl! forward, the diagonal solve and div_impl!
+ * backward, divide.jl:417-450).  C is n x nrhs column-major with leading
+ * dimension ldc, in permuted order.  device != 0: C is a device pointer.
+ */
+int mf_solve_natural(mf_factor* f, double* C, int64_t nrhs, int64_t ldc, int device);
+
+/*
+ * Full solve B <- A \ B = P' (L L')^-1 P B (ldiv!! divide.jl:1762-1764 with
+ * mul!(C, P, B) multiply.jl:277-280); perm is F.perm (1-based, host).
+ */
+int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t ldb,
+             int device);
+
+/*
+ * Drop-in forms with chol_impl!'s own arguments: Dval/Lval in (assembled A)
+ * and out (factor), d out (LDLt pivots), host memory, one call.
+ */
+int mf_chol_impl(mf_plan* plan, double* Dval, double* Lval, double* d, const double* signs,
+                 int diag, int reg_kind, double delta, double eps, int64_t* info);
+/* div_impl! forward + (LDLt) diagonal + backward on host arrays. */
+int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const double* d,
+                double* C, int64_t nrhs, int64_t ldc, int diag);
+
+/* Kernel timing (HIP events around each launch, on the factor's stream).
+ * times[k] = total milliseconds, counts[k] = launches, flops[k] = algorithmic
+ * work of kernel class k since the last reset (classes: MF_KC_*). */
+enum { MF_KC_SMALL = 0, MF_KC_EA, MF_KC_DIAG, MF_KC_GEMM, MF_KC_COPYOUT, MF_KC_SOLVE,
+       MF_KC_ASSEMBLE, MF_KC_COUNT };
+int mf_timing(mf_factor* f, int enable);
+int mf_timing_read(mf_factor* f, double* times_ms, int64_t* counts, double* work, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MFHIP_H */

@@ -1,0 +1,426 @@
+/*
+ * oracle.c — CPU FP64 oracle for the multifrontal numeric phase.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP engine is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg load it.  The product path never calls it.
+ *
+ * It is a plain-C restatement of the reference's serial front loop
+ * (paths relative to the CliqueTrees.jl tree, src/Multifrontal.jl/src/):
+ *   or_chol_impl   chol_impl!        cholesky.jl:361-392
+ *                  chol_loop_snd!    cholesky.jl:434-556
+ *                  chol_loop_nod!    cholesky.jl:560-692
+ *                  chol_factor!      cholesky.jl:694-737
+ *                  chol_send!        cholesky.jl:950-983 (addscattertri!, utils.jl:555-579)
+ *   potrf_rec      potrf! recursive  blas/potrf.jl:135-178, potrf2! blas/potrf.jl:3-41
+ *   regularize     no_regularization.jl:27-42, dynamic_regularization.jl:117-132
+ *   trsm_rlt       trsx2_fwd!(:R)    blas/trsx.jl:11-31 (B <- B A^-T, A lower)
+ *   or_div_fwd     div_impl! forward divide.jl:417-450, div_fwd_loop_snd!/nod! 493-821,
+ *                  div_fwd_update!   divide.jl:1206-1248
+ *   or_div_bwd     div_impl! backward, div_bwd_loop_snd!/nod! divide.jl:864-1204,
+ *                  div_bwd_update!   divide.jl:1250-1293
+ *
+ * The dense kernels are the reference's own pure-Julia fallbacks restated in
+ * C.  For Float64 the reference calls OpenBLAS instead (potrf.jl:87-103,
+ * trsx.jl:156-177, syrk.jl:137-159, gemx.jl:3-41); or_set_blas() installs
+ * host BLAS/LAPACK entry points (Fortran ABI, 32-bit integers) so the same
+ * loop can run at BLAS speed for the CPU baseline.  All index arrays are the
+ * reference's 1-based Int64 arrays.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef int64_t i64;
+
+/* ---- optional host BLAS (Fortran ABI) ---------------------------------- */
+typedef void (*f_dpotrf)(const char*, const int*, double*, const int*, int*);
+typedef void (*f_dtrsm)(const char*, const char*, const char*, const char*, const int*,
+                        const int*, const double*, const double*, const int*, double*,
+                        const int*);
+typedef void (*f_dsyrk)(const char*, const char*, const int*, const int*, const double*,
+                        const double*, const int*, const double*, double*, const int*);
+typedef void (*f_dgemm)(const char*, const char*, const int*, const int*, const int*,
+                        const double*, const double*, const int*, const double*, const int*,
+                        const double*, double*, const int*);
+
+static f_dpotrf B_potrf = 0;
+static f_dtrsm B_trsm = 0;
+static f_dsyrk B_syrk = 0;
+static f_dgemm B_gemm = 0;
+
+void or_set_blas(void* potrf, void* trsm, void* syrk, void* gemm) {
+    B_potrf = (f_dpotrf)potrf;
+    B_trsm = (f_dtrsm)trsm;
+    B_syrk = (f_dsyrk)syrk;
+    B_gemm = (f_dgemm)gemm;
+}
+
+/* ---- regularization ------------------------------------------------------ */
+enum { REG_NONE = 0, REG_DYNAMIC = 1 };
+typedef struct {
+    int kind;
+    double delta, eps;
+} reg_t;
+
+/* regularize(R, S, Djj, j): 0 signals a failed pivot. */
+static double regularize(const reg_t* R, double s, double Djj) {
+    if (R->kind == REG_DYNAMIC) {
+        if (s * Djj < R->eps) return R->delta * s;
+        return Djj;
+    }
+    if (s != 0.0 && !(s * Djj > 0.0)) return 0.0;
+    return Djj;
+}
+
+/* ---- dense kernels (column-major, leading dimension ld) ----------------- */
+#define A_(M, ld, i, j) (M)[(i64)(j) * (ld) + (i)]
+
+/* potrf2!(:L) blas/potrf.jl:3-41: unblocked right-looking; diag=0 Cholesky, 1 LDLt */
+static i64 potrf2(double* A, i64 lda, i64 n, double* d, const double* S, const reg_t* R,
+                  int diag) {
+    for (i64 j = 0; j < n; ++j) {
+        double Ajj = regularize(R, S[j], A_(A, lda, j, j));
+        if (Ajj == 0.0) return j + 1;
+        if (diag == 0) {
+            Ajj = sqrt(Ajj);
+            A_(A, lda, j, j) = Ajj;
+        } else {
+            d[j] = Ajj;
+        }
+        double iD = 1.0 / Ajj;
+        for (i64 i = j + 1; i < n; ++i) A_(A, lda, i, j) *= iD;
+        for (i64 k = j + 1; k < n; ++k) {
+            double Akj = A_(A, lda, k, j);
+            double f = diag == 0 ? Akj : Ajj * Akj;
+            A_(A, lda, k, k) -= f * Akj;
+            for (i64 i = k + 1; i < n; ++i) A_(A, lda, i, k) -= A_(A, lda, i, j) * f;
+        }
+    }
+    return 0;
+}
+
+/* B (m x n) <- B A^-T with A n x n lower (unit if diag==1): trsx2_fwd!(:R) */
+static void trsm_rlt(const double* A, i64 lda, double* B, i64 ldb, i64 m, i64 n, int unit) {
+    if (B_trsm && m > 0 && n > 0) {
+        int mi = (int)m, ni = (int)n, la = (int)lda, lb = (int)ldb;
+        double one = 1.0;
+        B_trsm("R", "L", "T", unit ? "U" : "N", &mi, &ni, &one, A, &la, B, &lb);
+        return;
+    }
+    for (i64 j = 0; j < n; ++j) {
+        for (i64 k = 0; k < j; ++k) {
+            double Ajk = A_(A, lda, j, k);
+            for (i64 i = 0; i < m; ++i) A_(B, ldb, i, j) -= A_(B, ldb, i, k) * Ajk;
+        }
+        if (!unit) {
+            double iA = 1.0 / A_(A, lda, j, j);
+            for (i64 i = 0; i < m; ++i) A_(B, ldb, i, j) *= iA;
+        }
+    }
+}
+
+/* C (n x n, lower) -= A diag(d) A^T, A n x k (d == NULL: identity) */
+static void syrk_ln(const double* A, i64 lda, const double* d, double* C, i64 ldc, i64 n,
+                    i64 k) {
+    if (!d && B_syrk && n > 0 && k > 0) {
+        int ni = (int)n, ki = (int)k, la = (int)lda, lc = (int)ldc;
+        double m1 = -1.0, one = 1.0;
+        B_syrk("L", "N", &ni, &ki, &m1, A, &la, &one, C, &lc);
+        return;
+    }
+    if (d && B_gemm && n > 0 && k > 0) {
+        /* W = A diag(d); C_lower -= W A^T, by column blocks of C */
+        double* W = (double*)malloc(sizeof(double) * (size_t)(n * k));
+        for (i64 c = 0; c < k; ++c)
+            for (i64 i = 0; i < n; ++i) W[c * n + i] = A_(A, lda, i, c) * d[c];
+        const i64 nb = 128;
+        for (i64 j0 = 0; j0 < n; j0 += nb) {
+            i64 jb = n - j0 < nb ? n - j0 : nb;
+            int mi = (int)(n - j0), ni = (int)jb, ki = (int)k, lw = (int)n, la = (int)lda,
+                lc = (int)ldc;
+            double m1 = -1.0, one = 1.0;
+            B_gemm("N", "T", &mi, &ni, &ki, &m1, W + j0, &lw, A + j0, &la, &one,
+                   C + j0 * ldc + j0, &lc);
+        }
+        free(W);
+        return;
+    }
+    for (i64 c = 0; c < k; ++c) {
+        for (i64 j = 0; j < n; ++j) {
+            double f = A_(A, lda, j, c) * (d ? d[c] : 1.0);
+            if (f == 0.0) continue;
+            for (i64 i = j; i < n; ++i) A_(C, ldc, i, j) -= A_(A, lda, i, c) * f;
+        }
+    }
+}
+
+/* potrf! recursive, blas/potrf.jl:135-178 (LAPACK for Cholesky without regularization) */
+static i64 potrf_rec(double* A, i64 lda, i64 n, double* d, const double* S, const reg_t* R,
+                     int diag) {
+    if (diag == 0 && R->kind == REG_NONE && B_potrf) {
+        /* potrf.jl:101-103: LAPACK.potrf! ignores signs (they are Ones here) */
+        int ni = (int)n, la = (int)lda, info = 0;
+        if (n == 0) return 0;
+        B_potrf("L", &ni, A, &la, &info);
+        return info;
+    }
+    if (n <= 64) return potrf2(A, lda, n, d, S, R, diag);
+    i64 n1 = 1;
+    while (n1 * 2 <= n) n1 *= 2;
+    n1 /= 2;
+    i64 info = potrf_rec(A, lda, n1, d, S, R, diag);
+    if (info) return info;
+    double* B = A + n1; /* rows n1.., cols 0..n1-1 */
+    trsm_rlt(A, lda, B, lda, n - n1, n1, diag);
+    if (diag) {
+        for (i64 c = 0; c < n1; ++c) {
+            double idc = 1.0 / d[c];
+            for (i64 i = 0; i < n - n1; ++i) A_(B, lda, i, c) *= idc;
+        }
+    }
+    syrk_ln(B, lda, diag ? d : NULL, A + n1 * lda + n1, lda, n - n1, n1);
+    info = potrf_rec(A + n1 * lda + n1, lda, n - n1, d + n1, S + n1, R, diag);
+    if (info) return n1 + info;
+    return 0;
+}
+
+/* ---- symbolic view ------------------------------------------------------- */
+typedef struct {
+    i64 n, nfr;
+    const i64 *resptr, *sepptr, *reltgt, *chdptr, *chdtgt, *Dptr, *Lptr;
+} sym_t;
+
+/* addscattertri!(F, M, inj, :L): F[inj[i], inj[j]] += M[i, j], j <= i */
+static void send_tri(double* F, i64 ldf, const double* M, i64 na, const i64* inj) {
+    for (i64 j = 0; j < na; ++j) {
+        i64 fj = inj[j] - 1;
+        for (i64 i = j; i < na; ++i) A_(F, ldf, inj[i] - 1, fj) += A_(M, na, i, j);
+    }
+}
+
+/*
+ * chol_impl! (cholesky.jl:361-392).  diag: 0 = Cholesky (:N), 1 = LDLt (:U).
+ * Mval: workspace of nMval doubles; Mptr: nMptr+1 entries; Fval: nFval^2.
+ * Returns 0, or the first failing column in permuted order (1-based).
+ */
+i64 or_chol_impl(i64 n, i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt,
+                 const i64* chdptr, const i64* chdtgt, const i64* Dptr, const i64* Lptr,
+                 double* Dval, double* Lval, double* d, const double* signs, int diag,
+                 int reg_kind, double delta, double eps, double* Mval, i64* Mptr,
+                 double* Fval) {
+    reg_t R = {reg_kind, delta, eps};
+    (void)n;
+    i64 ns = 0;
+    Mptr[1] = 0; /* 1-based stack index, 0-based offsets */
+    for (i64 j = 0; j < nfr; ++j) {
+        i64 r0 = resptr[j] - 1;
+        i64 nn = resptr[j + 1] - resptr[j];
+        i64 na = sepptr[j + 1] - sepptr[j];
+        i64 nj = nn + na;
+        double* F = Fval;
+        for (i64 c = 0; c < nj; ++c)
+            for (i64 r = c; r < nj; ++r) A_(F, nj, r, c) = 0.0;
+        for (i64 q = chdptr[j + 1] - 1; q >= chdptr[j]; --q) {
+            i64 i = chdtgt[q - 1] - 1;
+            i64 ma = sepptr[i + 1] - sepptr[i];
+            send_tri(F, nj, Mval + Mptr[ns], ma, reltgt + sepptr[i] - 1);
+            ns -= 1;
+        }
+        double* D11 = Dval + Dptr[j] - 1;
+        double* L21 = Lval + Lptr[j] - 1;
+        if (nn == 1) {
+            /* chol_loop_nod! */
+            double d11 = D11[0] + F[0];
+            for (i64 r = 0; r < na; ++r) L21[r] += F[1 + r];
+            d11 = regularize(&R, signs[r0], d11);
+            if (d11 == 0.0) return 1 + r0;
+            if (diag == 0) {
+                d11 = sqrt(d11);
+                D11[0] = d11;
+            } else {
+                d[r0] = d11;
+            }
+            if (na > 0) {
+                ns += 1;
+                i64 strt = Mptr[ns];
+                Mptr[ns + 1] = strt + na * na;
+                double* M = Mval + strt;
+                for (i64 c = 0; c < na; ++c)
+                    for (i64 r = c; r < na; ++r) A_(M, na, r, c) = A_(F, nj, 1 + r, 1 + c);
+                for (i64 r = 0; r < na; ++r) L21[r] /= d11;
+                double alpha = diag ? -d11 : -1.0;
+                for (i64 c = 0; c < na; ++c) {
+                    double f = alpha * L21[c];
+                    for (i64 r = c; r < na; ++r) A_(M, na, r, c) += L21[r] * f;
+                }
+            }
+            continue;
+        }
+        /* chol_loop_snd!: L <- L + F */
+        for (i64 c = 0; c < nn; ++c) {
+            for (i64 r = c; r < nn; ++r) A_(D11, nn, r, c) += A_(F, nj, r, c);
+            for (i64 r = 0; r < na; ++r) A_(L21, na, r, c) += A_(F, nj, nn + r, c);
+        }
+        /* chol_factor! */
+        double* dj = diag ? d + r0 : NULL;
+        double dummy_d[1];
+        i64 info = potrf_rec(D11, nn, nn, dj ? dj : dummy_d, signs + r0, &R, diag);
+        if (info) return info + r0;
+        if (na > 0) {
+            trsm_rlt(D11, nn, L21, na, na, nn, diag);
+            if (diag) {
+                for (i64 c = 0; c < nn; ++c) {
+                    double idc = 1.0 / dj[c];
+                    for (i64 r = 0; r < na; ++r) A_(L21, na, r, c) *= idc;
+                }
+            }
+            ns += 1;
+            i64 strt = Mptr[ns];
+            Mptr[ns + 1] = strt + na * na;
+            double* M = Mval + strt;
+            for (i64 c = 0; c < na; ++c)
+                for (i64 r = c; r < na; ++r) A_(M, na, r, c) = A_(F, nj, nn + r, nn + c);
+            syrk_ln(L21, na, dj, M, na, na, nn);
+        }
+    }
+    return 0;
+}
+
+/* C (m x nrhs) -= A (m x k) X (k x nrhs) ; trans: A^T (A is k x m) */
+static void gemm_sub(const double* A, i64 lda, int trans, const double* X, i64 ldx, double* C,
+                     i64 ldc, i64 m, i64 k, i64 nrhs) {
+    if (m == 0 || k == 0 || nrhs == 0) return;
+    if (B_gemm) {
+        int mi = (int)m, ni = (int)nrhs, ki = (int)k, la = (int)lda, lx = (int)ldx,
+            lc = (int)ldc;
+        double m1 = -1.0, one = 1.0;
+        B_gemm(trans ? "T" : "N", "N", &mi, &ni, &ki, &m1, A, &la, X, &lx, &one, C, &lc);
+        return;
+    }
+    for (i64 c = 0; c < nrhs; ++c) {
+        if (!trans) {
+            for (i64 kk = 0; kk < k; ++kk) {
+                double x = A_(X, ldx, kk, c);
+                for (i64 r = 0; r < m; ++r) A_(C, ldc, r, c) -= A_(A, lda, r, kk) * x;
+            }
+        } else {
+            for (i64 r = 0; r < m; ++r) {
+                double s = 0.0;
+                for (i64 kk = 0; kk < k; ++kk) s += A_(A, lda, kk, r) * A_(X, ldx, kk, c);
+                A_(C, ldc, r, c) -= s;
+            }
+        }
+    }
+}
+
+/* X (n x nrhs) <- A^-1 X (fwd, A lower) or A^-T X (bwd) */
+static void trsm_left(const double* A, i64 lda, double* X, i64 ldx, i64 n, i64 nrhs, int bwd,
+                      int unit) {
+    if (B_trsm && n > 0 && nrhs > 0) {
+        int ni = (int)n, mi = (int)nrhs, la = (int)lda, lx = (int)ldx;
+        double one = 1.0;
+        B_trsm("L", "L", bwd ? "T" : "N", unit ? "U" : "N", &ni, &mi, &one, A, &la, X, &lx);
+        return;
+    }
+    for (i64 c = 0; c < nrhs; ++c) {
+        double* x = X + c * ldx;
+        if (!bwd) {
+            for (i64 j = 0; j < n; ++j) {
+                for (i64 k = 0; k < j; ++k) x[j] -= A_(A, lda, j, k) * x[k];
+                if (!unit) x[j] *= 1.0 / A_(A, lda, j, j);
+            }
+        } else {
+            for (i64 j = n - 1; j >= 0; --j) {
+                if (!unit) x[j] *= 1.0 / A_(A, lda, j, j);
+                for (i64 k = 0; k < j; ++k) x[k] -= A_(A, lda, j, k) * x[j];
+            }
+        }
+    }
+}
+
+/*
+ * Forward solve C <- L^-1 C (div_impl!, forward branch), C is n x nrhs
+ * (ld ldc) in permuted order.  Mval: nNval*nrhs, Mptr: nMptr+1, Fval: nFval*nrhs.
+ */
+void or_div_fwd(i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt,
+                const i64* chdptr, const i64* chdtgt, const i64* Dptr, const i64* Lptr,
+                const double* Dval, const double* Lval, int diag, double* C, i64 ldc,
+                i64 nrhs, double* Mval, i64* Mptr, double* Fval) {
+    i64 ns = 0;
+    Mptr[1] = 0;
+    for (i64 j = 0; j < nfr; ++j) {
+        i64 r0 = resptr[j] - 1;
+        i64 nn = resptr[j + 1] - resptr[j];
+        i64 na = sepptr[j + 1] - sepptr[j];
+        i64 nj = nn + na;
+        double* F = Fval;
+        memset(F, 0, sizeof(double) * (size_t)(nj * nrhs));
+        for (i64 q = chdptr[j + 1] - 1; q >= chdptr[j]; --q) {
+            i64 i = chdtgt[q - 1] - 1;
+            i64 ma = sepptr[i + 1] - sepptr[i];
+            const i64* inj = reltgt + sepptr[i] - 1;
+            const double* M = Mval + Mptr[ns];
+            for (i64 c = 0; c < nrhs; ++c)
+                for (i64 r = 0; r < ma; ++r) A_(F, nj, inj[r] - 1, c) += A_(M, ma, r, c);
+            ns -= 1;
+        }
+        double* C1 = C + r0;
+        for (i64 c = 0; c < nrhs; ++c)
+            for (i64 r = 0; r < nn; ++r) A_(C1, ldc, r, c) += A_(F, nj, r, c);
+        const double* D11 = Dval + Dptr[j] - 1;
+        const double* L21 = Lval + Lptr[j] - 1;
+        trsm_left(D11, nn, C1, ldc, nn, nrhs, 0, diag);
+        if (na > 0) {
+            ns += 1;
+            i64 strt = Mptr[ns];
+            Mptr[ns + 1] = strt + na * nrhs;
+            double* M = Mval + strt;
+            for (i64 c = 0; c < nrhs; ++c)
+                for (i64 r = 0; r < na; ++r) A_(M, na, r, c) = A_(F, nj, nn + r, c);
+            gemm_sub(L21, na, 0, C1, ldc, M, na, na, nn, nrhs);
+        }
+    }
+}
+
+/* Backward solve C <- L^-T C (div_impl!, backward branch). */
+void or_div_bwd(i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt,
+                const i64* chdptr, const i64* chdtgt, const i64* Dptr, const i64* Lptr,
+                const double* Dval, const double* Lval, int diag, double* C, i64 ldc,
+                i64 nrhs, double* Mval, i64* Mptr, double* Fval) {
+    i64 ns = 0;
+    Mptr[1] = 0;
+    for (i64 j = nfr - 1; j >= 0; --j) {
+        i64 r0 = resptr[j] - 1;
+        i64 nn = resptr[j + 1] - resptr[j];
+        i64 na = sepptr[j + 1] - sepptr[j];
+        i64 nj = nn + na;
+        double* C1 = C + r0;
+        const double* D11 = Dval + Dptr[j] - 1;
+        const double* L21 = Lval + Lptr[j] - 1;
+        const double* M2 = NULL;
+        if (na > 0) {
+            M2 = Mval + Mptr[ns];
+            ns -= 1;
+            gemm_sub(L21, na, 1, M2, na, C1, ldc, nn, na, nrhs);
+        }
+        trsm_left(D11, nn, C1, ldc, nn, nrhs, 1, diag);
+        double* F = Fval;
+        for (i64 c = 0; c < nrhs; ++c) {
+            for (i64 r = 0; r < nn; ++r) A_(F, nj, r, c) = A_(C1, ldc, r, c);
+            for (i64 r = 0; r < na; ++r) A_(F, nj, nn + r, c) = A_(M2, na, r, c);
+        }
+        for (i64 q = chdptr[j]; q < chdptr[j + 1]; ++q) {
+            i64 i = chdtgt[q - 1] - 1;
+            i64 ma = sepptr[i + 1] - sepptr[i];
+            const i64* inj = reltgt + sepptr[i] - 1;
+            ns += 1;
+            i64 strt = Mptr[ns];
+            Mptr[ns + 1] = strt + ma * nrhs;
+            double* M = Mval + strt;
+            for (i64 c = 0; c < nrhs; ++c)
+                for (i64 r = 0; r < ma; ++r) A_(M, ma, r, c) = A_(F, nj, inj[r] - 1, c);
+        }
+    }
+}

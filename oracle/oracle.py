@@ -1,0 +1,171 @@
+"""ctypes front end of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  Never used by the product path.
+
+Functions mirror the reference's host verbs on the flat arrays of a
+``ChordalSymbolic`` (1-based Int64):
+  * ``assemble``  — copyto!(F, A) (src/Multifrontal.jl/src/chordal_triangular.jl:852-893)
+  * ``chol_impl`` — chol_impl!  (src/Multifrontal.jl/src/cholesky.jl:361-392)
+  * ``solve``     — ldiv!(F, B) (src/Multifrontal.jl/src/divide.jl:191-206,1762-1764)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+i64p = ctypes.POINTER(ctypes.c_int64)
+f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.or_set_blas.argtypes = [ctypes.c_void_p] * 4
+        L.or_set_blas.restype = None
+        L.or_chol_impl.argtypes = ([ctypes.c_int64] * 2 + [i64p] * 7 + [f64p] * 4 +
+                                   [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, f64p, i64p, f64p])
+        L.or_chol_impl.restype = ctypes.c_int64
+        for f in (L.or_div_fwd, L.or_div_bwd):
+            f.argtypes = ([ctypes.c_int64] + [i64p] * 7 + [f64p] * 2 + [ctypes.c_int, f64p,
+                          ctypes.c_int64, ctypes.c_int64, f64p, i64p, f64p])
+            f.restype = None
+        _lib = L
+    return _lib
+
+
+def _capsule_ptr(capsule):
+    get_name = ctypes.pythonapi.PyCapsule_GetName
+    get_name.restype = ctypes.c_char_p
+    get_name.argtypes = [ctypes.py_object]
+    get_ptr = ctypes.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype = ctypes.c_void_p
+    get_ptr.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    return get_ptr(capsule, get_name(capsule))
+
+
+def use_host_blas(on=True):
+    """Route the oracle's dense kernels to scipy's LAPACK/BLAS (OpenBLAS, LP64),
+    as the reference does for Float64; ``on=False`` restores the C loops."""
+    if not on:
+        lib().or_set_blas(None, None, None, None)
+        return
+    from scipy.linalg import cython_blas, cython_lapack
+    b, l = cython_blas.__pyx_capi__, cython_lapack.__pyx_capi__
+    lib().or_set_blas(_capsule_ptr(l["dpotrf"]), _capsule_ptr(b["dtrsm"]),
+                      _capsule_ptr(b["dsyrk"]), _capsule_ptr(b["dgemm"]))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _sym_args(S):
+    return [_p(S.resptr, i64p), _p(S.sepptr, i64p), _p(S.reltgt, i64p), _p(S.chdptr, i64p),
+            _p(S.chdtgt, i64p), _p(S.Dptr, i64p), _p(S.Lptr, i64p)]
+
+
+def assemble(S, P, A):
+    """Dval, Lval holding the lower triangle of P A P' (copyto!, uplo = :L)."""
+    import scipy.sparse as sp
+    A = sp.csc_matrix(A)
+    n = A.shape[0]
+    Dval = np.zeros(S.ndz)
+    Lval = np.zeros(S.nlz)
+    invp = P.invp - 1
+    coo = A.tocoo()
+    v = invp[coo.row]
+    w = invp[coo.col]
+    keep = v >= w
+    v, w, x = v[keep], w[keep], coo.data[keep]
+    j = S.idx[w] - 1
+    r0 = S.resptr[j] - 1
+    nn = S.resptr[j + 1] - S.resptr[j]
+    inres = v < r0 + nn
+    pd = S.Dptr[j] - 1 + (v - r0) + (w - r0) * nn
+    np.add.at(Dval, pd[inres], x[inres])
+    # rows in sep: locate with a search in septgt of the front
+    nl = ~inres
+    if nl.any():
+        # septgt is sorted inside each front and fronts are stored in order, so
+        # the key front*(n+1)+row is globally sorted.
+        jj = j[nl]
+        na_all = np.diff(S.sepptr)
+        fr = np.repeat(np.arange(S.nfr, dtype=np.int64), na_all)
+        keys = fr * (n + 1) + S.septgt
+        q = jj * (n + 1) + (v[nl] + 1)
+        g = np.searchsorted(keys, q)
+        ok = (g < len(keys)) & (keys[np.minimum(g, len(keys) - 1)] == q)
+        assert ok.all(), "entry outside the symbolic pattern"
+        pos = g - (S.sepptr[jj] - 1)
+        na = na_all[jj]
+        pl = S.Lptr[jj] - 1 + pos + (w[nl] - r0[nl]) * na
+        np.add.at(Lval, pl, x[nl])
+    assert n == S.n
+    return Dval, Lval
+
+
+def chol_impl(S, Dval, Lval, diag=0, signs=None, reg=None):
+    """In place on copies: returns (Dval, Lval, d, info); info as chol_impl!."""
+    Dval = np.array(Dval, dtype=np.float64, copy=True)
+    Lval = np.array(Lval, dtype=np.float64, copy=True)
+    n = S.n
+    d = np.ones(n) if diag == 0 else np.zeros(n)
+    if signs is None:
+        signs = np.ones(n) if diag == 0 else np.zeros(n)
+    signs = np.ascontiguousarray(signs, dtype=np.float64)
+    kind, delta, eps = 0, 0.0, 0.0
+    if reg is not None:
+        kind, delta, eps = 1, float(reg[0]), float(reg[1])
+    Mval = np.zeros(max(S.nMval, 1))
+    Mptr = np.zeros(S.nMptr + 2, dtype=np.int64)
+    Fval = np.zeros(max(S.nFval * S.nFval, 2))
+    info = lib().or_chol_impl(n, S.nfr, *_sym_args(S), _p(Dval, f64p), _p(Lval, f64p),
+                              _p(d, f64p), _p(signs, f64p), diag, kind, delta, eps,
+                              _p(Mval, f64p), _p(Mptr, i64p), _p(Fval, f64p))
+    return Dval, Lval, d, int(info)
+
+
+def div_natural(S, Dval, Lval, d, C, diag=0):
+    """C <- L^-T [D^-1] L^-1 C for C in permuted order (n x nrhs, Fortran)."""
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    if C.ndim == 1:
+        C = C.reshape(-1, 1, order="F")
+    nrhs = C.shape[1]
+    Mval = np.zeros(max(S.nNval * nrhs, 1))
+    Mptr = np.zeros(S.nMptr + 2, dtype=np.int64)
+    Fval = np.zeros(max(S.nFval * nrhs, 1))
+    Dval = np.ascontiguousarray(Dval)
+    Lval = np.ascontiguousarray(Lval)
+    args = [S.nfr, *_sym_args(S), _p(Dval, f64p), _p(Lval, f64p), diag, _p(C, f64p),
+            S.n, nrhs, _p(Mval, f64p), _p(Mptr, i64p), _p(Fval, f64p)]
+    lib().or_div_fwd(*args)
+    if diag == 1:
+        C /= np.asarray(d)[:, None]
+    lib().or_div_bwd(*args)
+    return C
+
+
+def solve(S, P, Dval, Lval, d, B, diag=0):
+    """x = A \\ B through the factorization (ldiv!!: P, natural solve, P')."""
+    B = np.asarray(B, dtype=np.float64)
+    vec = B.ndim == 1
+    C = P.apply(B)
+    X = div_natural(S, Dval, Lval, d, C, diag)
+    X = P.apply_inv(X)
+    return X[:, 0] if vec else X

@@ -1,0 +1,259 @@
+"""Parity of the HIP engine against the CPU oracle (same perm, same supernode
+partition, same assembled values).
+
+Factor entries are compared with the reference's own normwise measure
+(isapprox on ChordalTriangular: sqrt(dist2) <= rtol * max(norm),
+src/Multifrontal.jl/src/chordal_triangular.jl:648-660,710-713) over Dval and
+Lval; FP64 tolerance 1e-12.  Solves are checked against the oracle's solve
+and by the residual ||Ax-b||/||b|| (test/multifrontal.jl:23-40 pattern).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-12
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = max(np.linalg.norm(a), np.linalg.norm(b), 1e-300)
+    return np.linalg.norm(a - b) / den
+
+
+def diag_mask(S):
+    """Mask that hides the diagonal entries of the Dval blocks."""
+    m = np.ones(S.ndz, dtype=bool)
+    nn = S.nn()
+    for j in range(S.nfr):
+        p = S.Dptr[j] - 1
+        k = nn[j]
+        m[p + np.arange(k) * (k + 1)] = False
+    return m
+
+
+def problems(ct):
+    P = ct.problems
+    out = []
+    A5 = sp.csc_matrix(np.array([[4, 2, 0, 0, 2], [2, 5, 0, 0, 3], [0, 0, 4, 2, 0],
+                                 [0, 0, 2, 5, 2], [2, 3, 0, 2, 7]], float))
+    out.append(("readme5", A5, [3, 1, 2, 4, 5], "maximal"))
+    out.append(("lap2d_30_nd", P.laplacian2d_dirichlet(30, 30), ct.grid_nd(30, 30, 1), "maximal"))
+    out.append(("lap2d_20_nat", P.laplacian2d_dirichlet(20, 20), None, "maximal"))
+    out.append(("lap2d_12_nodal", P.laplacian2d_dirichlet(12, 12), ct.grid_nd(12, 12, 1), "nodal"))
+    out.append(("lap3d_12_fund", P.laplacian3d_7pt(12, 12, 12, seed=2), ct.grid_nd(12, 12, 12),
+                "fundamental"))
+    out.append(("lap3d_16", P.laplacian3d_7pt(16, 16, 16, seed=2), ct.grid_nd(16, 16, 16), "maximal"))
+    out.append(("st27_10", P.stencil3d_27pt(10, 10, 10, seed=3), ct.grid_nd(10, 10, 10), "maximal"))
+    out.append(("lap3d_20x18x9", P.laplacian3d_7pt(20, 18, 9, seed=5), ct.grid_nd(20, 18, 9), "maximal"))
+    return out
+
+
+PROBS = ["readme5", "lap2d_30_nd", "lap2d_20_nat", "lap2d_12_nodal", "lap3d_12_fund", "lap3d_16",
+         "st27_10", "lap3d_20x18x9"]
+
+
+@pytest.fixture(scope="module")
+def cases(ct):
+    return {name: (A, order, snd) for name, A, order, snd in problems(ct)}
+
+
+@pytest.mark.parametrize("name", PROBS)
+def test_cholesky_factor_parity(ct, oracle, cases, name):
+    A, order, snd = cases[name]
+    print(f"case {name} n={A.shape[0]}", flush=True)
+    P, S = ct.symbolic(A, alg=order, snd=snd)
+    Dv, Lv = oracle.assemble(S, P, A)
+    D0, L0, d0, info0 = oracle.chol_impl(S, Dv, Lv, diag=0)
+    assert info0 == 0
+    # drop-in seam (chol_impl! arguments)
+    D1, L1, d1, info1 = ct.chol_impl(S, Dv, Lv, diag=ct.MF_CHOLESKY)
+    assert info1 == 0
+    assert rel(D1, D0) <= RTOL64 and rel(L1, L0) <= RTOL64
+    # API path: assembly on the device from flatindices
+    F = ct.ChordalCholesky(A, P, S)
+    ct.cholesky_(F)
+    assert ct.issuccess(F)
+    assert rel(F.Dval, D0) <= RTOL64 and rel(F.Lval, L0) <= RTOL64
+    # the upper triangles of the diagonal blocks stay zero
+    nn = S.nn()
+    for j in range(0, S.nfr, max(1, S.nfr // 50)):
+        k = nn[j]
+        blk = F.Dval[S.Dptr[j] - 1: S.Dptr[j] - 1 + k * k].reshape(k, k, order="F")
+        assert np.all(np.triu(blk, 1) == 0)
+
+
+@pytest.mark.parametrize("name", PROBS)
+@pytest.mark.parametrize("nrhs", [1, 16, 21])
+def test_cholesky_solve_parity(ct, oracle, cases, name, nrhs):
+    A, order, snd = cases[name]
+    print(f"case {name} nrhs={nrhs}", flush=True)
+    P, S = ct.symbolic(A, alg=order, snd=snd)
+    F = ct.ChordalCholesky(A, P, S)
+    ct.cholesky_(F)
+    rng = np.random.default_rng(nrhs)
+    B = rng.standard_normal((A.shape[0], nrhs))
+    X = ct.solve(F, B)
+    Dv, Lv = oracle.assemble(S, P, A)
+    D0, L0, d0, _ = oracle.chol_impl(S, Dv, Lv, diag=0)
+    X0 = oracle.solve(S, P, D0, L0, d0, B, diag=0)
+    assert rel(X, X0) <= 1e-11
+    res = np.linalg.norm(A @ X - B) / np.linalg.norm(B)
+    assert res <= 1e-12
+    # natural solve through the div_impl! seam
+    C = P.apply(B)
+    Xn = ct.div_impl(S, D0, L0, None, C, diag=ct.MF_CHOLESKY)
+    Xn0 = oracle.div_natural(S, D0, L0, d0, C, diag=0)
+    assert rel(Xn, Xn0) <= 1e-11
+
+
+@pytest.mark.parametrize("name", ["lap2d_30_nd", "lap3d_16", "st27_10", "lap3d_12_fund"])
+def test_ldlt_parity(ct, oracle, cases, name):
+    A, order, snd = cases[name]
+    print(f"case {name} ldlt", flush=True)
+    P, S = ct.symbolic(A, alg=order, snd=snd)
+    Dv, Lv = oracle.assemble(S, P, A)
+    D0, L0, d0, info0 = oracle.chol_impl(S, Dv, Lv, diag=1)
+    assert info0 == 0
+    F = ct.ChordalLDLt(A, P, S)
+    ct.ldlt_(F)
+    assert ct.issuccess(F)
+    assert rel(F.d, d0) <= RTOL64
+    assert rel(F.Lval, L0) <= RTOL64
+    # Dval: off-diagonal entries pinned; diagonal entries hold the pivot before
+    # regularization (nn > 1) or the assembled entry of A (nn = 1)
+    assert rel(F.Dval, D0) <= RTOL64
+    B = np.random.default_rng(1).standard_normal((A.shape[0], 3))
+    X = ct.solve(F, B)
+    assert np.linalg.norm(A @ X - B) / np.linalg.norm(B) <= 1e-12
+
+
+def test_ldlt_quasidefinite_kkt(ct, oracle):
+    K, nh, m = ct.problems.kkt_matrix((6, 6, 5), seed=5)
+    P, S = ct.symbolic(K, alg="mmd") if _has_mmd(ct) else ct.symbolic(K, alg=_kkt_order(nh, m))
+    Dv, Lv = oracle.assemble(S, P, K)
+    D0, L0, d0, info0 = oracle.chol_impl(S, Dv, Lv, diag=1)
+    assert info0 == 0
+    F = ct.ChordalLDLt(K, P, S)
+    signs = np.concatenate([np.ones(nh), -np.ones(m)])
+    ct.ldlt_(F, signs=signs)
+    assert rel(F.d, d0) <= RTOL64 and rel(F.Lval, L0) <= RTOL64
+    assert np.all(np.sign(F.d[P.invp - 1]) == signs)
+    b = np.random.default_rng(2).standard_normal(K.shape[0])
+    x = ct.solve(F, b)
+    assert np.linalg.norm(K @ x - b) / np.linalg.norm(b) <= 1e-10
+
+
+def _has_mmd(ct):
+    try:
+        ct.mmd(np.eye(2))
+        return True
+    except NotImplementedError:
+        return False
+
+
+def _kkt_order(nh, m):
+    # H block first, constraint rows last: quasi-definite, so any order works
+    return np.arange(1, nh + m + 1)
+
+
+def test_readme_examples(ct):
+    """Known answers printed in src/Multifrontal.jl/README.md:29-124."""
+    A = np.array([[4, 2, 0, 0, 2], [2, 5, 0, 0, 3], [0, 0, 4, 2, 0], [0, 0, 2, 5, 2],
+                  [2, 3, 0, 2, 7]], float)
+    F = ct.ChordalCholesky(A)
+    ct.cholesky_(F)
+    assert np.allclose(ct.solve(F, np.array([4., 2, 0, 0, 2])), [1, 0, 0, 0, 0], atol=1e-14)
+    Ai = np.array([[-4, -2, 0, 0, 2], [-2, -5, 0, 0, 3], [0, 0, 4, 2, 0], [0, 0, 2, 5, 2],
+                   [2, 3, 0, 2, 7]], float)
+    G = ct.ChordalLDLt(Ai, alg=[3, 1, 2, 4, 5])
+    ct.ldlt_(G)
+    assert np.allclose(ct.solve(G, np.array([-4., -2, 0, 0, 2])), [1, 0, 0, 0, 0], atol=1e-14)
+    assert sorted(G.d.tolist()) == sorted([4.0, -4.0, -4.0, 4.0, 8.0])
+    G2 = ct.ChordalLDLt(Ai, alg=[3, 1, 2, 4, 5])
+    ct.ldlt_(G2, signs=[-1, -1, 1, 1, 1])
+    assert ct.issuccess(G2)
+
+
+@pytest.mark.parametrize("name", ["lap2d_30_nd", "lap3d_16"])
+def test_failure_info(ct, oracle, cases, name):
+    """Not positive definite: info is the first failing column (cholesky.jl:384-388),
+    mapped through perm (cholesky.jl:250-254); check=True raises PosDefException."""
+    A, order, snd = cases[name]
+    lam = 4.0 if name.startswith("lap2d") else 0.5
+    B = (A - lam * sp.identity(A.shape[0])).tocsc()
+    P, S = ct.symbolic(B, alg=order, snd=snd)
+    Dv, Lv = oracle.assemble(S, P, B)
+    _, _, _, info0 = oracle.chol_impl(S, Dv, Lv, diag=0)
+    assert info0 > 0
+    F = ct.ChordalCholesky(B, P, S)
+    ct.cholesky_(F, check=False)
+    assert not ct.issuccess(F)
+    assert F.info == P.perm[info0 - 1]
+    F2 = ct.ChordalCholesky(B, P, S)
+    with pytest.raises(ct.PosDefException) as e:
+        ct.cholesky_(F2)
+    assert e.value.info == info0
+
+
+def test_pattern_reuse(ct, oracle):
+    """flatindices -> fill!(F, 0) -> factor fails -> new values -> succeeds
+    (test/multifrontal.jl:76-113)."""
+    A = ct.problems.laplacian3d_7pt(10, 10, 10, seed=7)
+    P, S = ct.symbolic(A, alg=ct.grid_nd(10, 10, 10))
+    F = ct.ChordalCholesky(A * 0.0, P, S)
+    ct.cholesky_(F, check=False)
+    assert not ct.issuccess(F)
+    for k in range(3):
+        Ak = ct.problems.laplacian3d_7pt(10, 10, 10, seed=10 + k)
+        F.copyto_(Ak)
+        ct.cholesky_(F)
+        Dv, Lv = oracle.assemble(S, P, Ak)
+        D0, L0, _, _ = oracle.chol_impl(S, Dv, Lv)
+        assert rel(F.Lval, L0) <= RTOL64 and rel(F.Dval, D0) <= RTOL64
+
+
+def test_dynamic_regularization(ct, oracle):
+    """DynamicRegularization example (dynamic_regularization.jl:52-73)."""
+    A = np.array([[1, 1, 2], [1, 1, 3], [2, 3, 1]], float)
+    P, S = ct.symbolic(A)
+    Dv, Lv = oracle.assemble(S, P, A)
+    reg = ct.DynamicRegularization()
+    delta, eps = reg.initialize(np.finfo(float).eps)
+    D0, L0, _, info0 = oracle.chol_impl(S, Dv, Lv, diag=0, signs=np.ones(3), reg=(delta, eps))
+    F = ct.ChordalCholesky(A, P, S)
+    ct.cholesky_(F, reg=reg)
+    assert ct.issuccess(F) and info0 == 0
+    assert rel(F.Dval, D0) <= RTOL64 and rel(F.Lval, L0) <= RTOL64
+
+
+@pytest.mark.parametrize("name", ["lap3d_16", "st27_10"])
+def test_fp32_fronts(ct, oracle, cases, name):
+    """FP32 fronts: factor within FP32 rounding of the FP64 oracle."""
+    A, order, snd = cases[name]
+    P, S = ct.symbolic(A, alg=order, snd=snd)
+    Dv, Lv = oracle.assemble(S, P, A)
+    D0, L0, _, _ = oracle.chol_impl(S, Dv, Lv)
+    F = ct.ChordalCholesky(A, P, S, precision=ct.MF_FP32)
+    ct.cholesky_(F)
+    assert rel(F.Lval, L0) <= 1e-4 and rel(F.Dval, D0) <= 1e-4
+    b = np.random.default_rng(3).standard_normal(A.shape[0])
+    x = ct.solve(F, b)
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1e-3
+
+
+def test_edge_cases(ct, oracle):
+    # 1x1, diagonal matrix (forest of single-column fronts), disconnected blocks
+    for A in [sp.csc_matrix(np.array([[2.0]])), sp.diags(np.arange(1.0, 40.0)).tocsc(),
+              sp.block_diag([ct.problems.laplacian2d_dirichlet(5, 5),
+                             ct.problems.laplacian3d_7pt(4, 4, 4, seed=1)]).tocsc()]:
+        P, S = ct.symbolic(A)
+        Dv, Lv = oracle.assemble(S, P, A)
+        D0, L0, d0, _ = oracle.chol_impl(S, Dv, Lv)
+        F = ct.ChordalCholesky(A, P, S)
+        ct.cholesky_(F)
+        assert rel(F.Dval, D0) <= RTOL64 and rel(F.Lval, L0) <= RTOL64
+        b = np.arange(1.0, A.shape[0] + 1)
+        assert rel(ct.solve(F, b), oracle.solve(S, P, D0, L0, d0, b)) <= 1e-12
