@@ -51,6 +51,11 @@ struct mf_plan {
     DevPlan D{};
     std::vector<void*> allocs;
     std::vector<uint8_t> in_small;
+    std::vector<int32_t> col2ref;
+    PTask* d_ftask = nullptr;
+    PTask* d_btask = nullptr;
+    int32_t* d_ntf = nullptr;
+    int32_t* d_ntb = nullptr;
     double ea_bytes = 0, small_flops = 0, diag_flops = 0;
     ~mf_plan() {
         for (void* p : allocs) (void)hipFree(p);
@@ -76,6 +81,7 @@ struct mf_factor {
     double* signs = nullptr;
     unsigned long long* info = nullptr;
     int32_t* flags = nullptr;
+    int32_t* prog = nullptr;
     uint32_t* ticket = nullptr;
     int32_t* hang = nullptr;
     int64_t* tgt_panel = nullptr;
@@ -88,6 +94,7 @@ struct mf_factor {
     void* Bws = nullptr;
     size_t Bws_bytes = 0;
     int64_t* perm_dev = nullptr;
+    std::vector<int64_t> perm_host;   // the permutation held in perm_dev
     int diag = MF_CHOLESKY;
     int reg_kind = 0;
     double delta = 0, eps = 0;
@@ -230,37 +237,12 @@ void enqueue_solve(mf_factor* f, T* B, int64_t ldb, int nrhs) {
     mf_plan* pl = f->plan;
     const HostPlan& H = pl->H;
     const DevPlan& P = pl->D;
-    hipStream_t s = f->stream;
     DevFactor<T> F = dev_factor<T>(f);
     DevSolve<T> S{B, ldb, (T*)f->Mv, nrhs};
     double bytes = (double)sizeof(T) * (2.0 * (double)(H.ndz + H.nlz) + 4.0 * H.n * nrhs);
     timed(f, MF_KC_SOLVE, bytes, 1, [&] {
-        launch_fill<int32_t>(f->flags, std::max<int>(P.nJ, 1), 0, s);
-        launch_fill<uint32_t>(f->ticket, 1, 0u, s);
-        launch_solve_small_fwd<T>(P, F, S, P.small_order, (int)H.n_small_region, true, s);
-        for (const Level& L : H.levels) {
-            launch_solve_small_fwd<T>(P, F, S, P.level_small + L.smallL0, L.smallL1 - L.smallL0,
-                                      false, s);
-            launch_sgather<T>(P, F, S, L.sea0, L.sea1, s);
-            for (const SolveStep& st : L.fsteps) {
-                launch_sdiag<T>(P, F, S, st.t0, st.t1, false, s);
-                launch_supd_fwd<T>(P, F, S, st.u0, st.u1, s);
-            }
-        }
-        if (f->diag == MF_LDLT) launch_dscale<T>(F, S, (int)H.n, s);
-        for (auto it = H.levels.rbegin(); it != H.levels.rend(); ++it) {
-            const Level& L = *it;
-            launch_solve_small_bwd<T>(P, F, S, P.level_small + L.smallL0, L.smallL1 - L.smallL0,
-                                      false, s);
-            launch_sbgather<T>(P, F, S, L.bg0, L.bg1, s);
-            for (const SolveStep& st : L.bsteps) {
-                launch_sdiag<T>(P, F, S, st.t0, st.t1, true, s);
-                launch_supd_bwd<T>(P, F, S, st.u0, st.u1, s);
-            }
-        }
-        launch_fill<int32_t>(f->flags, std::max<int>(P.nJ, 1), 0, s);
-        launch_fill<uint32_t>(f->ticket, 1, 0u, s);
-        launch_solve_small_bwd<T>(P, F, S, P.small_order, (int)H.n_small_region, true, s);
+        launch_solve<T>(P, F, S, P.nJ, f->flags, f->prog, pl->d_ntf, pl->d_ntb, pl->d_ftask,
+                        (int)H.ftask.size(), pl->d_btask, (int)H.btask.size(), (int)H.n, f->stream);
     });
 }
 
@@ -302,6 +284,9 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
     int rc = MF_OK;
     DevPlan& D = pl->D;
     pl->in_small.assign(H.fronts.size(), 0);
+    pl->col2ref.assign(H.n, 0);
+    for (int64_t j = 0; j < H.nfr; ++j)
+        for (int64_t g = H.resptr[j]; g < H.resptr[j + 1]; ++g) pl->col2ref[g] = (int32_t)j;
     for (int32_t J : H.small_order) pl->in_small[J] = 1;
     std::vector<int32_t> ea_src = H.ea_src;
     if ((rc = upload(H.fronts, (Front**)&D.fronts, pl->allocs)) ||
@@ -315,11 +300,15 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
         (rc = upload(ea_src, (int32_t**)&D.ea_src, pl->allocs)) ||
         (rc = upload(H.gemm, (GemmTask**)&D.gemm, pl->allocs)) ||
         (rc = upload(H.diag, (DiagTask**)&D.diag, pl->allocs)) ||
-        (rc = upload(H.stask, (SolveTask**)&D.stask, pl->allocs)) ||
+        (rc = upload(H.ftask, &pl->d_ftask, pl->allocs)) ||
+        (rc = upload(H.btask, &pl->d_btask, pl->allocs)) ||
+        (rc = upload(H.ntf, &pl->d_ntf, pl->allocs)) ||
+        (rc = upload(H.ntb, &pl->d_ntb, pl->allocs)) ||
         (rc = upload(H.out, (OutFront**)&D.out, pl->allocs)) ||
         (rc = upload(H.septpos, (int32_t**)&D.septpos, pl->allocs)) ||
         (rc = upload(H.mvoff1, (int64_t**)&D.mvoff1, pl->allocs)) ||
-        (rc = upload(pl->in_small, (uint8_t**)&D.in_small, pl->allocs))) {
+        (rc = upload(pl->in_small, (uint8_t**)&D.in_small, pl->allocs)) ||
+        (rc = upload(pl->col2ref, (int32_t**)&D.col2ref, pl->allocs))) {
         delete pl;
         return rc;
     }
@@ -382,6 +371,7 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
               alloc((void**)&f->dref, 8 * (size_t)(H.n + 1)) &&
               alloc((void**)&f->signs, 8 * (size_t)(H.n + 1)) &&
               alloc((void**)&f->info, 64) && alloc((void**)&f->flags, 4 * (H.fronts.size() + 1)) &&
+              alloc((void**)&f->prog, 4 * (H.fronts.size() + 1)) &&
               alloc((void**)&f->ticket, 64) && alloc((void**)&f->hang, 64) &&
               alloc((void**)&f->perm_dev, 8 * (size_t)(H.n + 1));
     if (!ok) {
@@ -610,7 +600,9 @@ int mf_solve_natural(mf_factor* f, double* C, int64_t nrhs, int64_t ldc, int dev
             // permutation by the identity = conversion
             id.resize(H.n);
             for (int64_t i = 0; i < H.n; ++i) id[i] = i + 1;
-            HIPCHK(hipMemcpyAsync(f->perm_dev, id.data(), 8 * H.n, hipMemcpyHostToDevice, f->stream));
+            f->perm_host = id;
+            HIPCHK(hipMemcpyAsync(f->perm_dev, f->perm_host.data(), 8 * H.n, hipMemcpyHostToDevice,
+                                  f->stream));
             launch_permute<float>(src, lds, (float*)f->Bws, H.n, f->perm_dev, (int)H.n, (int)nrhs,
                                   false, f->stream);
             enqueue_solve<float>(f, (float*)f->Bws, H.n, (int)nrhs);
@@ -641,7 +633,12 @@ int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t
     size_t es = elt(f);
     if (ensure(&f->Mv, &f->Mv_bytes, es * (size_t)H.mv_elems1 * nrhs)) return MF_ENOMEM;
     if (ensure(&f->Bws, &f->Bws_bytes, es * (size_t)H.n * nrhs)) return MF_ENOMEM;
-    HIPCHK(hipMemcpyAsync(f->perm_dev, perm, 8 * H.n, hipMemcpyHostToDevice, f->stream));
+    if (f->perm_host.size() != (size_t)H.n ||
+        std::memcmp(f->perm_host.data(), perm, 8 * H.n) != 0) {
+        f->perm_host.assign(perm, perm + H.n);
+        HIPCHK(hipMemcpyAsync(f->perm_dev, f->perm_host.data(), 8 * H.n, hipMemcpyHostToDevice,
+                              f->stream));
+    }
     double* src = B;
     int64_t lds = ldb;
     if (!device) {

@@ -76,6 +76,9 @@ __device__ __forceinline__ void wait_flag(int32_t* flag, int32_t* hang) {
             atomicExch(hang, 1);
             break;
         }
+        // once any wait has given up, every other wait returns at once
+        if ((spins & 255) == 0 && __hip_atomic_load(hang, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            break;
         __builtin_amdgcn_s_sleep(2);
     }
 }
@@ -110,13 +113,13 @@ __device__ void factor_lds(T* M, int nn, int nj, int gc0, const DevFactor<T>& F,
         __syncthreads();
         if (*s_fail) return;
         const T inv = *s_inv;
-        for (int r = k + 1 + tid; r < nj; r += blockDim.x) M[r + k * LDS] *= inv;
+        const int r = tid & 63, cg = tid >> 6;
+        if (tid < 64 && r > k && r < nj) M[r + k * LDS] *= inv;
         __syncthreads();
         const T piv = *s_piv;
-        const int m = nj - k - 1;
-        for (int e = tid; e < m * m; e += blockDim.x) {
-            int r = k + 1 + e % m, c = k + 1 + e / m;
-            if (r >= c) M[r + c * LDS] -= (M[r + k * LDS] * piv) * M[c + k * LDS];
+        if (r < nj) {
+            const T lr = M[r + k * LDS] * piv;
+            for (int c = k + 1 + cg; c <= r; c += 4) M[r + c * LDS] -= lr * M[c + k * LDS];
         }
         __syncthreads();
     }
@@ -135,8 +138,9 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
             if (tid == 0) s_t = (int)atomicAdd(F.ticket, 1u);
             __syncthreads();
             t = s_t;
+            __syncthreads();
         }
-        if (t >= count) return;
+        if (t >= count) break;
         const int J = list[t];
         const Front f = P.fronts[J];
         const int nn = f.nn, na = f.na, nj = nn + na;
@@ -147,44 +151,38 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
         }
         __syncthreads();
         acquire_agent();
-        for (int e = tid; e < nj * LDS; e += blockDim.x) M[e] = (T)0;
+        const int r = tid & 63, cg = tid >> 6;
+        for (int c = cg; c < nj; c += 4) M[r + c * LDS] = (T)0;
         __syncthreads();
         for (int q = f.chd0; q < f.chd1; ++q) {
             const Front kf = P.fronts[P.chd[q]];
             const int ma = kf.na;
             const int32_t* relK = P.rel + kf.reloff;
             const T* UK = F.U + kf.uoff;
-            for (int e = tid; e < ma * ma; e += blockDim.x) {
-                int r = e % ma, c = e / ma;
-                if (r >= c) M[relK[r] + relK[c] * LDS] += UK[e];
+            if (r < ma) {
+                const int rr = relK[r];
+                for (int c = cg; c <= r; c += 4) M[rr + relK[c] * LDS] += UK[r + c * ma];
             }
             __syncthreads();
         }
         T* Pj = F.panel + f.poff;
-        for (int e = tid; e < nj * nn; e += blockDim.x) {
-            int r = e % nj, c = e / nj;
-            if (r >= c) M[r + c * LDS] += Pj[e];
-        }
+        if (r < nj)
+            for (int c = cg; c <= r && c < nn; c += 4) M[r + c * LDS] += Pj[r + c * nj];
         __syncthreads();
         factor_lds<T>(M, nn, nj, f.c0, F, &s_piv, &s_inv, &s_fail);
-        for (int e = tid; e < nj * nn; e += blockDim.x) {
-            int r = e % nj, c = e / nj;
-            if (r >= c) Pj[e] = M[r + c * LDS];
-        }
-        if (na > 0) {
+        if (r < nj)
+            for (int c = cg; c <= r && c < nn; c += 4) Pj[r + c * nj] = M[r + c * LDS];
+        if (na > 0 && r < na) {
             T* Uj = F.U + f.uoff;
-            for (int e = tid; e < na * na; e += blockDim.x) {
-                int r = e % na, c = e / na;
-                if (r >= c) Uj[e] = M[(nn + r) + (nn + c) * LDS];
-            }
+            for (int c = cg; c <= r; c += 4) Uj[r + c * na] = M[(nn + r) + (nn + c) * LDS];
         }
         release_agent();
         __syncthreads();
-        if (persistent) {
-            if (tid == 0) __hip_atomic_store(F.flags + J, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            return;
-        }
+        if (!persistent) break;
+        if (tid == 0) __hip_atomic_store(F.flags + J, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // closes the thread-0 block before the back-edge: every thread of the
+        // workgroup reaches the next ticket together
+        __syncthreads();
     }
 }
 
@@ -229,77 +227,102 @@ __global__ __launch_bounds__(256) void k_ea(DevPlan P, DevFactor<T> F, int t0) {
     }
 }
 
+__device__ __forceinline__ double rdlane(double x, int l) {
+    long long b = __double_as_longlong(x);
+    int lo = __builtin_amdgcn_readlane((int)b, l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float rdlane(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// One wavefront factors and inverts a kb x kb (kb <= 64) diagonal block.
+// Lane i keeps row i of the block in registers; the scaled column k is
+// broadcast through LDS, so every step is 63-k independent FMAs per lane.
+// The factorization is potrf2!'s right-looking column loop
+// (blas/potrf.jl:3-41); the inverse W = L^-1 (unit L for LDLt) is formed by
+// right-looking forward substitution with lane c owning column c, and is
+// stored (ld = nn) for the panel solve and the triangular solves.
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag(DevPlan P, DevFactor<T> F, int t0, int invert_only) {
-    __shared__ T M[NB * (NB + 1)];
-    __shared__ T Wi[NB * (NB + 1)];
-    __shared__ T s_piv, s_inv;
-    __shared__ int s_fail;
+__global__ __launch_bounds__(64) void k_diag(DevPlan P, DevFactor<T> F, int t0, int invert_only) {
+    __shared__ T Tr[NB * (NB + 1)];
+    __shared__ T colk[NB];
     const DiagTask dt = P.diag[t0 + blockIdx.x];
     const Front f = P.fronts[dt.front];
-    const int nj = f.nn + f.na, kb = dt.kb, k0 = dt.k0, tid = threadIdx.x;
-    constexpr int LD = NB + 1;
+    const int nj = f.nn + f.na, kb = dt.kb, k0 = dt.k0, lane = threadIdx.x;
     T* blk = F.panel + f.poff + k0 + (int64_t)k0 * nj;
-    if (tid == 0) s_fail = 0;
-    for (int e = tid; e < kb * kb; e += blockDim.x) {
-        int r = e % kb, c = e / kb;
-        M[r + c * LD] = r >= c ? blk[r + (int64_t)c * nj] : (T)0;
-    }
-    __syncthreads();
-    for (int k = 0; k < (invert_only ? 0 : kb); ++k) {
-        if (tid == 0) {
-            const int g = f.c0 + k0 + k;
-            double piv = regularize(F, sign_of(F, g), (double)M[k + k * LD]);
-            if (piv == 0.0) {
-                s_fail = 1;
-                atomicMin(F.info, (unsigned long long)(g + 1));
-            } else if (F.diag == 0) {
-                T r = (T)sqrt(piv);
-                M[k + k * LD] = r;
-                s_piv = (T)1;
-                s_inv = (T)1 / r;
-            } else {
-                F.d[g] = (T)piv;
-                s_piv = (T)piv;
-                s_inv = (T)1 / (T)piv;
+    T a[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        a[c] = (lane < kb && c < kb && c <= lane) ? blk[lane + (int64_t)c * nj] : (T)0;
+    if (!invert_only) {
+        bool failed = false;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            if (k < kb) {
+                const int g = f.c0 + k0 + k;
+                const T akk = rdlane(a[k], k);
+                double piv = regularize(F, sign_of(F, g), (double)akk);
+                if (!failed && piv == 0.0) {
+                    failed = true;
+                    if (lane == 0) atomicMin(F.info, (unsigned long long)(g + 1));
+                }
+                if (piv == 0.0) piv = 1.0;  // keeps the rest finite after a failure
+                T dk, fm;
+                if (F.diag == 0) {
+                    dk = (T)sqrt(piv);
+                    fm = (T)1;
+                    if (lane == k) a[k] = dk;
+                } else {
+                    dk = (T)piv;
+                    fm = dk;
+                    if (lane == 0 && !failed) F.d[g] = dk;
+                }
+                const T inv = (T)1 / dk;
+                const T lik = (lane > k) ? a[k] * inv : (T)0;
+                if (lane > k) a[k] = lik;
+                colk[lane] = lik;
+                __syncthreads();
+                const T lf = lik * fm;
+#pragma unroll
+                for (int j = k + 1; j < NB; ++j) {
+                    const T ljk = colk[j];
+                    if (lane >= j) a[j] -= lf * ljk;
+                }
+                __syncthreads();
             }
         }
-        __syncthreads();
-        if (s_fail) break;
-        const T inv = s_inv;
-        for (int r = k + 1 + tid; r < kb; r += blockDim.x) M[r + k * LD] *= inv;
-        __syncthreads();
-        const T piv = s_piv;
-        const int m = kb - k - 1;
-        for (int e = tid; e < m * m; e += blockDim.x) {
-            int r = k + 1 + e % m, c = k + 1 + e / m;
-            if (r >= c) M[r + c * LD] -= (M[r + k * LD] * piv) * M[c + k * LD];
-        }
-        __syncthreads();
-    }
-    if (!invert_only) {
-        for (int e = tid; e < kb * kb; e += blockDim.x) {
-            int r = e % kb, c = e / kb;
-            if (r >= c) blk[r + (int64_t)c * nj] = M[r + c * LD];
+        if (lane < kb) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < kb && c <= lane) blk[lane + (int64_t)c * nj] = a[c];
         }
     }
-    // inverse of the (unit, for LDLt) lower triangular block, one column per thread
-    if (tid < kb) {
-        const int c = tid;
-        const bool unit = F.diag != 0;
-        for (int i = 0; i < c; ++i) Wi[i + c * LD] = (T)0;
-        Wi[c + c * LD] = unit ? (T)1 : (T)1 / M[c + c * LD];
-        for (int i = c + 1; i < kb; ++i) {
-            T s = (T)0;
-            for (int k = c; k < i; ++k) s += M[i + k * LD] * Wi[k + c * LD];
-            Wi[i + c * LD] = unit ? -s : -s / M[i + i * LD];
+    // W = L^-1, lane c owns column c: w[i] = W[i][c]
+    const bool unit = F.diag != 0;
+    T w[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) w[i] = (lane == i) ? (T)1 : (T)0;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        if (k < kb) {
+            if (!unit) w[k] *= (T)1 / rdlane(a[k], k);
+            colk[lane] = a[k];  // column k of L (lane i holds L[i][k])
+            __syncthreads();
+            const T wk = w[k];
+#pragma unroll
+            for (int i = k + 1; i < NB; ++i) w[i] -= colk[i] * wk;
+            __syncthreads();
         }
     }
+    // transpose through LDS so that the column-major stores are coalesced
+#pragma unroll
+    for (int i = 0; i < NB; ++i) Tr[i * (NB + 1) + lane] = (i < kb) ? w[i] : (T)0;
     __syncthreads();
     T* W = F.W + f.woff + k0;
-    for (int e = tid; e < kb * kb; e += blockDim.x) {
-        int r = e % kb, c = e / kb;
-        W[r + (int64_t)c * f.nn] = Wi[r + c * LD];
+    if (lane < kb) {
+        for (int c = 0; c < kb; ++c) W[lane + (int64_t)c * f.nn] = Tr[lane * (NB + 1) + c];
     }
 }
 
@@ -389,22 +412,20 @@ __global__ void k_fill(T* p, int64_t count, T value) {
     for (; i < count; i += stride) p[i] = value;
 }
 
-// copyto!-layout Dval/Lval (assembled A) -> internal panels (zeroed before).
+// copyto!-layout Dval/Lval (assembled A) -> internal panels (zeroed before);
+// one workgroup per global column.
 template <typename T>
 __global__ __launch_bounds__(256) void k_assemble_ref(DevPlan P, DevFactor<T> F,
                                                       const double* Dval, const double* Lval) {
-    const OutFront o = P.out[blockIdx.x];
+    const int g = blockIdx.x;
+    const OutFront o = P.out[P.col2ref[g]];
     const Front f = P.fronts[o.J];
-    const int nj = f.nn + f.na, cl = o.r0 - f.c0;
-    T* Pj = F.panel + f.poff;
-    for (int e = threadIdx.x; e < o.nn * o.nn; e += blockDim.x) {
-        int r = e % o.nn, c = e / o.nn;
-        if (r >= c) Pj[(cl + r) + (int64_t)(cl + c) * nj] = (T)Dval[o.Dp + e];
-    }
-    for (int e = threadIdx.x; e < o.na * o.nn; e += blockDim.x) {
-        int i = e % o.na, c = e / o.na;
-        Pj[P.septpos[o.sepoff + i] + (int64_t)(cl + c) * nj] = (T)Lval[o.Lp + e];
-    }
+    const int nj = f.nn + f.na, c = g - o.r0, cl = o.r0 - f.c0;
+    T* col = F.panel + f.poff + (int64_t)(cl + c) * nj;
+    for (int r = c + threadIdx.x; r < o.nn; r += blockDim.x)
+        col[cl + r] = (T)Dval[o.Dp + r + (int64_t)c * o.nn];
+    for (int i = threadIdx.x; i < o.na; i += blockDim.x)
+        col[P.septpos[o.sepoff + i]] = (T)Lval[o.Lp + i + (int64_t)c * o.na];
 }
 
 // setflatindex! for every nonzero: panel slot and reference slot.
@@ -423,29 +444,25 @@ __global__ void k_refresh(DevFactor<T> F, const int64_t* tp, const int64_t* tr, 
     }
 }
 
+// Internal panels -> the reference's Dval/Lval/d; one workgroup per global column.
 template <typename T>
 __global__ __launch_bounds__(256) void k_copyout(DevPlan P, DevFactor<T> F, double* Dval,
                                                  double* Lval, double* dout) {
-    const OutFront o = P.out[blockIdx.x];
+    const int g = blockIdx.x;
+    const OutFront o = P.out[P.col2ref[g]];
     const Front f = P.fronts[o.J];
-    const int nj = f.nn + f.na, cl = o.r0 - f.c0;
-    const T* Pj = F.panel + f.poff;
+    const int nj = f.nn + f.na, c = g - o.r0, cl = o.r0 - f.c0;
+    const T* col = F.panel + f.poff + (int64_t)(cl + c) * nj;
     const bool keep_diag = (F.diag != 0 && o.nn == 1);  // chol_loop_nod!: Dval keeps A's entry
     if (Dval && !keep_diag) {
-        for (int e = threadIdx.x; e < o.nn * o.nn; e += blockDim.x) {
-            int r = e % o.nn, c = e / o.nn;
-            Dval[o.Dp + e] = r >= c ? (double)Pj[(cl + r) + (int64_t)(cl + c) * nj] : 0.0;
-        }
+        for (int r = threadIdx.x; r < o.nn; r += blockDim.x)
+            Dval[o.Dp + r + (int64_t)c * o.nn] = r >= c ? (double)col[cl + r] : 0.0;
     }
     if (Lval) {
-        for (int e = threadIdx.x; e < o.na * o.nn; e += blockDim.x) {
-            int i = e % o.na, c = e / o.na;
-            Lval[o.Lp + e] = (double)Pj[P.septpos[o.sepoff + i] + (int64_t)(cl + c) * nj];
-        }
+        for (int i = threadIdx.x; i < o.na; i += blockDim.x)
+            Lval[o.Lp + i + (int64_t)c * o.na] = (double)col[P.septpos[o.sepoff + i]];
     }
-    if (dout && F.diag != 0) {
-        for (int c = threadIdx.x; c < o.nn; c += blockDim.x) dout[o.r0 + c] = (double)F.d[o.r0 + c];
-    }
+    if (dout && F.diag != 0 && threadIdx.x == 0) dout[g] = (double)F.d[g];
 }
 
 inline int grid_for(int64_t count, int block) {
@@ -465,8 +482,8 @@ void launch_fill(T* p, int64_t count, T value, hipStream_t s) {
 template <typename T>
 void launch_assemble_ref(const DevPlan& P, const DevFactor<T>& F, const double* Dval,
                          const double* Lval, int nfr, hipStream_t s) {
-    if (nfr <= 0) return;
-    hipLaunchKernelGGL(k_assemble_ref<T>, dim3(nfr), dim3(256), 0, s, P, F, Dval, Lval);
+    if (nfr <= 0 || P.n <= 0) return;
+    hipLaunchKernelGGL(k_assemble_ref<T>, dim3(P.n), dim3(256), 0, s, P, F, Dval, Lval);
 }
 
 template <typename T>
@@ -497,7 +514,7 @@ template <typename T>
 void launch_diag(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, bool invert_only,
                  hipStream_t s) {
     if (t1 <= t0) return;
-    hipLaunchKernelGGL(k_diag<T>, dim3(t1 - t0), dim3(256), 0, s, P, F, t0, invert_only ? 1 : 0);
+    hipLaunchKernelGGL(k_diag<T>, dim3(t1 - t0), dim3(64), 0, s, P, F, t0, invert_only ? 1 : 0);
 }
 
 template <typename T>
@@ -509,9 +526,8 @@ void launch_gemm(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, hipStr
 template <typename T>
 void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, double* Lval,
                     double* dout, int nfr, int n, hipStream_t s) {
-    (void)n;
-    if (nfr <= 0) return;
-    hipLaunchKernelGGL(k_copyout<T>, dim3(nfr), dim3(256), 0, s, P, F, Dval, Lval, dout);
+    if (nfr <= 0 || n <= 0) return;
+    hipLaunchKernelGGL(k_copyout<T>, dim3(n), dim3(256), 0, s, P, F, Dval, Lval, dout);
 }
 
 #define MF_INST(T)                                                                              \

@@ -24,11 +24,11 @@ struct DevPlan {
     const int32_t* ea_src;
     const GemmTask* gemm;
     const DiagTask* diag;
-    const SolveTask* stask;
     const OutFront* out;
     const int32_t* septpos;
     const int64_t* mvoff1;
     const uint8_t* in_small;     // 1: front of the small region
+    const int32_t* col2ref;      // global column -> reference front
     int32_t nJ;
     int32_t n;
     int32_t nfr;
@@ -85,29 +85,12 @@ struct DevSolve {
     T* Mv;       // update vectors, na_J x nrhs at mvoff1[J]*nrhs
     int nrhs;
 };
+// Forward (children first) and backward (parents first) persistent solves,
+// the D^-1 scaling in between for LDLt.
 template <typename T>
-void launch_solve_small_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
-                            const int32_t* list, int count, bool persistent, hipStream_t s);
-template <typename T>
-void launch_solve_small_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
-                            const int32_t* list, int count, bool persistent, hipStream_t s);
-template <typename T>
-void launch_sgather(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
-                    int t1, hipStream_t s);
-template <typename T>
-void launch_sdiag(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
-                  int t1, bool backward, hipStream_t s);
-template <typename T>
-void launch_supd_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
-                     int t1, hipStream_t s);
-template <typename T>
-void launch_sbgather(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
-                     int t1, hipStream_t s);
-template <typename T>
-void launch_supd_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int t0,
-                     int t1, hipStream_t s);
-template <typename T>
-void launch_dscale(const DevFactor<T>& F, const DevSolve<T>& S, int n, hipStream_t s);
+void launch_solve(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int nJ,
+                  int32_t* done, int32_t* prog, const int32_t* ntf, const int32_t* ntb,
+                  const PTask* ftask, int nf, const PTask* btask, int nb, int n, hipStream_t s);
 template <typename TB>
 void launch_permute(const double* src, int64_t lds, TB* dst, int64_t ldd, const int64_t* perm,
                     int n, int nrhs, bool inverse, hipStream_t s);

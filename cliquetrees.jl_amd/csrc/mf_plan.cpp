@@ -329,7 +329,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     P.ea_src.clear();
     P.gemm.clear();
     P.diag.clear();
-    P.stask.clear();
+
     auto add_gemm = [&](int8_t mode, int8_t aA, int64_t ao, int32_t lda, int8_t aB, int64_t bo,
                         int32_t ldb, int8_t aC, int64_t co, int32_t ldc, int32_t m, int32_t nn_,
                         int32_t k, int32_t scol, int8_t lower) {
@@ -457,71 +457,65 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             }
         }
         L.syrk1 = (int32_t)P.gemm.size();
+    }
 
-        // solve tasks: forward gather (row chunks of EA_CHUNK)
-        L.sea0 = (int32_t)P.stask.size();
+    // ---- persistent solve task lists ------------------------------------------
+    // Forward: children before parents; inside a big level the diagonal-block
+    // chains of all fronts are interleaved by block, then the update rows.
+    // Backward: the reverse topological order, chains by decreasing block.
+    P.ftask.clear();
+    P.btask.clear();
+    P.ntf.assign(NJ, 0);
+    P.ntb.assign(NJ, 0);
+    for (int32_t J : P.small_order) {
+        P.ftask.push_back(PTask{PT_SMALL, J, 0, 0});
+        P.ntf[J] = 1;
+    }
+    auto nblk = [&](int32_t J) { return (P.fronts[J].nn + NB - 1) / NB; };
+    for (int32_t l = 0; l < nlev; ++l) {
+        const Level& L = P.levels[l];
+        for (int32_t J : L.small_list) {
+            P.ftask.push_back(PTask{PT_SMALL, J, 0, 0});
+            P.ntf[J] = 1;
+        }
+        int32_t maxp = 0;
+        for (int32_t J : L.big_list) maxp = std::max(maxp, nblk(J));
+        for (int32_t k = 0; k < maxp; ++k)
+            for (int32_t J : L.big_list)
+                if (k < nblk(J)) {
+                    const Front& F = P.fronts[J];
+                    P.ftask.push_back(PTask{PT_DIAG, J, k * NB, std::min(NB, F.nn - k * NB)});
+                    P.ntf[J]++;
+                }
         for (int32_t J : L.big_list) {
             const Front& F = P.fronts[J];
-            int32_t nj = F.nn + F.na;
-            for (int32_t r = 0; r < nj; r += EA_CHUNK)
-                P.stask.push_back(SolveTask{J, 0, 0, r, std::min(nj, r + EA_CHUNK)});
-        }
-        L.sea1 = (int32_t)P.stask.size();
-        L.fsteps.assign(maxp, SolveStep());
-        for (int32_t s = 0; s < maxp; ++s) {
-            SolveStep& st = L.fsteps[s];
-            st.t0 = (int32_t)P.stask.size();
-            for (int32_t J : L.big_list) {
-                const Front& F = P.fronts[J];
-                int32_t k0 = s * NB;
-                if (k0 >= F.nn) continue;
-                int32_t kb = std::min(NB, F.nn - k0);
-                P.stask.push_back(SolveTask{J, k0, kb, k0, k0 + kb});
+            for (int32_t r = F.nn; r < F.nn + F.na; r += NB) {
+                P.ftask.push_back(PTask{PT_ROWS, J, r, std::min(F.nn + F.na, r + NB)});
+                P.ntf[J]++;
             }
-            st.t1 = (int32_t)P.stask.size();
-            st.u0 = (int32_t)P.stask.size();
-            for (int32_t J : L.big_list) {
-                const Front& F = P.fronts[J];
-                int32_t nj = F.nn + F.na, k0 = s * NB;
-                if (k0 >= F.nn) continue;
-                int32_t kb = std::min(NB, F.nn - k0);
-                for (int32_t r = k0 + kb; r < nj; r += NB)
-                    P.stask.push_back(SolveTask{J, k0, kb, r, std::min(nj, r + NB)});
-            }
-            st.u1 = (int32_t)P.stask.size();
         }
-        // backward: L' x_sep gather over column chunks, then panels in reverse
-        L.bg0 = (int32_t)P.stask.size();
-        for (int32_t J : L.big_list) {
-            const Front& F = P.fronts[J];
-            for (int32_t c = 0; c < F.nn; c += NB)
-                P.stask.push_back(SolveTask{J, 0, 0, c, std::min(F.nn, c + NB)});
+    }
+    for (int32_t l = nlev - 1; l >= 0; --l) {
+        const Level& L = P.levels[l];
+        for (int32_t J : L.small_list) {
+            P.btask.push_back(PTask{PT_SMALL, J, 0, 0});
+            P.ntb[J] = 1;
         }
-        L.bg1 = (int32_t)P.stask.size();
-        L.bsteps.clear();
-        for (int32_t s = maxp - 1; s >= 0; --s) {
-            SolveStep st{};
-            st.t0 = (int32_t)P.stask.size();
+        int32_t maxp = 0;
+        for (int32_t J : L.big_list) maxp = std::max(maxp, nblk(J));
+        for (int32_t t = 0; t < maxp; ++t)
             for (int32_t J : L.big_list) {
-                const Front& F = P.fronts[J];
-                int32_t k0 = s * NB;
-                if (k0 >= F.nn) continue;
-                int32_t kb = std::min(NB, F.nn - k0);
-                P.stask.push_back(SolveTask{J, k0, kb, k0, k0 + kb});
+                int32_t p = nblk(J), k = p - 1 - t;
+                if (k >= 0) {
+                    const Front& F = P.fronts[J];
+                    P.btask.push_back(PTask{PT_DIAG, J, k * NB, std::min(NB, F.nn - k * NB)});
+                    P.ntb[J]++;
+                }
             }
-            st.t1 = (int32_t)P.stask.size();
-            st.u0 = (int32_t)P.stask.size();
-            for (int32_t J : L.big_list) {
-                const Front& F = P.fronts[J];
-                int32_t k0 = s * NB;
-                if (k0 >= F.nn) continue;
-                int32_t kb = std::min(NB, F.nn - k0);
-                for (int32_t c = 0; c < k0; c += NB)
-                    P.stask.push_back(SolveTask{J, k0, kb, c, std::min(k0, c + NB)});
-            }
-            st.u1 = (int32_t)P.stask.size();
-            L.bsteps.push_back(st);
-        }
+    }
+    for (auto it = P.small_order.rbegin(); it != P.small_order.rend(); ++it) {
+        P.btask.push_back(PTask{PT_SMALL, *it, 0, 0});
+        P.ntb[*it] = 1;
     }
 
     // ---- bounds validation of every task against the pools ----------------------
@@ -572,11 +566,52 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 if (r < 0 || r >= Pf.nn + Pf.na) return "rel out of range";
             }
         }
-        for (const SolveTask& t : P.stask) {
-            const Front& F = P.fronts[t.front];
-            int32_t nj = F.nn + F.na;
-            if (t.r0 < 0 || t.r1 > nj || t.r1 <= t.r0 || t.r1 - t.r0 > EA_CHUNK) return "solve task";
-            if (t.kb > 0 && (t.k0 + t.kb > F.nn || F.woff < 0 || t.r1 - t.r0 > NB)) return "solve panel";
+        for (const std::vector<PTask>* tl : {&P.ftask, &P.btask}) {
+            if ((int64_t)tl->size() != (int64_t)P.small_order.size() + [&] {
+                    int64_t c = 0;
+                    for (const Level& L : P.levels) c += (int64_t)L.small_list.size();
+                    return c;
+                }() + [&] {
+                    int64_t c = 0;
+                    for (const PTask& t : *tl) c += t.kind != PT_SMALL;
+                    return c;
+                }())
+                return "solve task count";
+            for (const PTask& t : *tl) {
+                const Front& F = P.fronts[t.front];
+                int32_t nj = F.nn + F.na;
+                if (t.kind == PT_SMALL) {
+                    if (nj > SMALL_NJ) return "small solve task on a big front";
+                } else if (t.kind == PT_DIAG) {
+                    if (F.woff < 0 || t.a < 0 || t.b <= 0 || t.b > NB || t.a + t.b > F.nn || t.a % NB)
+                        return "diag solve task";
+                } else if (t.kind == PT_ROWS) {
+                    if (t.a < F.nn || t.b > nj || t.b <= t.a || t.b - t.a > NB) return "rows solve task";
+                } else {
+                    return "solve task kind";
+                }
+            }
+        }
+        // every dependency points to an earlier ticket
+        {
+            std::vector<int32_t> first(NJ, -1), last(NJ, -1);
+            for (int32_t i = 0; i < (int32_t)P.ftask.size(); ++i) {
+                int32_t J = P.ftask[i].front;
+                if (first[J] < 0) first[J] = i;
+                last[J] = i;
+            }
+            for (int32_t J = 0; J < NJ; ++J)
+                for (int32_t q = P.fronts[J].chd0; q < P.fronts[J].chd1; ++q)
+                    if (last[P.chd[q]] >= first[J]) return "forward solve order";
+            std::fill(first.begin(), first.end(), -1);
+            for (int32_t i = 0; i < (int32_t)P.btask.size(); ++i) {
+                int32_t J = P.btask[i].front;
+                if (first[J] < 0) first[J] = i;
+                last[J] = i;
+            }
+            for (int32_t J = 0; J < NJ; ++J)
+                if (P.fronts[J].parent >= 0 && last[P.fronts[J].parent] >= first[J])
+                    return "backward solve order";
         }
     }
 
@@ -610,7 +645,7 @@ extern "C" int mf_plan_check(int64_t n, int64_t nfr, const int64_t* resptr, cons
     }
     int64_t v[] = {(int64_t)P.fronts.size(), P.n_small_region, P.n_big, (int64_t)P.levels.size(),
                    P.panel_elems, P.u_elems, P.w_elems, (int64_t)P.gemm.size(),
-                   (int64_t)P.ea.size(), (int64_t)P.stask.size(), (int64_t)P.ref_flops,
+                   (int64_t)P.ea.size(), (int64_t)(P.ftask.size() + P.btask.size()), (int64_t)P.ref_flops,
                    (int64_t)P.engine_flops, P.max_nj};
     for (int64_t i = 0; stats && i < nstats && i < (int64_t)(sizeof(v) / sizeof(v[0])); ++i)
         stats[i] = v[i];

@@ -16,16 +16,13 @@ struct Step {            // one panel step of a big level
     int32_t trail0, trail1;  // GemmTask range (trailing update inside the panel)
 };
 
-struct SolveStep {       // one panel step of the big-front solves
-    int32_t t0, t1;          // SolveTask range (diag block rows)
-    int32_t u0, u1;          // SolveTask range (update rows / columns)
-};
-
-// Row-block task of the big-front solves.
-struct SolveTask {
+// Task of the persistent triangular solves (one launch per direction).
+enum : int32_t { PT_SMALL = 0, PT_DIAG = 1, PT_ROWS = 2 };
+struct PTask {
+    int32_t kind;    // PT_SMALL: a whole small front; PT_DIAG: diagonal block rows
+                     // [a, a+b) of a big front; PT_ROWS: update rows [a, b)
     int32_t front;
-    int32_t k0, kb;      // panel (local columns)
-    int32_t r0, r1;      // local rows (forward) or local columns (backward)
+    int32_t a, b;
 };
 
 struct Level {
@@ -36,11 +33,6 @@ struct Level {
     int32_t ea0 = 0, ea1 = 0;          // EaTask range
     std::vector<Step> steps;
     int32_t syrk0 = 0, syrk1 = 0;
-    // solves
-    int32_t sea0 = 0, sea1 = 0;        // forward gather tasks (SolveTask, r-chunks)
-    std::vector<SolveStep> fsteps;     // forward panel steps
-    int32_t bg0 = 0, bg1 = 0;          // backward L' x_sep tasks
-    std::vector<SolveStep> bsteps;     // backward panel steps (already reversed)
 };
 
 struct HostPlan {
@@ -62,7 +54,8 @@ struct HostPlan {
     std::vector<int32_t> ea_src;       // pairs (child, child column)
     std::vector<GemmTask> gemm;
     std::vector<DiagTask> diag;
-    std::vector<SolveTask> stask;
+    std::vector<PTask> ftask, btask;   // forward / backward solve tasks, ticket order
+    std::vector<int32_t> ntf, ntb;     // tasks per front (completion counts)
     std::vector<int64_t> mvoff1;       // per internal front: offset of Mv (na x 1)
     // copy-out
     std::vector<OutFront> out;

@@ -205,16 +205,37 @@ static void send_tri(double* F, i64 ldf, const double* M, i64 na, const i64* inj
  * Mval: workspace of nMval doubles; Mptr: nMptr+1 entries; Fval: nFval^2.
  * Returns 0, or the first failing column in permuted order (1-based).
  */
+i64 or_chol_range(i64 n, i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt,
+                  const i64* chdptr, const i64* chdtgt, const i64* Dptr, const i64* Lptr,
+                  double* Dval, double* Lval, double* d, const double* signs, int diag,
+                  int reg_kind, double delta, double eps, double* Mval, i64* Mptr,
+                  double* Fval, i64 j0, i64 j1);
+
 i64 or_chol_impl(i64 n, i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt,
                  const i64* chdptr, const i64* chdtgt, const i64* Dptr, const i64* Lptr,
                  double* Dval, double* Lval, double* d, const double* signs, int diag,
                  int reg_kind, double delta, double eps, double* Mval, i64* Mptr,
                  double* Fval) {
+    return or_chol_range(n, nfr, resptr, sepptr, reltgt, chdptr, chdtgt, Dptr, Lptr, Dval, Lval,
+                         d, signs, diag, reg_kind, delta, eps, Mval, Mptr, Fval, 0, nfr);
+}
+
+/*
+ * The front loop over fronts [j0, j1) only.  When [j0, j1) is a whole
+ * subtree of the postorder (a front and all its descendants) the loop is
+ * self-contained; the update matrix of its root is left on the stack.
+ */
+i64 or_chol_range(i64 n, i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt,
+                  const i64* chdptr, const i64* chdtgt, const i64* Dptr, const i64* Lptr,
+                  double* Dval, double* Lval, double* d, const double* signs, int diag,
+                  int reg_kind, double delta, double eps, double* Mval, i64* Mptr,
+                  double* Fval, i64 j0, i64 j1) {
     reg_t R = {reg_kind, delta, eps};
     (void)n;
+    (void)nfr;
     i64 ns = 0;
     Mptr[1] = 0; /* 1-based stack index, 0-based offsets */
-    for (i64 j = 0; j < nfr; ++j) {
+    for (i64 j = j0; j < j1; ++j) {
         i64 r0 = resptr[j] - 1;
         i64 nn = resptr[j + 1] - resptr[j];
         i64 na = sepptr[j + 1] - sepptr[j];

@@ -41,6 +41,8 @@ def lib():
                                    [ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, f64p, i64p, f64p])
         L.or_chol_impl.restype = ctypes.c_int64
+        L.or_chol_range.argtypes = L.or_chol_impl.argtypes + [ctypes.c_int64, ctypes.c_int64]
+        L.or_chol_range.restype = ctypes.c_int64
         for f in (L.or_div_fwd, L.or_div_bwd):
             f.argtypes = ([ctypes.c_int64] + [i64p] * 7 + [f64p] * 2 + [ctypes.c_int, f64p,
                           ctypes.c_int64, ctypes.c_int64, f64p, i64p, f64p])
@@ -139,6 +141,35 @@ def chol_impl(S, Dval, Lval, diag=0, signs=None, reg=None):
                               _p(d, f64p), _p(signs, f64p), diag, kind, delta, eps,
                               _p(Mval, f64p), _p(Mptr, i64p), _p(Fval, f64p))
     return Dval, Lval, d, int(info)
+
+
+def chol_subtree(S, Dval, Lval, root):
+    """chol_impl!'s loop restricted to the subtree of front ``root`` (1-based):
+    returns (seconds, reference flops of the subtree, info)."""
+    import time
+    nfr = S.nfr
+    size = np.ones(nfr, dtype=np.int64)
+    for j in range(nfr):
+        p = S.pnt[j] - 1
+        if p >= 0:
+            size[p] += size[j]
+    j1 = root
+    j0 = root - size[root - 1]
+    nn = S.nn()[j0:j1].astype(np.float64)
+    na = S.na()[j0:j1].astype(np.float64)
+    flops = float((nn ** 3 / 3 + na * nn ** 2 + na ** 2 * nn).sum())
+    Dval = np.array(Dval, dtype=np.float64, copy=True)
+    Lval = np.array(Lval, dtype=np.float64, copy=True)
+    d = np.ones(S.n)
+    signs = np.ones(S.n)
+    Mval = np.zeros(max(S.nMval, 1))
+    Mptr = np.zeros(S.nMptr + 2, dtype=np.int64)
+    Fval = np.zeros(max(S.nFval * S.nFval, 2))
+    t0 = time.perf_counter()
+    info = lib().or_chol_range(S.n, S.nfr, *_sym_args(S), _p(Dval, f64p), _p(Lval, f64p),
+                               _p(d, f64p), _p(signs, f64p), 0, 0, 0.0, 0.0,
+                               _p(Mval, f64p), _p(Mptr, i64p), _p(Fval, f64p), j0, j1)
+    return time.perf_counter() - t0, flops, int(info)
 
 
 def div_natural(S, Dval, Lval, d, C, diag=0):
