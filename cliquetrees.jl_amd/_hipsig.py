@@ -30,6 +30,8 @@ SIGNATURES = {
     "mf_div_impl": ([vp, f64p, f64p, vp, f64p, i64, i64, ci], ci),
     "mf_timing": ([vp, ci], ci),
     "mf_timing_read": ([vp, f64p, i64p, f64p, ci], ci),
+    "mf_dag_profile": ([vp, ci], ci),
+    "mf_dag_profile_read": ([vp, f64p, ci], ci),
 }
 
 # Names of the C-ABI entry points declared in include/mfhip.h.

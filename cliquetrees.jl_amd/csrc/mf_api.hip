@@ -56,6 +56,10 @@ struct mf_plan {
     PTask* d_btask = nullptr;
     int32_t* d_ntf = nullptr;
     int32_t* d_ntb = nullptr;
+    DagTask* d_dag = nullptr;
+    DagFront* d_dagf = nullptr;
+    int32_t* d_uwait_ptr = nullptr;
+    int32_t* d_uwait = nullptr;
     double ea_bytes = 0, small_flops = 0, diag_flops = 0;
     ~mf_plan() {
         for (void* p : allocs) (void)hipFree(p);
@@ -84,6 +88,10 @@ struct mf_factor {
     int32_t* prog = nullptr;
     uint32_t* ticket = nullptr;
     int32_t* hang = nullptr;
+    int32_t* cnt = nullptr;     // tile-DAG counters
+    int32_t* eacnt = nullptr;
+    unsigned long long* prof = nullptr;  // task-kind profile of the tile-DAG (NULL: off)
+    unsigned long long* prof_buf = nullptr;
     int64_t* tgt_panel = nullptr;
     int64_t* tgt_ref = nullptr;
     int64_t nnz = -1;
@@ -207,25 +215,13 @@ void enqueue_factor(mf_factor* f) {
     timed(f, MF_KC_SMALL, pl->small_flops, H.n_small_region, [&] {
         launch_small<T>(P, F, P.small_order, (int)H.n_small_region, true, s);
     });
-    for (const Level& L : H.levels) {
-        timed(f, MF_KC_SMALL, 0, L.smallL1 - L.smallL0, [&] {
-            launch_small<T>(P, F, P.level_small + L.smallL0, L.smallL1 - L.smallL0, false, s);
-        });
-        timed(f, MF_KC_EA, 0, L.ea1 - L.ea0, [&] { launch_ea<T>(P, F, L.ea0, L.ea1, s); });
-        for (const Step& st : L.steps) {
-            timed(f, MF_KC_DIAG, 0, st.diag1 - st.diag0,
-                  [&] { launch_diag<T>(P, F, st.diag0, st.diag1, false, s); });
-            timed(f, MF_KC_GEMM, 0, st.trsm1 - st.trsm0,
-                  [&] { launch_gemm<T>(P, F, st.trsm0, st.trsm1, s); });
-            timed(f, MF_KC_GEMM, 0, st.trail1 - st.trail0,
-                  [&] { launch_gemm<T>(P, F, st.trail0, st.trail1, s); });
-        }
-        timed(f, MF_KC_GEMM, 0, L.syrk1 - L.syrk0, [&] { launch_gemm<T>(P, F, L.syrk0, L.syrk1, s); });
-    }
-    if (f->timing) {
-        f->t_work[MF_KC_GEMM] += H.gemm_flops;
-        f->t_work[MF_KC_EA] += pl->ea_bytes;
-        f->t_work[MF_KC_DIAG] += pl->diag_flops;
+    if (!H.dag.empty()) {
+        launch_fill<int32_t>(f->cnt, H.dag_counters, 0, s);
+        launch_fill<int32_t>(f->eacnt, std::max<int>(P.nJ, 1), 0, s);
+        launch_fill<uint32_t>(f->ticket, 1, 0u, s);
+        DevDag D{pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->prof,
+                 (int32_t)H.dag.size()};
+        timed(f, MF_KC_DAG, H.engine_flops - pl->small_flops, 1, [&] { launch_dag<T>(P, F, D, s); });
     }
     timed(f, MF_KC_COPYOUT, 8.0 * 2.0 * (double)(H.ndz + H.nlz), H.nfr, [&] {
         launch_copyout<T>(P, F, f->Dref, f->Lref, f->dref, (int)H.nfr, (int)H.n, s);
@@ -298,12 +294,18 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
         (rc = upload(H.level_big, (int32_t**)&D.level_big, pl->allocs)) ||
         (rc = upload(H.ea, (EaTask**)&D.ea, pl->allocs)) ||
         (rc = upload(ea_src, (int32_t**)&D.ea_src, pl->allocs)) ||
+        (rc = upload(H.wstart, (int32_t**)&D.wstart, pl->allocs)) ||
+        (rc = upload(H.wsoff, (int32_t**)&D.wsoff, pl->allocs)) ||
         (rc = upload(H.gemm, (GemmTask**)&D.gemm, pl->allocs)) ||
         (rc = upload(H.diag, (DiagTask**)&D.diag, pl->allocs)) ||
         (rc = upload(H.ftask, &pl->d_ftask, pl->allocs)) ||
         (rc = upload(H.btask, &pl->d_btask, pl->allocs)) ||
         (rc = upload(H.ntf, &pl->d_ntf, pl->allocs)) ||
         (rc = upload(H.ntb, &pl->d_ntb, pl->allocs)) ||
+        (rc = upload(H.dag, &pl->d_dag, pl->allocs)) ||
+        (rc = upload(H.dagf, &pl->d_dagf, pl->allocs)) ||
+        (rc = upload(H.uwait_ptr, &pl->d_uwait_ptr, pl->allocs)) ||
+        (rc = upload(H.uwait, &pl->d_uwait, pl->allocs)) ||
         (rc = upload(H.out, (OutFront**)&D.out, pl->allocs)) ||
         (rc = upload(H.septpos, (int32_t**)&D.septpos, pl->allocs)) ||
         (rc = upload(H.mvoff1, (int64_t**)&D.mvoff1, pl->allocs)) ||
@@ -373,6 +375,8 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
               alloc((void**)&f->info, 64) && alloc((void**)&f->flags, 4 * (H.fronts.size() + 1)) &&
               alloc((void**)&f->prog, 4 * (H.fronts.size() + 1)) &&
               alloc((void**)&f->ticket, 64) && alloc((void**)&f->hang, 64) &&
+              alloc((void**)&f->cnt, 4 * (size_t)(H.dag_counters + 1)) &&
+              alloc((void**)&f->eacnt, 4 * (H.fronts.size() + 1)) &&
               alloc((void**)&f->perm_dev, 8 * (size_t)(H.n + 1));
     if (!ok) {
         delete f;
@@ -695,11 +699,11 @@ int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const dou
                 rc = fail(MF_EHIP, "hipMemcpy d");
         }
         // the big-front solves use the inverted 64x64 diagonal blocks
-        DevFactor<double> F = dev_factor<double>(f);
-        for (const Level& L : plan->H.levels)
-            for (const Step& st : L.steps)
-                launch_diag<double>(plan->D, F, st.diag0, st.diag1, true, f->stream);
-        if (!rc) rc = mf_solve_natural(f, C, nrhs, ldc, 0);
+        if (!rc) {
+            DevFactor<double> F = dev_factor<double>(f);
+            launch_diag<double>(plan->D, F, 0, (int)plan->H.diag.size(), true, f->stream);
+            rc = mf_solve_natural(f, C, nrhs, ldc, 0);
+        }
     }
     mf_factor_destroy(f);
     return rc;
@@ -708,6 +712,39 @@ int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const dou
 int mf_timing(mf_factor* f, int enable) {
     if (!f) return fail(MF_EARG, "null factor");
     f->timing = enable != 0;
+    return MF_OK;
+}
+
+int mf_dag_profile(mf_factor* f, int enable) {
+    if (!f) return fail(MF_EARG, "null factor");
+    if (enable && !f->prof) {
+        void* p = nullptr;
+        if (hipMalloc(&p, 8 * MF_DT_KINDS * 3) != hipSuccess) return fail(MF_ENOMEM, "hipMalloc failed (profile)");
+        f->allocs.push_back(p);
+        f->prof_buf = (unsigned long long*)p;
+        launch_fill<unsigned long long>(f->prof_buf, MF_DT_KINDS * 3, 0ull, f->stream);
+    }
+    f->prof = enable ? f->prof_buf : nullptr;
+    return MF_OK;
+}
+
+int mf_dag_profile_read(mf_factor* f, double* out, int reset) {
+    if (!f || !out) return fail(MF_EARG, "null argument");
+    for (int i = 0; i < MF_DT_KINDS * 3; ++i) out[i] = 0;
+    if (!f->prof_buf) return MF_OK;
+    unsigned long long v[MF_DT_KINDS * 3];
+    HIPCHK(hipMemcpyAsync(v, f->prof_buf, sizeof(v), hipMemcpyDeviceToHost, f->stream));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    int dev = 0, khz = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    const double us = khz > 0 ? 1e3 / khz : 0.0;
+    for (int k = 0; k < MF_DT_KINDS; ++k) {
+        out[3 * k] = v[3 * k] * us;
+        out[3 * k + 1] = v[3 * k + 1] * us;
+        out[3 * k + 2] = (double)v[3 * k + 2];
+    }
+    if (reset) launch_fill<unsigned long long>(f->prof_buf, MF_DT_KINDS * 3, 0ull, f->stream);
     return MF_OK;
 }
 

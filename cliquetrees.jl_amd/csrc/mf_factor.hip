@@ -4,23 +4,23 @@
 //               front lives in LDS; extend-add of the children's update
 //               matrices, assembly, partial factorization and the write-out
 //               of the panel and of U in one workgroup.  Persistent mode
-//               hands out fronts by ticket in level order and waits on the
-//               children's completion flags (bounded wait).
+//               hands out fronts by ticket in postorder levels and waits on
+//               the children's completion flags (bounded wait).
 //               Reference: chol_loop_snd!/nod! cholesky.jl:434-692,
 //               chol_send! cholesky.jl:950-983, potrf2! blas/potrf.jl:3-41.
-//   k_ea        extend-add of big fronts, one task per parent column: the
-//               column is accumulated in LDS over all children (in the
-//               reference's reverse child order) and then added to the panel
-//               (F11/F21 += F, cholesky.jl:520-521) or written to U (M22 <-
-//               F22, cholesky.jl:542).
-//   k_diag      factor + invert one 64x64 diagonal block (potrf2!).
-//   k_gemm      batched 64x64-tile MFMA GEMM: panel solve (trsm!,
-//               blas/trsx.jl:156-159, as a product with the inverted
-//               diagonal block), trailing updates and the update matrix
+//   k_dag       persistent tile-DAG over all big fronts: extend-add column
+//               groups (F11/F21 += F, cholesky.jl:520-521; M22 <- F22,
+//               cholesky.jl:542), small fronts above the small region,
+//               diagonal-tile factor + inverse (potrf2!), panel solve tiles
+//               (trsm!, blas/trsx.jl:156-159, as a product with the inverted
+//               diagonal tile), trailing-update tiles and update-matrix tiles
 //               M22 -= L21 [D] L21' (syrk!, blas/syrk.jl:137-159).
+//   k_diag      diagonal-tile inverse alone (solves of a factor given in the
+//               reference layout).
 //   k_copyout   internal panels -> the reference's Dval/Lval/d layout.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "mf_kernels.h"
@@ -65,13 +65,18 @@ __device__ __forceinline__ double regularize(const DevFactor<T>& F, double s, do
     return Djj;
 }
 
-__device__ __forceinline__ void release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+// Waits until this wave's global stores (and loads) have completed, i.e. reached
+// L2.  Before the closing barrier of a task every wave drains, so that thread
+// 0's agent-scope release (an L2 write-back) covers the whole workgroup's data.
+__device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0); }
 __device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
-// Bounded wait for a completion flag; gives up (and records it) after ~1 s.
-__device__ __forceinline__ void wait_flag(int32_t* flag, int32_t* hang) {
+// Bounded wait until *flag >= target; gives up (and records it) after ~1 s.
+__device__ __forceinline__ void wait_ge(int32_t* flag, int32_t target, int32_t* hang) {
     int spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    // poll with relaxed loads (an acquire per poll would invalidate the caches
+    // every other workgroup is using); the caller issues one acquire fence
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         if (++spins > (1 << 24)) {
             atomicExch(hang, 1);
             break;
@@ -83,7 +88,9 @@ __device__ __forceinline__ void wait_flag(int32_t* flag, int32_t* hang) {
     }
 }
 
-constexpr int LDS = SMALL_NJ + 1;  // leading dimension of LDS fronts
+constexpr int LDS = SMALL_NJ + 1;  // leading dimension of LDS fronts and tiles
+constexpr int KC = 32;             // k-chunk of the GEMM tile
+constexpr int LDT = 64 + 4;        // leading dimension of the GEMM LDS stages
 
 // Right-looking partial factorization of the first nn columns of the
 // nj x nj front M (lower triangle, leading dimension LDS) in LDS.
@@ -125,6 +132,42 @@ __device__ void factor_lds(T* M, int nn, int nj, int gc0, const DevFactor<T>& F,
     }
 }
 
+// One small front (nJ <= 64) in LDS: extend-add of the children's update
+// matrices, assembly with the panel's A entries, partial factorization,
+// write-out of the panel and of U.  The caller has made the children final.
+template <typename T>
+__device__ void small_front(const DevPlan& P, const DevFactor<T>& F, int J, T* M, T* s_piv,
+                            T* s_inv, int* s_fail) {
+    const int tid = threadIdx.x;
+    const Front f = P.fronts[J];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const int r = tid & 63, cg = tid >> 6;
+    for (int c = cg; c < nj; c += 4) M[r + c * LDS] = (T)0;
+    __syncthreads();
+    for (int q = f.chd0; q < f.chd1; ++q) {
+        const Front kf = P.fronts[P.chd[q]];
+        const int ma = kf.na;
+        const int32_t* relK = P.rel + kf.reloff;
+        const T* UK = F.U + kf.uoff;
+        if (r < ma) {
+            const int rr = relK[r];
+            for (int c = cg; c <= r; c += 4) M[rr + relK[c] * LDS] += UK[r + c * ma];
+        }
+        __syncthreads();
+    }
+    T* Pj = F.panel + f.poff;
+    if (r < nj)
+        for (int c = cg; c <= r && c < nn; c += 4) M[r + c * LDS] += Pj[r + c * nj];
+    __syncthreads();
+    factor_lds<T>(M, nn, nj, f.c0, F, s_piv, s_inv, s_fail);
+    if (r < nj)
+        for (int c = cg; c <= r && c < nn; c += 4) Pj[r + c * nj] = M[r + c * LDS];
+    if (na > 0 && r < na) {
+        T* Uj = F.U + f.uoff;
+        for (int c = cg; c <= r; c += 4) Uj[r + c * na] = M[(nn + r) + (nn + c) * LDS];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const int32_t* list,
                                                int count, int persistent) {
@@ -142,41 +185,17 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
         }
         if (t >= count) break;
         const int J = list[t];
-        const Front f = P.fronts[J];
-        const int nn = f.nn, na = f.na, nj = nn + na;
         if (tid == 0) {
             s_fail = 0;
-            if (persistent)
-                for (int q = f.chd0; q < f.chd1; ++q) wait_flag(F.flags + P.chd[q], F.hang);
-        }
-        __syncthreads();
-        acquire_agent();
-        const int r = tid & 63, cg = tid >> 6;
-        for (int c = cg; c < nj; c += 4) M[r + c * LDS] = (T)0;
-        __syncthreads();
-        for (int q = f.chd0; q < f.chd1; ++q) {
-            const Front kf = P.fronts[P.chd[q]];
-            const int ma = kf.na;
-            const int32_t* relK = P.rel + kf.reloff;
-            const T* UK = F.U + kf.uoff;
-            if (r < ma) {
-                const int rr = relK[r];
-                for (int c = cg; c <= r; c += 4) M[rr + relK[c] * LDS] += UK[r + c * ma];
+            if (persistent) {
+                const Front f = P.fronts[J];
+                for (int q = f.chd0; q < f.chd1; ++q) wait_ge(F.flags + P.chd[q], 1, F.hang);
+                acquire_agent();
             }
-            __syncthreads();
         }
-        T* Pj = F.panel + f.poff;
-        if (r < nj)
-            for (int c = cg; c <= r && c < nn; c += 4) M[r + c * LDS] += Pj[r + c * nj];
         __syncthreads();
-        factor_lds<T>(M, nn, nj, f.c0, F, &s_piv, &s_inv, &s_fail);
-        if (r < nj)
-            for (int c = cg; c <= r && c < nn; c += 4) Pj[r + c * nj] = M[r + c * LDS];
-        if (na > 0 && r < na) {
-            T* Uj = F.U + f.uoff;
-            for (int c = cg; c <= r; c += 4) Uj[r + c * na] = M[(nn + r) + (nn + c) * LDS];
-        }
-        release_agent();
+        small_front<T>(P, F, J, M, &s_piv, &s_inv, &s_fail);
+        drain_stores();
         __syncthreads();
         if (!persistent) break;
         if (tid == 0) __hip_atomic_store(F.flags + J, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -186,144 +205,148 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
     }
 }
 
+// Extend-add of one parent column by one wavefront: for each window of
+// EA_WIN parent rows the children's contributions (in the reference's
+// reverse child order) are accumulated in a wave-private LDS window, then
+// added to the panel (F11/F21 += F, cholesky.jl:520-521) or written to U
+// (M22 <- F22, cholesky.jl:542).  The child rows that land in window w are
+// [wstart[w], wstart[w+1]) (rel is increasing), so no search is needed; a
+// wavefront's LDS operations complete in order, so no barrier is needed.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ea(DevPlan P, DevFactor<T> F, int t0) {
-    __shared__ T acc[EA_CHUNK];
-    const EaTask e = P.ea[t0 + blockIdx.x];
+__device__ void ea_column(const DevPlan& P, const DevFactor<T>& F, const EaTask& e, T* acc,
+                          int lane) {
     const Front f = P.fronts[e.front];
-    const int nj = f.nn + f.na, pc = e.col, tid = threadIdx.x;
-    T* Pj = F.panel + f.poff;
-    for (int r0 = e.row0; r0 < e.row1; r0 += EA_CHUNK) {
-        const int r1 = min(r0 + EA_CHUNK, e.row1);
-        acc[tid] = (T)0;
-        __syncthreads();
-        for (int s = e.src0; s < e.src1; ++s) {
-            const Front kf = P.fronts[P.ea_src[2 * s]];
-            const int tcol = P.ea_src[2 * s + 1];
-            const int ma = kf.na;
-            const int32_t* relK = P.rel + kf.reloff;
-            // rows tcol..ma-1 of the child column land on parent rows relK[.] (increasing)
-            int lo = tcol, hi = ma;
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (relK[mid] < r0) lo = mid + 1; else hi = mid;
-            }
-            int a = lo;
-            hi = ma;
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (relK[mid] < r1) lo = mid + 1; else hi = mid;
-            }
-            const T* UK = F.U + kf.uoff + (int64_t)tcol * ma;
-            for (int i = a + tid; i < lo; i += blockDim.x) acc[relK[i] - r0] += UK[i];
-            __syncthreads();
-        }
-        const int r = r0 + tid;
-        if (r < r1) {
-            if (pc < f.nn) Pj[r + (int64_t)pc * nj] += acc[tid];
-            else F.U[f.uoff + (r - f.nn) + (int64_t)(pc - f.nn) * f.na] = acc[tid];
-        }
-        __syncthreads();
-    }
-}
-
-__device__ __forceinline__ double rdlane(double x, int l) {
-    long long b = __double_as_longlong(x);
-    int lo = __builtin_amdgcn_readlane((int)b, l);
-    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ float rdlane(float x, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
-}
-
-// One wavefront factors and inverts a kb x kb (kb <= 64) diagonal block.
-// Lane i keeps row i of the block in registers; the scaled column k is
-// broadcast through LDS, so every step is 63-k independent FMAs per lane.
-// The factorization is potrf2!'s right-looking column loop
-// (blas/potrf.jl:3-41); the inverse W = L^-1 (unit L for LDLt) is formed by
-// right-looking forward substitution with lane c owning column c, and is
-// stored (ld = nn) for the panel solve and the triangular solves.
-template <typename T>
-__global__ __launch_bounds__(64) void k_diag(DevPlan P, DevFactor<T> F, int t0, int invert_only) {
-    __shared__ T Tr[NB * (NB + 1)];
-    __shared__ T colk[NB];
-    const DiagTask dt = P.diag[t0 + blockIdx.x];
-    const Front f = P.fronts[dt.front];
-    const int nj = f.nn + f.na, kb = dt.kb, k0 = dt.k0, lane = threadIdx.x;
-    T* blk = F.panel + f.poff + k0 + (int64_t)k0 * nj;
-    T a[NB];
+    const int nj = f.nn + f.na, pc = e.col;
+    const bool to_u = pc >= f.nn;
+    T* dst = to_u ? F.U + f.uoff + (int64_t)(pc - f.nn) * f.na - f.nn
+                  : F.panel + f.poff + (int64_t)pc * nj;  // dst[p]: parent row p
+    for (int w0 = (e.row0 / EA_WIN) * EA_WIN; w0 < e.row1; w0 += EA_WIN) {
+        const int w = w0 / EA_WIN;
 #pragma unroll
-    for (int c = 0; c < NB; ++c)
-        a[c] = (lane < kb && c < kb && c <= lane) ? blk[lane + (int64_t)c * nj] : (T)0;
+        for (int j = 0; j < EA_WIN / 64; ++j) acc[lane + 64 * j] = (T)0;
+        for (int s = e.src0; s < e.src1; ++s) {
+            const int K = P.ea_src[2 * s], tcol = P.ea_src[2 * s + 1];
+            const Front kf = P.fronts[K];
+            const int32_t* ws = P.wstart + P.wsoff[K];
+            const int i0 = max(ws[w], tcol), i1 = ws[w + 1];
+            const int32_t* relK = P.rel + kf.reloff;
+            const T* UK = F.U + kf.uoff + (int64_t)tcol * kf.na;
+            for (int i = i0 + lane; i < i1; i += 64) acc[relK[i] - w0] += UK[i];
+        }
+#pragma unroll
+        for (int j = 0; j < EA_WIN / 64; ++j) {
+            const int p = w0 + lane + 64 * j;
+            if (p >= e.row0 && p < e.row1) {
+                if (to_u) dst[p] = acc[lane + 64 * j];
+                else dst[p] += acc[lane + 64 * j];
+            }
+        }
+    }
+    (void)nj;
+}
+
+// Diagonal tile of a big front: factors the kb x kb (kb <= 64) block at local
+// column k0 (potrf2!'s right-looking loop, blas/potrf.jl:3-41, unless
+// invert_only) and writes W = L^-1 (unit L for LDLt), ld = nn, for the panel
+// solve and the triangular solves.  256 threads; thread (r, g) = (tid % 64,
+// tid / 64) keeps row r, columns g + 4j of the tile in registers and, for the
+// inverse, rows g + 4j of column r of W.  Step k publishes column k through
+// a double-buffered LDS vector, so each step costs one barrier.  The update
+// of step k uses the unscaled column: a_rc -= a_rk a_ck / piv_k, which is
+// l_rk l_ck for Cholesky and l_rk d_k l_ck for LDLt.
+template <typename T>
+__device__ void potrf_tile(const DevFactor<T>& F, const Front& f, int k0, bool invert_only,
+                           T* Ms, T* cb) {
+    const int nj = f.nn + f.na, kb = min(NB, f.nn - k0), tid = threadIdx.x;
+    const int r = tid & 63, cg = tid >> 6, gc0 = f.c0 + k0;
+    T* blk = F.panel + f.poff + k0 + (int64_t)k0 * nj;
+    T m[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int c = cg + 4 * j;
+        m[j] = (r < kb && c < kb && c <= r) ? blk[r + (int64_t)c * nj] : (r == c ? (T)1 : (T)0);
+    }
     if (!invert_only) {
-        bool failed = false;
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
-            if (k < kb) {
-                const int g = f.c0 + k0 + k;
-                const T akk = rdlane(a[k], k);
-                double piv = regularize(F, sign_of(F, g), (double)akk);
-                if (!failed && piv == 0.0) {
-                    failed = true;
-                    if (lane == 0) atomicMin(F.info, (unsigned long long)(g + 1));
-                }
-                if (piv == 0.0) piv = 1.0;  // keeps the rest finite after a failure
-                T dk, fm;
-                if (F.diag == 0) {
-                    dk = (T)sqrt(piv);
-                    fm = (T)1;
-                    if (lane == k) a[k] = dk;
-                } else {
-                    dk = (T)piv;
-                    fm = dk;
-                    if (lane == 0 && !failed) F.d[g] = dk;
-                }
-                const T inv = (T)1 / dk;
-                const T lik = (lane > k) ? a[k] * inv : (T)0;
-                if (lane > k) a[k] = lik;
-                colk[lane] = lik;
-                __syncthreads();
-                const T lf = lik * fm;
+            if (k >= kb) continue;  // uniform; a constant trip count keeps m[] in registers
+            T* col = cb + (k & 1) * NB;
+            if (cg == (k & 3)) col[r] = m[k >> 2];
+            __syncthreads();
+            const T akk = col[k];
+            double piv = regularize(F, sign_of(F, gc0 + k), (double)akk);
+            const bool bad = (piv == 0.0);
+            if (bad) piv = 1.0;  // keeps the rest finite after a failure
+            if (tid == 0) {
+                if (bad) atomicMin(F.info, (unsigned long long)(gc0 + k + 1));
+                else if (F.diag != 0) F.d[gc0 + k] = (T)piv;
+            }
+            const T ip = (T)(1.0 / piv);
+            const T ark = col[r];
+            const T ar = ark * ip;
 #pragma unroll
-                for (int j = k + 1; j < NB; ++j) {
-                    const T ljk = colk[j];
-                    if (lane >= j) a[j] -= lf * ljk;
+            for (int j = 0; j < 16; ++j) {
+                const int c = cg + 4 * j;
+                if (c > k) m[j] -= ar * col[c];
+            }
+            if (cg == (k & 3) && r >= k) {
+                if (F.diag == 0) {
+                    const T sq = (T)sqrt(piv);
+                    m[k >> 2] = (r == k) ? sq : ark * ((T)1 / sq);
+                } else if (r > k) {
+                    m[k >> 2] = ar;  // LDLt keeps the unregularized pivot on the diagonal
                 }
-                __syncthreads();
             }
         }
-        if (lane < kb) {
 #pragma unroll
-            for (int c = 0; c < NB; ++c)
-                if (c < kb && c <= lane) blk[lane + (int64_t)c * nj] = a[c];
+        for (int j = 0; j < 16; ++j) {
+            const int c = cg + 4 * j;
+            if (r < kb && c < kb && c <= r) blk[r + (int64_t)c * nj] = m[j];
         }
     }
-    // W = L^-1, lane c owns column c: w[i] = W[i][c]
-    const bool unit = F.diag != 0;
-    T w[NB];
 #pragma unroll
-    for (int i = 0; i < NB; ++i) w[i] = (lane == i) ? (T)1 : (T)0;
+    for (int j = 0; j < 16; ++j) Ms[r + (cg + 4 * j) * LDS] = m[j];
+    __syncthreads();
+    const bool unit = F.diag != 0;
+    T w[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) w[j] = (cg + 4 * j == r) ? (T)1 : (T)0;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        if (k < kb) {
-            if (!unit) w[k] *= (T)1 / rdlane(a[k], k);
-            colk[lane] = a[k];  // column k of L (lane i holds L[i][k])
-            __syncthreads();
-            const T wk = w[k];
+        if (k >= kb) continue;
+        T* col = cb + (k & 1) * NB;
+        if (cg == (k & 3)) {
+            T x = w[k >> 2];
+            if (!unit) x = x / Ms[k + k * LDS];
+            w[k >> 2] = x;
+            col[r] = x;
+        }
+        __syncthreads();
+        const T x = col[r];
 #pragma unroll
-            for (int i = k + 1; i < NB; ++i) w[i] -= colk[i] * wk;
-            __syncthreads();
+        for (int j = 0; j < 16; ++j) {
+            const int i = cg + 4 * j;
+            if (i > k) w[j] -= Ms[i + k * LDS] * x;
         }
     }
-    // transpose through LDS so that the column-major stores are coalesced
+    __syncthreads();
 #pragma unroll
-    for (int i = 0; i < NB; ++i) Tr[i * (NB + 1) + lane] = (i < kb) ? w[i] : (T)0;
+    for (int j = 0; j < 16; ++j) Ms[(cg + 4 * j) + r * LDS] = w[j];
     __syncthreads();
     T* W = F.W + f.woff + k0;
-    if (lane < kb) {
-        for (int c = 0; c < kb; ++c) W[lane + (int64_t)c * f.nn] = Tr[lane * (NB + 1) + c];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int c = cg + 4 * j;
+        if (r < kb && c < kb) W[r + (int64_t)c * f.nn] = Ms[r + c * LDS];
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(DevPlan P, DevFactor<T> F, int t0, int invert_only) {
+    __shared__ T Ms[NB * LDS];
+    __shared__ T cb[2 * NB];
+    const DiagTask dt = P.diag[t0 + blockIdx.x];
+    potrf_tile<T>(F, P.fronts[dt.front], dt.k0, invert_only != 0, Ms, cb);
 }
 
 template <typename T>
@@ -331,31 +354,48 @@ __device__ __forceinline__ T* arr_base(const DevFactor<T>& F, int a) {
     return a == ARR_PANEL ? F.panel : (a == ARR_U ? F.U : F.W);
 }
 
-// 64x64 output tile per workgroup; wave w owns a 32x32 quarter as 2x2 blocks
-// of 16x16 MFMA tiles (v_mfma_*_16x16x4: lane l holds A[l%16][l/16] and
-// B[l/16][l%16]; the row of output register v is Mfma<T>::row).
+// One 64x64 output tile; wave w owns a 32x32 quarter as 2x2 blocks of 16x16
+// MFMA tiles (v_mfma_*_16x16x4: lane l holds A[l%16][l/16] and B[l/16][l%16];
+// the row of output register v is Mfma<T>::row).  The k-loop is software
+// pipelined: the next KC-chunk of A and B is loaded into registers while the
+// MFMAs consume the current chunk from LDS (As/Bs: KC x LDT each).  For
+// C -= A B' the C tile is fetched before the k-loop.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm(DevFactor<T> F, const GemmTask* tasks, int t0) {
+__device__ void gemm_tile(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs) {
     typedef typename Mfma<T>::v4 v4;
-    constexpr int KC = 16;
-    constexpr int LDT = 64 + 4;
-    __shared__ T As[KC * LDT];
-    __shared__ T Bs[KC * LDT];
-    const GemmTask g = tasks[t0 + blockIdx.x];
+    constexpr int PER = KC * 64 / 256;  // elements of A (and of B) per thread and chunk
     const T* A = arr_base(F, g.arrA) + g.a_off;
     const T* B = arr_base(F, g.arrB) + g.b_off;
     T* C = arr_base(F, g.arrC) + g.c_off;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
-    const T* s = (F.diag != 0 && g.mode == G_SUB && g.s_col >= 0) ? F.d + g.s_col : nullptr;
-    v4 acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4{0, 0, 0, 0};
-    for (int k0 = 0; k0 < g.k; k0 += KC) {
+    const bool sub = g.mode == G_SUB;
+    const T* s = (F.diag != 0 && sub && g.s_col >= 0) ? F.d + g.s_col : nullptr;
+    T cv[2][2][4];
+    if (sub) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int e = tid + 256 * i, r = e & 63, kk = e >> 6, gk = k0 + kk;
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = wr + i * 16 + Mfma<T>::row(lk, v), col = wc + j * 16 + li;
+                    cv[i][j][v] = (row < g.m && col < g.n && (!g.lower || row >= col))
+                                      ? C[row + (int64_t)col * g.ldc] : (T)0;
+                }
+    }
+    v4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4{0, 0, 0, 0};
+    T ra[PER], rb[PER];
+    const int r = tid & 63, kq = tid >> 6;  // element (r, kq + 4i) of each chunk
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int gk = k0 + kq + 4 * i;
             T av = (T)0, bv = (T)0;
             if (gk < g.k) {
                 if (r < g.m) {
@@ -364,10 +404,19 @@ __global__ __launch_bounds__(256) void k_gemm(DevFactor<T> F, const GemmTask* ta
                 }
                 if (r < g.n) bv = B[r + (int64_t)gk * g.ldb];
             }
-            As[kk * LDT + r] = av;
-            Bs[kk * LDT + r] = bv;
+            ra[i] = av;
+            rb[i] = bv;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < g.k; k0 += KC) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            As[(kq + 4 * i) * LDT + r] = ra[i];
+            Bs[(kq + 4 * i) * LDT + r] = rb[i];
         }
         __syncthreads();
+        if (k0 + KC < g.k) fetch(k0 + KC);
 #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
             const T a0 = As[(kk + lk) * LDT + wr + li];
@@ -392,8 +441,8 @@ __global__ __launch_bounds__(256) void k_gemm(DevFactor<T> F, const GemmTask* ta
                 const int col = wc + j * 16 + li;
                 if (row < g.m && col < g.n && (!g.lower || row >= col)) {
                     T* cp = C + row + (int64_t)col * g.ldc;
-                    if (g.mode == G_SUB) {
-                        *cp -= acc[i][j][v];
+                    if (sub) {
+                        *cp = cv[i][j][v] - acc[i][j][v];
                     } else {
                         T x = acc[i][j][v];
                         if (scale) x /= F.d[g.s_col + col];
@@ -402,6 +451,118 @@ __global__ __launch_bounds__(256) void k_gemm(DevFactor<T> F, const GemmTask* ta
                 }
             }
         }
+    }
+}
+
+// Persistent tile-DAG factorization of the big region.  Workgroups take
+// tickets in the planner's order; a task waits (bounded) only on counters
+// that tasks with smaller tickets set, so the order is deadlock-free on any
+// number of resident workgroups.  Counters:
+//   F.flags[J]   finished update-matrix tiles of front J (1 for a small front)
+//   D.eacnt[J]   finished extend-add tasks of J (1 for a small front)
+//   D.cnt[toff_J + K*R_J + I]  panel updates applied to tile (I, K); K+1
+//                once the tile is final
+// Synchronization per task: thread 0 polls with relaxed loads and issues one
+// agent-scope acquire; at the end every wave drains its stores, then thread 0
+// publishes with an agent-scope release.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDag D) {
+    constexpr int SMEM = (NB * LDS + 2 * NB) > (2 * KC * LDT) ? (NB * LDS + 2 * NB) : (2 * KC * LDT);
+    __shared__ T smem[SMEM];
+    __shared__ T s_piv, s_inv;
+    __shared__ int s_fail, s_t;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_t = (int)atomicAdd(F.ticket, 1u);
+        __syncthreads();
+        const int t = s_t;
+        __syncthreads();
+        if (t >= D.ntask) break;
+        const DagTask d = D.task[t];
+        const int J = d.front;
+        const DagFront g = D.df[J];
+        int32_t* cnt = D.cnt + g.toff;
+        unsigned long long tw0 = 0, tw1 = 0;
+        if (tid == 0) {
+            s_fail = 0;
+            if (D.prof) tw0 = wall_clock64();
+            switch (d.kind) {
+                case DT_EA:
+                case DT_SMALL: {
+                    const Front f = P.fronts[J];
+                    for (int q = f.chd0; q < f.chd1; ++q) {
+                        const int K = P.chd[q];
+                        wait_ge(F.flags + K, D.df[K].n_done, F.hang);
+                    }
+                    for (int q = D.uwait_ptr[J]; q < D.uwait_ptr[J + 1]; ++q) {
+                        const int c = D.uwait[q];
+                        wait_ge(D.eacnt + c, D.df[c].n_ea, F.hang);
+                    }
+                    break;
+                }
+                case DT_POTRF:
+                    if (d.a == 0) wait_ge(D.eacnt + J, g.n_ea, F.hang);
+                    else wait_ge(cnt + d.a * g.R + d.a, d.a, F.hang);
+                    break;
+                case DT_TRSM:
+                    wait_ge(cnt + d.c * g.R + d.c, d.c + 1, F.hang);
+                    if (d.c > 0) wait_ge(cnt + d.c * g.R + d.b, d.c, F.hang);
+                    break;
+                case DT_UPD:
+                    wait_ge(cnt + d.d * g.R + d.b, d.d + 1, F.hang);
+                    wait_ge(cnt + d.d * g.R + d.c, d.d + 1, F.hang);
+                    if (d.d > 0) wait_ge(cnt + d.c * g.R + d.b, d.d, F.hang);
+                    break;
+                default:  // DT_SYRK
+                    wait_ge(cnt + (g.p - 1) * g.R + d.b, g.p, F.hang);
+                    wait_ge(cnt + (g.p - 1) * g.R + d.c, g.p, F.hang);
+                    break;
+            }
+            acquire_agent();
+            if (D.prof) tw1 = wall_clock64();
+        }
+        __syncthreads();
+        switch (d.kind) {
+            case DT_EA: {
+                const int wave = tid >> 6, lane = tid & 63;
+                for (int e = d.a + wave; e < d.b; e += 4)
+                    ea_column<T>(P, F, P.ea[e], smem + wave * EA_WIN, lane);
+                break;
+            }
+            case DT_SMALL:
+                small_front<T>(P, F, J, smem, &s_piv, &s_inv, &s_fail);
+                break;
+            case DT_POTRF:
+                potrf_tile<T>(F, P.fronts[J], d.a * NB, false, smem, smem + NB * LDS);
+                break;
+            default:
+                gemm_tile<T>(F, P.gemm[d.a], smem, smem + KC * LDT);
+                break;
+        }
+        drain_stores();
+        __syncthreads();
+        if (tid == 0) {
+            if (D.prof) {
+                const unsigned long long tw2 = wall_clock64();
+                atomicAdd(D.prof + 3 * d.kind, tw1 - tw0);
+                atomicAdd(D.prof + 3 * d.kind + 1, tw2 - tw1);
+                atomicAdd(D.prof + 3 * d.kind + 2, 1ull);
+            }
+            constexpr int R = __ATOMIC_RELEASE, S = __HIP_MEMORY_SCOPE_AGENT;
+            switch (d.kind) {
+                case DT_EA: __hip_atomic_fetch_add(D.eacnt + J, 1, R, S); break;
+                case DT_SMALL:
+                    __hip_atomic_store(D.eacnt + J, 1, R, S);
+                    __hip_atomic_store(F.flags + J, 1, R, S);
+                    break;
+                case DT_POTRF: __hip_atomic_store(cnt + d.a * g.R + d.a, d.a + 1, R, S); break;
+                case DT_TRSM: __hip_atomic_store(cnt + d.c * g.R + d.b, d.c + 1, R, S); break;
+                case DT_UPD: __hip_atomic_store(cnt + d.c * g.R + d.b, d.d + 1, R, S); break;
+                default: __hip_atomic_fetch_add(F.flags + J, 1, R, S); break;
+            }
+        }
+        // closes the thread-0 block before the back-edge
+        __syncthreads();
     }
 }
 
@@ -505,22 +666,22 @@ void launch_small(const DevPlan& P, const DevFactor<T>& F, const int32_t* list, 
 }
 
 template <typename T>
-void launch_ea(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, hipStream_t s) {
-    if (t1 <= t0) return;
-    hipLaunchKernelGGL(k_ea<T>, dim3(t1 - t0), dim3(256), 0, s, P, F, t0);
-}
-
-template <typename T>
 void launch_diag(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, bool invert_only,
                  hipStream_t s) {
     if (t1 <= t0) return;
-    hipLaunchKernelGGL(k_diag<T>, dim3(t1 - t0), dim3(64), 0, s, P, F, t0, invert_only ? 1 : 0);
+    hipLaunchKernelGGL(k_diag<T>, dim3(t1 - t0), dim3(256), 0, s, P, F, t0, invert_only ? 1 : 0);
 }
 
 template <typename T>
-void launch_gemm(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, hipStream_t s) {
-    if (t1 <= t0) return;
-    hipLaunchKernelGGL(k_gemm<T>, dim3(t1 - t0), dim3(256), 0, s, F, P.gemm, t0);
+void launch_dag(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, hipStream_t s) {
+    if (D.ntask <= 0) return;
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_dag<T>, 256, 0);
+    int grid = std::max(1, cus) * std::max(1, per);
+    if (grid > D.ntask) grid = D.ntask;
+    hipLaunchKernelGGL(k_dag<T>, dim3(grid), dim3(256), 0, s, P, F, D);
 }
 
 template <typename T>
@@ -539,12 +700,11 @@ void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, doubl
                                     hipStream_t);                                               \
     template void launch_small<T>(const DevPlan&, const DevFactor<T>&, const int32_t*, int,     \
                                   bool, hipStream_t);                                           \
-    template void launch_ea<T>(const DevPlan&, const DevFactor<T>&, int, int, hipStream_t);     \
     template void launch_diag<T>(const DevPlan&, const DevFactor<T>&, int, int, bool,           \
                                  hipStream_t);                                                 \
-    template void launch_gemm<T>(const DevPlan&, const DevFactor<T>&, int, int, hipStream_t);   \
     template void launch_copyout<T>(const DevPlan&, const DevFactor<T>&, double*, double*,      \
-                                    double*, int, int, hipStream_t);
+                                    double*, int, int, hipStream_t);                            \
+    template void launch_dag<T>(const DevPlan&, const DevFactor<T>&, const DevDag&, hipStream_t);
 MF_INST(double)
 MF_INST(float)
 template void launch_fill<int32_t>(int32_t*, int64_t, int32_t, hipStream_t);

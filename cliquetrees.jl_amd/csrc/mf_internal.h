@@ -23,7 +23,8 @@ namespace mf {
 
 constexpr int NB = 64;          // panel width and GEMM tile edge of big fronts
 constexpr int SMALL_NJ = 64;    // fronts with nJ <= SMALL_NJ use the fused LDS kernel
-constexpr int EA_CHUNK = 256;   // rows per extend-add task
+constexpr int EA_WIN = 256;     // parent rows per extend-add window (one wave)
+constexpr int EA_ROWS = 8192;   // parent rows per extend-add task (about)
 
 struct Front {
     int64_t poff;    // panel offset (elements) in the panel pool
@@ -72,6 +73,27 @@ struct DiagTask {
     int32_t front;
     int32_t k0;     // first local column of the block
     int32_t kb;     // block width (<= NB)
+    int32_t pad;
+};
+
+// Task of the persistent tile-DAG factorization of the big region.
+enum : int32_t { DT_EA = 0, DT_SMALL = 1, DT_POTRF = 2, DT_TRSM = 3, DT_UPD = 4, DT_SYRK = 5 };
+struct DagTask {
+    int32_t kind;
+    int32_t front;
+    int32_t a;   // DT_EA: first EaTask; DT_POTRF: panel block K; else: GemmTask index
+    int32_t b;   // DT_EA: end EaTask; DT_TRSM/UPD/SYRK: row block I
+    int32_t c;   // DT_TRSM: panel block K; DT_UPD/SYRK: column block Jc
+    int32_t d;   // DT_UPD: panel block K
+};
+
+// Per-front constants of the DAG.
+struct DagFront {
+    int32_t toff;    // offset of the front's tile counters (K * R + I)
+    int32_t R;       // row blocks (panel blocks p, then update-row blocks)
+    int32_t p;       // panel (column) blocks
+    int32_t n_ea;    // extend-add tasks that must finish before the panel
+    int32_t n_done;  // tasks whose completion makes U final (SYRK tiles, or 1 for small)
     int32_t pad;
 };
 

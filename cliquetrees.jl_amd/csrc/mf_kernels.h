@@ -22,6 +22,8 @@ struct DevPlan {
     const int32_t* level_big;
     const EaTask* ea;
     const int32_t* ea_src;
+    const int32_t* wstart;       // extend-add window tables
+    const int32_t* wsoff;
     const GemmTask* gemm;
     const DiagTask* diag;
     const OutFront* out;
@@ -52,6 +54,19 @@ struct DevFactor {
     double delta, eps;
 };
 
+// Device view of the tile-DAG schedule of the big region (one factor object's
+// counters; the task list and per-front constants belong to the plan).
+struct DevDag {
+    const DagTask* task;
+    const DagFront* df;
+    const int32_t* uwait_ptr;
+    const int32_t* uwait;
+    int32_t* cnt;      // tile counters
+    int32_t* eacnt;    // finished extend-add tasks per front
+    unsigned long long* prof;  // NULL, or per task kind: wait, body (wall-clock ticks), count
+    int32_t ntask;
+};
+
 // ---- factorization (mf_factor.hip) ----
 template <typename T>
 void launch_fill(T* p, int64_t count, T value, hipStream_t s);
@@ -65,14 +80,12 @@ void launch_refresh(const DevFactor<T>& F, const int64_t* tgt_panel, const int64
 template <typename T>
 void launch_small(const DevPlan& P, const DevFactor<T>& F, const int32_t* list, int count,
                   bool persistent, hipStream_t s);
-template <typename T>
-void launch_ea(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, hipStream_t s);
 // invert_only: the block already holds the factor; only W is produced.
 template <typename T>
 void launch_diag(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, bool invert_only,
                  hipStream_t s);
 template <typename T>
-void launch_gemm(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, hipStream_t s);
+void launch_dag(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, hipStream_t s);
 template <typename T>
 void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, double* Lval,
                     double* dout, int nfr, int n, hipStream_t s);

@@ -286,13 +286,40 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     }
     int64_t small_u = uoff;
     Allocator A;
+    // Last user of every address range of the big U region: a front that
+    // reuses the range must wait until the previous user's parent has read it
+    // (its extend-add), since the tile-DAG runs levels concurrently.
+    std::map<int64_t, std::pair<int64_t, int32_t>> paint;  // begin -> (end, front)
+    std::vector<std::vector<int32_t>> uw(NJ);
+    auto claim = [&](int32_t J, int64_t b, int64_t e) {
+        auto it = paint.upper_bound(b);
+        if (it != paint.begin()) --it;
+        std::vector<std::pair<int64_t, std::pair<int64_t, int32_t>>> keep;
+        while (it != paint.end() && it->first < e) {
+            int64_t b0 = it->first, e0 = it->second.first;
+            int32_t K = it->second.second;
+            if (e0 <= b) { ++it; continue; }
+            int32_t c = P.fronts[K].parent;
+            if (c >= 0 && std::find(uw[J].begin(), uw[J].end(), c) == uw[J].end()) uw[J].push_back(c);
+            if (b0 < b) keep.push_back({b0, {b, K}});
+            if (e0 > e) keep.push_back({e, {e0, K}});
+            it = paint.erase(it);
+        }
+        for (auto& kv : keep) paint[kv.first] = kv.second;
+        paint[b] = {e, J};
+    };
     for (int32_t l = 0; l < nlev; ++l) {
         Level& L = P.levels[l];
         std::vector<int32_t> here(L.small_list);
         here.insert(here.end(), L.big_list.begin(), L.big_list.end());
         for (int32_t J : here) {
             Front& F = P.fronts[J];
-            if (F.na > 0) F.uoff = small_u + A.alloc(rup((int64_t)F.na * F.na, 32));
+            if (F.na > 0) {
+                int64_t sz = rup((int64_t)F.na * F.na, 32);
+                int64_t o = A.alloc(sz);
+                F.uoff = small_u + o;
+                claim(J, o, o + sz);
+            }
         }
         for (int32_t J : here) {
             const Front& F = P.fronts[J];
@@ -305,6 +332,12 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         }
     }
     P.u_elems = small_u + A.peak + 32;
+    P.uwait_ptr.assign(NJ + 1, 0);
+    P.uwait.clear();
+    for (int32_t J = 0; J < NJ; ++J) {
+        P.uwait.insert(P.uwait.end(), uw[J].begin(), uw[J].end());
+        P.uwait_ptr[J + 1] = (int32_t)P.uwait.size();
+    }
     int64_t woff = 0;
     for (int32_t J = 0; J < NJ; ++J) {
         Front& F = P.fronts[J];
@@ -327,6 +360,8 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     P.level_big.clear();
     P.ea.clear();
     P.ea_src.clear();
+    P.wstart.clear();
+    P.wsoff.assign(NJ, -1);
     P.gemm.clear();
     P.diag.clear();
 
@@ -389,74 +424,130 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             }
         }
         L.ea1 = (int32_t)P.ea.size();
-        // panel steps
-        int32_t maxp = 0;
-        for (int32_t J : L.big_list) maxp = std::max(maxp, (P.fronts[J].nn + NB - 1) / NB);
-        L.steps.assign(maxp, Step());
-        for (int32_t s = 0; s < maxp; ++s) {
-            Step& st = L.steps[s];
-            st.diag0 = (int32_t)P.diag.size();
-            for (int32_t J : L.big_list) {
-                const Front& F = P.fronts[J];
-                int32_t k0 = s * NB;
-                if (k0 >= F.nn) continue;
+        // window tables of the children read by this level's extend-add:
+        // wstart[wsoff[K] + w] = first row i of child K with rel_K[i] >= EA_WIN*w
+        for (int32_t J : L.big_list) {
+            const Front& F = P.fronts[J];
+            int32_t nw = (F.nn + F.na + EA_WIN - 1) / EA_WIN;
+            for (int32_t q = F.chd0; q < F.chd1; ++q) {
+                int32_t K = P.chd[q];
+                const Front& Kf = P.fronts[K];
+                P.wsoff[K] = (int32_t)P.wstart.size();
+                int32_t i = 0;
+                for (int32_t w = 0; w <= nw; ++w) {
+                    while (i < Kf.na && P.rel[Kf.reloff + i] < w * EA_WIN) ++i;
+                    P.wstart.push_back(i);
+                }
+            }
+        }
+        // diagonal tiles (the inverse-only pass of solves on given factors)
+        for (int32_t J : L.big_list) {
+            const Front& F = P.fronts[J];
+            for (int32_t k0 = 0; k0 < F.nn; k0 += NB) {
                 DiagTask d{};
                 d.front = J;
                 d.k0 = k0;
                 d.kb = std::min(NB, F.nn - k0);
                 P.diag.push_back(d);
             }
-            st.diag1 = (int32_t)P.diag.size();
-            st.trsm0 = (int32_t)P.gemm.size();
-            for (int32_t J : L.big_list) {
-                const Front& F = P.fronts[J];
-                int32_t nj = F.nn + F.na, k0 = s * NB;
-                if (k0 >= F.nn) continue;
-                int32_t kb = std::min(NB, F.nn - k0);
-                for (int32_t r = k0 + kb; r < nj; r += NB) {
-                    int32_t m = std::min(NB, nj - r);
-                    int64_t ao = F.poff + r + (int64_t)k0 * nj;
-                    add_gemm(G_TRSM, ARR_PANEL, ao, nj, ARR_W, F.woff + k0, F.nn, ARR_PANEL, ao, nj,
-                             m, kb, kb, F.c0 + k0, 0);
-                }
+        }
+    }
+
+    // ---- tile-DAG factorization of the big region -----------------------------
+    // Row blocks of a big front: panel blocks [64K, 64K+kb_K) for K < p, then
+    // update rows [nn + 64m, ...).  Tile (I, K) of the panel carries a counter
+    // = number of panel updates applied; K+1 once it is final (factored or
+    // solved).  Order per level: small fronts, extend-add, then per panel step
+    // K: POTRF(K), TRSM(., K), the look-ahead column updates (., K+1, K), the
+    // other updates; finally the update-matrix tiles (SYRK, K = nn).
+    {
+        P.dag.clear();
+        P.dagf.assign(NJ, DagFront{0, 0, 0, 0, 0, 0});
+        int64_t toff = 0;
+        for (int32_t J : P.small_order) P.dagf[J].n_done = 1;
+        auto row0 = [&](const Front& F, int32_t p, int32_t I) { return I < p ? I * NB : F.nn + (I - p) * NB; };
+        auto rlen = [&](const Front& F, int32_t p, int32_t I) {
+            return I < p ? std::min(NB, F.nn - I * NB) : std::min(NB, F.na - (I - p) * NB);
+        };
+        for (int32_t l = 0; l < nlev; ++l) {
+            const Level& L = P.levels[l];
+            for (int32_t J : L.small_list) {
+                P.dagf[J].n_done = 1;
+                P.dagf[J].n_ea = 1;
+                P.dag.push_back(DagTask{DT_SMALL, J, 0, 0, 0, 0});
             }
-            st.trsm1 = (int32_t)P.gemm.size();
-            st.trail0 = (int32_t)P.gemm.size();
             for (int32_t J : L.big_list) {
                 const Front& F = P.fronts[J];
-                int32_t nj = F.nn + F.na, k0 = s * NB;
-                if (k0 >= F.nn) continue;
-                int32_t kb = std::min(NB, F.nn - k0);
-                for (int32_t c = k0 + kb; c < F.nn; c += NB) {
-                    int32_t nc = std::min(NB, F.nn - c);
-                    for (int32_t r = c; r < nj; r += NB) {
-                        int32_t m = std::min(NB, nj - r);
-                        add_gemm(G_SUB, ARR_PANEL, F.poff + r + (int64_t)k0 * nj, nj, ARR_PANEL,
-                                 F.poff + c + (int64_t)k0 * nj, nj, ARR_PANEL,
-                                 F.poff + r + (int64_t)c * nj, nj, m, nc, kb, F.c0 + k0,
-                                 r == c ? 1 : 0);
+                DagFront& D = P.dagf[J];
+                D.p = (F.nn + NB - 1) / NB;
+                D.R = D.p + (F.na + NB - 1) / NB;
+                D.toff = (int32_t)toff;
+                toff += (int64_t)D.R * D.p;
+                D.n_done = 0;
+                for (int32_t I = D.p; I < D.R; ++I) D.n_done += I - D.p + 1;
+            }
+            // extend-add groups: consecutive columns of one front, about
+            // EA_ROWS parent rows (each wave of the task takes every fourth column)
+            for (int32_t e = L.ea0; e < L.ea1;) {
+                int32_t J = P.ea[e].front, e1 = e + 1;
+                int64_t rows = P.ea[e].row1 - P.ea[e].row0;
+                while (e1 < L.ea1 && e1 - e < 32 && P.ea[e1].front == J && rows < EA_ROWS) {
+                    rows += P.ea[e1].row1 - P.ea[e1].row0;
+                    ++e1;
+                }
+                P.dag.push_back(DagTask{DT_EA, J, e, e1, 0, 0});
+                P.dagf[J].n_ea++;
+                e = e1;
+            }
+            int32_t maxp = 0;
+            for (int32_t J : L.big_list) maxp = std::max(maxp, P.dagf[J].p);
+            for (int32_t K = 0; K < maxp; ++K) {
+                for (int32_t J : L.big_list)
+                    if (K < P.dagf[J].p) P.dag.push_back(DagTask{DT_POTRF, J, K, 0, 0, 0});
+                for (int32_t J : L.big_list) {
+                    const Front& F = P.fronts[J];
+                    const DagFront& D = P.dagf[J];
+                    if (K >= D.p) continue;
+                    int32_t nj = F.nn + F.na, k0 = K * NB, kb = rlen(F, D.p, K);
+                    for (int32_t I = K + 1; I < D.R; ++I) {
+                        int64_t ao = F.poff + row0(F, D.p, I) + (int64_t)k0 * nj;
+                        P.dag.push_back(DagTask{DT_TRSM, J, (int32_t)P.gemm.size(), I, K, 0});
+                        add_gemm(G_TRSM, ARR_PANEL, ao, nj, ARR_W, F.woff + k0, F.nn, ARR_PANEL, ao, nj,
+                                 rlen(F, D.p, I), kb, kb, F.c0 + k0, 0);
                     }
                 }
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int32_t J : L.big_list) {
+                        const Front& F = P.fronts[J];
+                        const DagFront& D = P.dagf[J];
+                        if (K >= D.p) continue;
+                        int32_t nj = F.nn + F.na, k0 = K * NB, kb = rlen(F, D.p, K);
+                        int32_t jc0 = pass == 0 ? K + 1 : K + 2, jc1 = pass == 0 ? std::min(K + 2, D.p) : D.p;
+                        for (int32_t Jc = jc0; Jc < jc1; ++Jc)
+                            for (int32_t I = Jc; I < D.R; ++I) {
+                                P.dag.push_back(DagTask{DT_UPD, J, (int32_t)P.gemm.size(), I, Jc, K});
+                                add_gemm(G_SUB, ARR_PANEL, F.poff + row0(F, D.p, I) + (int64_t)k0 * nj, nj,
+                                         ARR_PANEL, F.poff + row0(F, D.p, Jc) + (int64_t)k0 * nj, nj,
+                                         ARR_PANEL, F.poff + row0(F, D.p, I) + (int64_t)(Jc * NB) * nj, nj,
+                                         rlen(F, D.p, I), rlen(F, D.p, Jc), kb, F.c0 + k0, I == Jc ? 1 : 0);
+                            }
+                    }
             }
-            st.trail1 = (int32_t)P.gemm.size();
-        }
-        // update matrices
-        L.syrk0 = (int32_t)P.gemm.size();
-        for (int32_t J : L.big_list) {
-            const Front& F = P.fronts[J];
-            int32_t nj = F.nn + F.na;
-            if (F.na == 0) continue;
-            for (int32_t c = 0; c < F.na; c += NB) {
-                int32_t nc = std::min(NB, F.na - c);
-                for (int32_t r = c; r < F.na; r += NB) {
-                    int32_t m = std::min(NB, F.na - r);
-                    add_gemm(G_SUB, ARR_PANEL, F.poff + F.nn + r, nj, ARR_PANEL, F.poff + F.nn + c,
-                             nj, ARR_U, F.uoff + r + (int64_t)c * F.na, F.na, m, nc, F.nn, F.c0,
-                             r == c ? 1 : 0);
-                }
+            for (int32_t J : L.big_list) {
+                const Front& F = P.fronts[J];
+                const DagFront& D = P.dagf[J];
+                int32_t nj = F.nn + F.na;
+                for (int32_t Jc = D.p; Jc < D.R; ++Jc)
+                    for (int32_t I = Jc; I < D.R; ++I) {
+                        int32_t r = row0(F, D.p, I) - F.nn, c = row0(F, D.p, Jc) - F.nn;
+                        P.dag.push_back(DagTask{DT_SYRK, J, (int32_t)P.gemm.size(), I, Jc, 0});
+                        add_gemm(G_SUB, ARR_PANEL, F.poff + F.nn + r, nj, ARR_PANEL, F.poff + F.nn + c, nj,
+                                 ARR_U, F.uoff + r + (int64_t)c * F.na, F.na, rlen(F, D.p, I),
+                                 rlen(F, D.p, Jc), F.nn, F.c0, I == Jc ? 1 : 0);
+                    }
             }
         }
-        L.syrk1 = (int32_t)P.gemm.size();
+        P.dag_counters = toff + 1;
     }
 
     // ---- persistent solve task lists ------------------------------------------
@@ -552,6 +643,17 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 const Front& Kf = P.fronts[K];
                 if (Kf.parent != e.front || tc < 0 || tc >= Kf.na) return "ea source";
                 if (Kf.uoff < 0 || Kf.uoff + (int64_t)Kf.na * Kf.na > P.u_elems) return "ea child U";
+                int32_t nw = (nj + EA_WIN - 1) / EA_WIN;
+                if (P.wsoff[K] < 0 || (int64_t)P.wsoff[K] + nw >= (int64_t)P.wstart.size())
+                    return "ea window table";
+                for (int32_t w = 0; w < nw; ++w) {
+                    int32_t i0 = P.wstart[P.wsoff[K] + w], i1 = P.wstart[P.wsoff[K] + w + 1];
+                    if (i0 < 0 || i1 < i0 || i1 > Kf.na) return "ea window range";
+                    for (int32_t i = i0; i < i1; ++i) {
+                        int32_t pr = P.rel[Kf.reloff + i];
+                        if (pr < w * EA_WIN || pr >= (w + 1) * EA_WIN || pr >= nj) return "ea window row";
+                    }
+                }
             }
         }
         for (size_t J = 0; J < P.fronts.size(); ++J) {
@@ -615,6 +717,77 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         }
     }
 
+    // ---- DAG order validation: every wait targets an earlier ticket ------------
+    {
+        const int32_t NT = (int32_t)P.dag.size();
+        std::vector<int32_t> last_ea(NJ, -1), first_panel(NJ, INT32_MAX), last_done(NJ, -1);
+        std::vector<int32_t> setter(P.dag_counters + 1, -1);  // ticket that finalizes (I, K) at value K+1
+        std::map<int64_t, int32_t> upd_at;                      // (tile, K) -> ticket of UPD
+        auto cnt_id = [&](int32_t J, int32_t I, int32_t K) {
+            return (int64_t)P.dagf[J].toff + (int64_t)K * P.dagf[J].R + I;
+        };
+        for (int32_t t = 0; t < NT; ++t) {
+            const DagTask& d = P.dag[t];
+            const DagFront& D = P.dagf[d.front];
+            switch (d.kind) {
+                case DT_EA: last_ea[d.front] = t; break;
+                case DT_SMALL: last_done[d.front] = t; break;
+                case DT_POTRF: setter[cnt_id(d.front, d.a, d.a)] = t; first_panel[d.front] = std::min(first_panel[d.front], t); break;
+                case DT_TRSM: setter[cnt_id(d.front, d.b, d.c)] = t; break;
+                case DT_UPD: upd_at[cnt_id(d.front, d.b, d.c) * 4096 + d.d] = t; break;
+                case DT_SYRK: last_done[d.front] = t; break;
+                default: return "dag task kind";
+            }
+            if (d.kind >= DT_POTRF && (d.front < 0 || D.p <= 0)) return "dag task on a front without panel";
+        }
+        std::vector<int32_t> last_consume(NJ, -1);
+        for (int32_t t = 0; t < NT; ++t)
+            if (P.dag[t].kind == DT_EA || P.dag[t].kind == DT_SMALL) last_consume[P.dag[t].front] = t;
+        for (int32_t t = 0; t < NT; ++t) {
+            const DagTask& d = P.dag[t];
+            const Front& F = P.fronts[d.front];
+            if (d.kind == DT_EA || d.kind == DT_SMALL) {
+                for (int32_t q = P.uwait_ptr[d.front]; q < P.uwait_ptr[d.front + 1]; ++q) {
+                    int32_t c = P.uwait[q];
+                    if (c == d.front || last_consume[c] >= t) return "dag: U reuse before the previous user is consumed";
+                    if (P.dagf[c].n_ea < 1) return "dag: consumer without extend-add";
+                }
+                for (int32_t q = F.chd0; q < F.chd1; ++q) {
+                    int32_t K = P.chd[q];
+                    if (P.dagf[K].n_done > 0 && last_done[K] >= t && P.dagf[K].n_done != 1) return "dag: child after parent";
+                    if (last_done[K] >= t) return "dag: child completes after parent extend-add";
+                }
+                continue;
+            }
+            if (last_ea[d.front] >= t) return "dag: panel task before extend-add";
+            auto need = [&](int32_t I, int32_t K) { return setter[cnt_id(d.front, I, K)]; };
+            if (d.kind == DT_POTRF && d.a > 0) {
+                auto it = upd_at.find(cnt_id(d.front, d.a, d.a) * 4096 + d.a - 1);
+                if (it == upd_at.end() || it->second >= t) return "dag: potrf before its update";
+            }
+            if (d.kind == DT_TRSM) {
+                if (need(d.c, d.c) < 0 || need(d.c, d.c) >= t) return "dag: trsm before potrf";
+                if (d.c > 0) {
+                    auto it = upd_at.find(cnt_id(d.front, d.b, d.c) * 4096 + d.c - 1);
+                    if (it == upd_at.end() || it->second >= t) return "dag: trsm before its update";
+                }
+            }
+            if (d.kind == DT_UPD) {
+                if (need(d.b, d.d) < 0 || need(d.b, d.d) >= t || need(d.c, d.d) < 0 || need(d.c, d.d) >= t)
+                    return "dag: update before its operands";
+                if (d.d > 0) {
+                    auto it = upd_at.find(cnt_id(d.front, d.b, d.c) * 4096 + d.d - 1);
+                    if (it == upd_at.end() || it->second >= t) return "dag: update order";
+                }
+            }
+            if (d.kind == DT_SYRK) {
+                int32_t p = P.dagf[d.front].p;
+                if (need(d.b, p - 1) < 0 || need(d.b, p - 1) >= t || need(d.c, p - 1) < 0 || need(d.c, p - 1) >= t)
+                    return "dag: syrk before its rows";
+            }
+        }
+    }
+
     // ---- statistics ------------------------------------------------------------
     for (int64_t j = 0; j < nfr; ++j) {
         double a = (double)nnref(j), b = (double)naref(j);
@@ -646,7 +819,7 @@ extern "C" int mf_plan_check(int64_t n, int64_t nfr, const int64_t* resptr, cons
     int64_t v[] = {(int64_t)P.fronts.size(), P.n_small_region, P.n_big, (int64_t)P.levels.size(),
                    P.panel_elems, P.u_elems, P.w_elems, (int64_t)P.gemm.size(),
                    (int64_t)P.ea.size(), (int64_t)(P.ftask.size() + P.btask.size()), (int64_t)P.ref_flops,
-                   (int64_t)P.engine_flops, P.max_nj};
+                   (int64_t)P.engine_flops, P.max_nj, (int64_t)P.dag.size(), P.dag_counters};
     for (int64_t i = 0; stats && i < nstats && i < (int64_t)(sizeof(v) / sizeof(v[0])); ++i)
         stats[i] = v[i];
     return err.empty() ? 0 : -1;

@@ -10,12 +10,6 @@
 
 namespace mf {
 
-struct Step {            // one panel step of a big level
-    int32_t diag0, diag1;    // DiagTask range
-    int32_t trsm0, trsm1;    // GemmTask range (panel solve)
-    int32_t trail0, trail1;  // GemmTask range (trailing update inside the panel)
-};
-
 // Task of the persistent triangular solves (one launch per direction).
 enum : int32_t { PT_SMALL = 0, PT_DIAG = 1, PT_ROWS = 2 };
 struct PTask {
@@ -31,8 +25,6 @@ struct Level {
     int32_t smallL0 = 0, smallL1 = 0;  // range in level_small (device)
     int32_t bigL0 = 0, bigL1 = 0;      // range in level_big (device)
     int32_t ea0 = 0, ea1 = 0;          // EaTask range
-    std::vector<Step> steps;
-    int32_t syrk0 = 0, syrk1 = 0;
 };
 
 struct HostPlan {
@@ -52,9 +44,15 @@ struct HostPlan {
     std::vector<Level> levels;
     std::vector<EaTask> ea;
     std::vector<int32_t> ea_src;       // pairs (child, child column)
+    std::vector<int32_t> wstart, wsoff;  // extend-add window tables per child (-1: none)
     std::vector<GemmTask> gemm;
     std::vector<DiagTask> diag;
     std::vector<PTask> ftask, btask;   // forward / backward solve tasks, ticket order
+    std::vector<DagTask> dag;          // tile-DAG factorization of the big region
+    std::vector<DagFront> dagf;        // per internal front (small region: n_done = 1)
+    int64_t dag_counters = 0;          // tile counters
+    std::vector<int32_t> uwait_ptr, uwait;  // per front: fronts whose extend-add must end
+                                            // before the front's U slot is written
     std::vector<int32_t> ntf, ntb;     // tasks per front (completion counts)
     std::vector<int64_t> mvoff1;       // per internal front: offset of Mv (na x 1)
     // copy-out

@@ -17,7 +17,9 @@
 //             the product with the inverted diagonal block W;
 //   PT_ROWS   64 update rows of a big front: M2 rows = gather - L21 X.
 // Completion: per-front task counters (children complete / parent complete)
-// and per-front block progress counters, with agent-scope release/acquire.
+// and per-front block progress counters.  Thread 0 polls with relaxed loads
+// and issues one agent-scope acquire; after the closing barrier (which drains
+// every wave's stores to L2) thread 0's release atomics publish the task.
 // All waits are bounded.  Right-hand sides are processed 16 at a time.
 #include <hip/hip_runtime.h>
 
@@ -33,13 +35,18 @@ constexpr int RB = 16;  // right-hand sides per pass
 constexpr int LDS = SMALL_NJ + 1;
 constexpr int LDP = NB + 1;
 
-__device__ __forceinline__ void release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+// Waits until this wave's global stores (and loads) have completed, i.e. reached
+// L2.  Before the closing barrier of a task every wave drains, so that thread
+// 0's agent-scope release (an L2 write-back) covers the whole workgroup's data.
+__device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0); }
 __device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 // Bounded wait until *p >= target; gives up (and records it) after ~1 s.
 __device__ __forceinline__ void wait_geq(int32_t* p, int32_t target, int32_t* hang) {
     int spins = 0;
-    while (__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    // poll with relaxed loads (an acquire per poll would invalidate the caches
+    // every other workgroup is using); the caller issues one acquire fence
+    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         if (++spins > (1 << 24)) {
             atomicExch(hang, 1);
             break;
@@ -199,9 +206,11 @@ __device__ void diag_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
                                         : (T)0;
         }
         for (int ib = 0; ib < kblk; ++ib) {
-            if (tid == 0) wait_geq(Y_.prog + J, ib + 1, F.hang);
+            if (tid == 0) {
+                wait_geq(Y_.prog + J, ib + 1, F.hang);
+                acquire_agent();
+            }
             __syncthreads();
-            acquire_agent();
             for (int c = g; c < nb; c += 4) X[r + c * NB] = S.B[f.c0 + ib * NB + r + (int64_t)(cb + c) * S.ldb];
             for (int c = g; c < NB; c += 4) Ps[r + c * LDP] = (r < kb) ? Pj[(k0 + r) + (int64_t)(ib * NB + c) * nj] : (T)0;
             __syncthreads();
@@ -249,9 +258,11 @@ __device__ void rows_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
         for (int i = 0; i < RB / 4; ++i) acc[i] = Y[r + (g + 4 * i) * NB];
         for (int ib = 0; ib < p; ++ib) {
             const int kbi = min(NB, nn - ib * NB);
-            if (tid == 0) wait_geq(Y_.prog + J, ib + 1, F.hang);
+            if (tid == 0) {
+                wait_geq(Y_.prog + J, ib + 1, F.hang);
+                acquire_agent();
+            }
             __syncthreads();
-            acquire_agent();
             for (int c = g; c < nb; c += 4)
                 X[r + c * NB] = (r < kbi) ? S.B[f.c0 + ib * NB + r + (int64_t)(cb + c) * S.ldb] : (T)0;
             for (int c = g; c < NB; c += 4)
@@ -312,9 +323,11 @@ __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
         // blocks below the diagonal block, finished in decreasing order
         for (int ib = p - 1; ib > kblk; --ib) {
             const int kbi = min(NB, nn - ib * NB);
-            if (tid == 0) wait_geq(Y_.prog + J, p - ib, F.hang);
+            if (tid == 0) {
+                wait_geq(Y_.prog + J, p - ib, F.hang);
+                acquire_agent();
+            }
             __syncthreads();
-            acquire_agent();
             for (int c = g; c < nb; c += 4)
                 X[r + c * NB] = (r < kbi) ? S.B[f.c0 + ib * NB + r + (int64_t)(cb + c) * S.ldb] : (T)0;
             for (int c = g; c < NB; c += 4)
@@ -364,17 +377,18 @@ __global__ __launch_bounds__(256) void k_fsolve(DevPlan P, DevFactor<T> F, DevSo
         const PTask tk = tasks[t];
         const int J = tk.front;
         const Front f = P.fronts[J];
-        if (tid == 0)
+        if (tid == 0) {
             for (int q = f.chd0; q < f.chd1; ++q) {
                 const int K = P.chd[q];
                 wait_geq(Y_.done + K, Y_.nt[K], F.hang);
             }
+            acquire_agent();
+        }
         __syncthreads();
-        acquire_agent();
         if (tk.kind == PT_SMALL) small_fwd(P, F, S, f, J, Ps, X);
         else if (tk.kind == PT_DIAG) diag_fwd(P, F, S, f, J, tk.a, tk.b, Y_, Ps, X, Y);
         else rows_fwd(P, F, S, f, J, tk.a, tk.b, Y_, Ps, X, Y);
-        release_agent();
+        drain_stores();
         __syncthreads();
         if (tid == 0) {
             if (tk.kind == PT_DIAG)
@@ -404,12 +418,14 @@ __global__ __launch_bounds__(256) void k_bsolve(DevPlan P, DevFactor<T> F, DevSo
         const PTask tk = tasks[t];
         const int J = tk.front;
         const Front f = P.fronts[J];
-        if (tid == 0 && f.parent >= 0) wait_geq(Y_.done + f.parent, Y_.nt[f.parent], F.hang);
+        if (tid == 0) {
+            if (f.parent >= 0) wait_geq(Y_.done + f.parent, Y_.nt[f.parent], F.hang);
+            acquire_agent();
+        }
         __syncthreads();
-        acquire_agent();
         if (tk.kind == PT_SMALL) small_bwd(P, F, S, f, Ps, X);
         else diag_bwd(P, F, S, f, J, tk.a, tk.b, Y_, Ps, X, Y);
-        release_agent();
+        drain_stores();
         __syncthreads();
         if (tid == 0) {
             if (tk.kind == PT_DIAG) {

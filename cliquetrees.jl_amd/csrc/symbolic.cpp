@@ -505,6 +505,14 @@ extern "C" int mfs_grid_nd(int64_t nx, int64_t ny, int64_t nz, int64_t leaf, int
     return s.k == nx * ny * nz ? 0 : -2;
 }
 
+namespace mfs {
+// shared with the MMD ordering (mmd.cpp)
+bool build_adjacency(int64_t n, const int64_t* colptr, const int64_t* rowval,
+                     std::vector<int64_t>& ptr, std::vector<int64_t>& tgt) {
+    return ::build_adjacency(n, colptr, rowval, ptr, tgt);
+}
+}  // namespace mfs
+
 // Multiple minimum degree: implemented in mmd.cpp when present in the build.
 #ifndef MFS_HAVE_MMD
 extern "C" int mfs_mmd(int64_t, const int64_t*, const int64_t*, int64_t, int64_t*, int64_t*) {

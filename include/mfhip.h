@@ -144,9 +144,18 @@ int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const dou
  * times[k] = total milliseconds, counts[k] = launches, flops[k] = algorithmic
  * work of kernel class k since the last reset (classes: MF_KC_*). */
 enum { MF_KC_SMALL = 0, MF_KC_EA, MF_KC_DIAG, MF_KC_GEMM, MF_KC_COPYOUT, MF_KC_SOLVE,
-       MF_KC_ASSEMBLE, MF_KC_COUNT };
+       MF_KC_ASSEMBLE, MF_KC_DAG, MF_KC_COUNT };
 int mf_timing(mf_factor* f, int enable);
 int mf_timing_read(mf_factor* f, double* times_ms, int64_t* counts, double* work, int reset);
+
+/* Task-kind profile of the big-front tile-DAG kernel (no counterpart in the
+ * reference; a diagnostic).  Kinds: extend-add, small front, diagonal tile,
+ * panel-solve tile, trailing-update tile, update-matrix tile.  out[3k..3k+2]
+ * = microseconds spent waiting on dependencies, microseconds of work (summed
+ * over workgroups), task count. */
+enum { MF_DT_KINDS = 6 };
+int mf_dag_profile(mf_factor* f, int enable);
+int mf_dag_profile_read(mf_factor* f, double* out, int reset);
 
 #ifdef __cplusplus
 }

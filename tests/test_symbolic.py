@@ -174,3 +174,48 @@ def test_invalid_input(ct):
         ct.symbolic(A5, alg=[1, 1, 2, 3, 4])
     with pytest.raises(ValueError):
         ct.symbolic(A5, alg=[1, 2, 3])
+
+
+def test_mmd_readme_golden(ct):
+    """src/MMDLib.jl/README.md:12-32: invp = mmd(8, colptr, rowval)."""
+    perm = ct.mmd(G8)
+    invp = np.empty(8, dtype=np.int64)
+    invp[perm - 1] = np.arange(1, 9)
+    assert invp.tolist() == [2, 3, 6, 1, 5, 7, 8, 4]
+
+
+def _fill(A, perm):
+    return int(fill_pattern(A, perm).sum())
+
+
+@pytest.mark.parametrize("delta", [0, 1, 3])
+def test_mmd_properties(ct, delta):
+    """Valid permutation; isolated vertices first in vertex order; the first
+    vertex eliminated has minimum degree; far less fill than the natural order
+    on a grid; the symbolic factorization built on it satisfies the invariants."""
+    A = ct.problems.laplacian2d_dirichlet(12, 11)
+    n = A.shape[0]
+    perm = ct.mmd(A, delta=delta)
+    assert sorted(perm.tolist()) == list(range(1, n + 1))
+    deg = np.diff(sp.csc_matrix(A).indptr) - 1
+    assert deg[perm[0] - 1] == deg.min()
+    Ad = A.toarray()
+    assert _fill(Ad, perm) < 0.6 * _fill(Ad, np.arange(1, n + 1))
+    P, S = ct.symbolic(A, alg=perm)
+    check_invariants(ct, sp.csc_matrix(A), P, S)
+    # isolated vertices come first, in vertex order
+    B = sp.block_diag([sp.identity(3), A, sp.identity(2)]).tocsc()
+    pb = ct.mmd(B, delta=delta)
+    assert pb[:5].tolist() == [1, 2, 3, n + 4, n + 5]
+
+
+def test_mmd_random_graphs(ct):
+    rng = np.random.default_rng(7)
+    for trial in range(20):
+        n = int(rng.integers(5, 60))
+        M = sp.random(n, n, density=float(rng.uniform(0.02, 0.3)), random_state=int(rng.integers(1 << 30)))
+        M = ((M + M.T) != 0).astype(float) + sp.identity(n)
+        perm = ct.mmd(M.tocsc())
+        assert sorted(perm.tolist()) == list(range(1, n + 1))
+        P, S = ct.symbolic(M.tocsc(), alg=perm)
+        check_invariants(ct, sp.csc_matrix(M), P, S)
