@@ -1,11 +1,11 @@
 """Benchmark of the multifrontal numeric engine on one MI355X.
 
-Workload (default): plain FP64 sparse Cholesky of the 3-D 7-point weighted
-Laplacian on a 64^3 grid (n = 262,144; edge weights i.i.d. U[0.5, 1.5],
-seed 2, plus 1e-3 I), geometric nested-dissection order, 16 right-hand sides
-N(0,1) (seed 2).  One step = assembly of A's values into the factor storage
-(setflatindex! of every nonzero, device-resident values) + numeric
-factorization + a 16-RHS solve, all on the GPU.
+Default workload (cfg2): plain FP64 sparse Cholesky of the 3-D 7-point
+weighted Laplacian on a 64^3 grid (n = 262,144; edge weights i.i.d.
+U[0.5, 1.5], seed 2, plus 1e-3 I), geometric nested-dissection order, 16
+right-hand sides N(0,1) (seed 2).  One step = assembly of A's values into the
+factor storage (setflatindex! of every nonzero, device-resident values) +
+numeric factorization + a 16-RHS solve, all on the GPU.
 
   value        = reference factor flops / step time, TFLOP/s (whole step:
                  assembly + factorization + solve), with the flop count of
@@ -16,14 +16,28 @@ factorization + a 16-RHS solve, all on the GPU.
                  engine's stream over the timed region; its algorithmic work
                  is the engine flop count of the fronts it factors.
   cpu_baseline = the CPU oracle (restatement of the reference's serial front
-                 loop) with host OpenBLAS on the same problem, timed on the
-                 host cores.
+                 loop) with host OpenBLAS on a bounded sample of the same
+                 factorization, timed on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2|cfg3]
+Other workloads (--workload):
+  cfg2mp  the cfg2 matrix with FP32 fronts + FP64 iterative refinement to
+          1e-10, against the plain FP64 step on the same matrix
+  cfg3    3-D 27-point stencil 48^3 (seed 3), FP32 fronts + FP64 refinement
+          to 1e-10, error vs the FP64 solution, plain FP64 step alongside
+  cfg3fp  the cfg3 matrix in plain FP64
+  cfg4    4,096 independent grid Laplacians, n ~ U{50..2000} (seed 4), half
+          2-D / half 3-D, factored and solved as one forest (FP64 and
+          FP32 + refinement); value = systems per second
+  cfg5    serving: Poisson stream of KKT LDLt requests, patterns reused over
+          20 value updates, continuous batching; p50/p99 latency, solves/s
+          against one-at-a-time serving
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import importlib.util
 import json
 import os
@@ -36,6 +50,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 FP64_PEAK = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 FP32_PEAK = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 HBM_PEAK = 8000.0  # GB/s
+METRIC = "factor TFLOP/s + solves/s at <=1e-10 rel residual vs plain FP64; p50/p99 ms"
 
 
 def load_package():
@@ -51,15 +66,22 @@ def load_package():
 
 
 def workload(ct, name):
-    if name == "cfg2":
+    if name in ("cfg2", "cfg2mp"):
         A = ct.problems.laplacian3d_7pt(64, 64, 64, seed=2)
-        return dict(A=A, order=ct.grid_nd(64, 64, 64), nrhs=16, seed=2, precision=ct.MF_FP64,
-                    desc="3D 7-point Laplacian 64^3, w~U[0.5,1.5] + 1e-3 I, geometric ND, "
-                         "16 RHS, plain FP64 Cholesky")
-    if name == "cfg3":
+        mp = name == "cfg2mp"
+        return dict(A=A, order=ct.grid_nd(64, 64, 64), nrhs=16, seed=2,
+                    precision=ct.MF_FP32 if mp else ct.MF_FP64, refine=mp,
+                    desc="3D 7-point Laplacian 64^3, w~U[0.5,1.5] + 1e-3 I, geometric ND, 16 RHS, "
+                         + ("FP32 fronts + FP64 iterative refinement to 1e-10" if mp
+                            else "plain FP64 Cholesky"))
+    if name in ("cfg3", "cfg3fp"):
         A = ct.problems.stencil3d_27pt(48, 48, 48, seed=3)
-        return dict(A=A, order=ct.grid_nd(48, 48, 48), nrhs=16, seed=3, precision=ct.MF_FP64,
-                    desc="3D 27-point stencil 48^3, diagonally dominant, geometric ND, 16 RHS, FP64")
+        mp = name == "cfg3"
+        return dict(A=A, order=ct.grid_nd(48, 48, 48), nrhs=16, seed=3,
+                    precision=ct.MF_FP32 if mp else ct.MF_FP64, refine=mp,
+                    desc="3D 27-point stencil 48^3, w~U[0.5,1.5], diagonally dominant, geometric ND, "
+                         "16 RHS, " + ("FP32 fronts + FP64 iterative refinement to 1e-10" if mp
+                                       else "plain FP64 Cholesky"))
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -96,6 +118,307 @@ def cpu_baseline(ct, A, P, S, frac=0.125):
                       f"loop + OpenBLAS dpotrf/dtrsm/dsyrk/dgemm"}
 
 
+class Engine:
+    """One factor object of a workload, its stream and device buffers."""
+
+    def __init__(self, ct, torch, A, P, S, precision, nrhs, B0, stream):
+        from cliquetrees_amd import _native
+        self.ct, self.torch, self.lib, self._native = ct, torch, _native.hip(), _native
+        self.A, self.n, self.nrhs = A, A.shape[0], nrhs
+        self.plan = ct.device_plan(S)
+        self.stats = self.plan.stats()
+        self.fac = ct.DeviceFactor(self.plan, precision)
+        self.h = self.fac.handle
+        self.precision = precision
+        flat = ct.flatindices(S, A)
+        self.c(self.lib.mf_factor_set_pattern(self.h, len(flat), flat.ctypes.data_as(_native.i64p)))
+        self.stream = stream
+        self.c(self.lib.mf_factor_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)))
+        dev = torch.device("cuda:0")
+        self.nz = torch.tensor(np.ascontiguousarray(A.data), dtype=torch.float64, device=dev)
+        self.B0 = torch.tensor(np.asfortranarray(B0).T.copy(), dtype=torch.float64, device=dev)
+        self.X = self.B0.clone()
+        self.perm = np.ascontiguousarray(P.perm, dtype=np.int64)
+        self.iters = []
+        self.relres = []
+
+    def c(self, rc):
+        self.ct.factorization._check(rc)
+
+    def register_matrix(self):
+        A = self.A
+        colptr = np.ascontiguousarray(A.indptr, dtype=np.int64) + 1
+        rowval = np.ascontiguousarray(A.indices, dtype=np.int64) + 1
+        self.c(self.lib.mf_factor_set_matrix(self.h, self.n, colptr.ctypes.data_as(self._native.i64p),
+                                             rowval.ctypes.data_as(self._native.i64p),
+                                             ctypes.c_void_p(self.nz.data_ptr()), 1))
+
+    def factor(self):
+        self.c(self.lib.mf_factor_refresh(self.h, ctypes.c_void_p(self.nz.data_ptr()), 1))
+        self.c(self.lib.mf_factorize_async(self.h, self.ct.MF_CHOLESKY, None, 0, 0.0, 0.0))
+
+    def step_plain(self):
+        self.factor()
+        self.X.copy_(self.B0)
+        self.c(self.lib.mf_solve(self.h, self.perm.ctypes.data_as(self._native.i64p),
+                                 ctypes.c_void_p(self.X.data_ptr()), self.nrhs, self.n, 1))
+
+    def step_refine(self, tol=1e-10, maxit=20):
+        self.factor()
+        it, rr = ctypes.c_int(0), ctypes.c_double(0.0)
+        self.c(self.lib.mf_refine(self.h, self.perm.ctypes.data_as(self._native.i64p),
+                                  ctypes.c_void_p(self.B0.data_ptr()), self.n,
+                                  ctypes.c_void_p(self.X.data_ptr()), self.n, self.nrhs, 1, tol,
+                                  maxit, ctypes.byref(it), ctypes.byref(rr)))
+        self.iters.append(it.value)
+        self.relres.append(rr.value)
+
+    def info(self):
+        info = ctypes.c_int64(0)
+        self.c(self.lib.mf_factor_info(self.h, ctypes.byref(info)))
+        return int(info.value)
+
+    def solution(self):
+        return self.X.T.contiguous().cpu().numpy()
+
+
+def time_steps(torch, stream, step, steps, warmup):
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for k in range(steps):
+        step()
+        evs[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = np.array([evs[k].elapsed_time(evs[k + 1]) for k in range(steps)])
+    return wall * 1e3 / steps, step_ms
+
+
+def read_timing(ct, lib, _native, h):
+    nkc = len(ct.KC_NAMES)
+    times = np.zeros(nkc)
+    cnts = np.zeros(nkc, dtype=np.int64)
+    work = np.zeros(nkc)
+    ct.factorization._check(lib.mf_timing_read(h, times.ctypes.data_as(_native.f64p),
+                                               cnts.ctypes.data_as(_native.i64p),
+                                               work.ctypes.data_as(_native.f64p), 1))
+    return times, cnts, work
+
+
+def run_single(args, ct, torch, name):
+    from cliquetrees_amd import _native
+    W = workload(ct, name)
+    A = W["A"]
+    n = A.shape[0]
+    t0 = time.perf_counter()
+    P, S = ct.symbolic(A, alg=W["order"])
+    t_sym = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    # a dedicated (non-null, default-priority) stream shared by torch and the engine
+    stream = torch.cuda.Stream(dev)
+    torch.cuda.set_stream(stream)
+    rng = np.random.default_rng(W["seed"])
+    B0 = rng.standard_normal((n, W["nrhs"]))
+    E = Engine(ct, torch, A, P, S, W["precision"], W["nrhs"], B0, stream)
+    lib, h = E.lib, E.h
+    step = E.step_refine if W["refine"] else E.step_plain
+    if W["refine"]:
+        E.register_matrix()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    if E.info() != 0:
+        raise SystemExit(f"factorization failed, info={E.info()}")
+    E.iters.clear()
+    E.relres.clear()
+    E.c(lib.mf_timing(h, 1))
+    if args.dag_profile:
+        E.c(lib.mf_dag_profile(h, 1))
+    read_timing(ct, lib, _native, h)
+    ms_per_step, step_ms = time_steps(torch, stream, step, args.steps, 0)
+    times, cnts, work = read_timing(ct, lib, _native, h)
+    E.c(lib.mf_timing(h, 0))
+    if args.dag_profile:
+        prof = np.zeros(18)
+        E.c(lib.mf_dag_profile_read(h, prof.ctypes.data_as(_native.f64p), 1))
+        for k, kname in enumerate(["ea", "small", "potrf", "trsm", "upd", "syrk"]):
+            w, b, c = prof[3 * k:3 * k + 3] / args.steps
+            print(f"dag {kname:6s} tasks/step {c:9.0f}  wait {w / 1e3:9.2f} ms  work {b / 1e3:9.2f} ms"
+                  f"  work/task {b / max(c, 1):8.2f} us", file=sys.stderr)
+    info = E.info()
+    X = E.solution()
+    res = np.linalg.norm(A @ X - B0) / np.linalg.norm(B0)
+
+    stats = E.stats
+    ref_flops = float(stats["ref_flops"])
+    kc = {k: i for i, k in enumerate(ct.KC_NAMES)}
+    g = int(np.argmax(times))
+    dom_ms = times[g] / max(cnts[g], 1)
+    dom_flops_per_launch = work[g] / max(cnts[g], 1)
+    achieved = dom_flops_per_launch / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
+    fp32 = W["precision"] == ct.MF_FP32
+    peak = FP32_PEAK if fp32 else FP64_PEAK
+    solve_ms = times[kc["solve"]] / max(args.steps, 1)
+    factor_ms = sum(times[kc[c]] for c in ["small", "ea", "diag", "gemm", "copyout", "dag"]) / args.steps
+    out = {
+        "metric": METRIC,
+        "value": round(ref_flops / (ms_per_step * 1e-3) / 1e12, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if fp32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": W["desc"], "n": n, "nnz": int(A.nnz), "nrhs": W["nrhs"],
+                   "fronts": stats["nfr"], "internal_fronts": stats["internal_fronts"],
+                   "max_front": stats["max_nj"], "ref_flops": ref_flops,
+                   "engine_flops": float(stats["engine_flops"]), "symbolic_s": round(t_sym, 3)},
+        "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
+        "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
+        "solves_per_s": round(W["nrhs"] * 1e3 / ms_per_step, 2),
+        "solve_ms_per_step": round(float(solve_ms), 4),
+        "factor_ms_per_step": round(float(factor_ms), 4),
+        "factor_tflops": round(ref_flops / (factor_ms * 1e-3) / 1e12, 4) if factor_ms > 0 else None,
+        "rel_residual": float(res),
+        "info": info,
+        "kernel_ms_per_step": {k: round(times[i] / args.steps, 4) for i, k in enumerate(ct.KC_NAMES)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "k_" + ct.KC_NAMES[g], "launches": int(cnts[g]),
+                     "flops_per_launch": float(dom_flops_per_launch),
+                     "avg_launch_ms": round(float(dom_ms), 5)},
+    }
+    if W["refine"]:
+        out["refinement"] = {"iters_mean": float(np.mean(E.iters)), "iters_max": int(np.max(E.iters)),
+                             "relres_max": float(np.max(E.relres))}
+        # the plain FP64 step on the same matrix, and the error against its solution
+        fac64 = Engine(ct, torch, A, P, S, ct.MF_FP64, W["nrhs"], B0, stream)
+        ms64, _ = time_steps(torch, stream, fac64.step_plain, max(3, args.steps // 2), 1)
+        X64 = fac64.solution()
+        out["plain_fp64"] = {"ms_per_step": round(ms64, 4),
+                             "value": round(ref_flops / (ms64 * 1e-3) / 1e12, 4),
+                             "rel_residual": float(np.linalg.norm(A @ X64 - B0) / np.linalg.norm(B0))}
+        out["error_vs_fp64"] = float(np.linalg.norm(X - X64) / np.linalg.norm(X64))
+        out["speedup_vs_plain_fp64"] = round(ms64 / ms_per_step, 3)
+    if not args.no_cpu and not fp32:
+        out["cpu_baseline"] = cpu_baseline(ct, A, P, S)
+    return out
+
+
+def run_batched(args, ct, torch):
+    """cfg4: 4,096 uneven systems as one forest; FP64 and FP32 + refinement."""
+    specs = ct.problems.batch_specs(4096, 4)
+    mats = [ct.problems.grid_laplacian_random(d, 4000 + k) for k, d in enumerate(specs)]
+    orders = [ct.grid_nd(*(d if len(d) == 3 else (d[0], d[1], 1))) for d in specs]
+    rng = np.random.default_rng(4)
+    bs = [rng.standard_normal(m.shape[0]) for m in mats]
+    B = np.concatenate(bs)
+    res = {}
+    t0 = time.perf_counter()
+    Bf64 = ct.BatchedFactorization(mats, orders, precision=ct.MF_FP64)
+    t_setup = time.perf_counter() - t0
+    Bf32 = ct.BatchedFactorization(mats, orders, precision=ct.MF_FP32)
+    for label, Bf in [("fp64", Bf64), ("fp32_ir", Bf32)]:
+        def step():
+            Bf.factorize()
+            return Bf.solve(B) if label == "fp64" else Bf.refine(B, tol=1e-10)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            X = step()
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t1)
+        t = np.array(t)
+        worst = max(np.linalg.norm(m @ X[o:o + m.shape[0]] - b) / np.linalg.norm(b)
+                    for m, b, o in zip(mats, bs, Bf.offsets[:-1]))
+        res[label] = {"ms_per_batch": round(float(np.median(t) * 1e3), 3),
+                      "p99_ms": round(float(np.percentile(t, 99) * 1e3), 3),
+                      "systems_per_s": round(4096 / float(np.median(t)), 1),
+                      "worst_rel_residual": float(worst)}
+    st = Bf64.F.stats()
+    best = max(res.values(), key=lambda r: r["systems_per_s"])
+    return {
+        "metric": "systems/s (factor + solve of every system) at <=1e-10 rel residual",
+        "value": best["systems_per_s"], "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best["ms_per_batch"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 | f32+f64", "data": "synthetic",
+        "config": {"workload": "4096 grid Laplacians n~U{50..2000}, half 2-D/half 3-D, "
+                               "w~U[0.5,1.5]+1e-3 I, geometric ND, one forest, 1 RHS each",
+                   "n_total": int(Bf64.n), "internal_fronts": st["internal_fronts"],
+                   "ref_flops": float(st["ref_flops"]), "setup_s": round(t_setup, 2)},
+        "fp64": res["fp64"], "fp32_ir": res["fp32_ir"],
+    }
+
+
+def run_serving(args, ct, torch):
+    """cfg5: Poisson stream of KKT LDLt requests with pattern reuse."""
+    srv = ct.serving
+    rng = np.random.default_rng(5)
+    npat, updates = 6, 20
+    # n_H log-uniform; capped at 3e4: random couplings make the fill grow
+    # roughly quadratically (n_H = 2.7e4 already has a 1e4-wide front)
+    nhs = np.exp(rng.uniform(np.log(1e3), np.log(3e4), size=npat)).astype(int)
+    pats = []
+    for p, nh in enumerate(nhs):
+        K0, nh_, m = ct.problems.kkt_matrix(ct.problems.near_cubic_dims(int(nh), 3), seed=500 + p)
+        pats.append((K0, nh_))
+    reqs = []
+    for p, (K0, nh_) in enumerate(pats):
+        for u in range(updates):
+            K = ct.problems.kkt_value_update(K0, nh_, seed=10000 * p + u)
+            reqs.append(srv.Request(K=K, B=rng.standard_normal((K.shape[0], 1)), tag=(p, u)))
+    order = rng.permutation(len(reqs))
+    reqs = [reqs[i] for i in order]
+
+    def fresh(rs, t):
+        return [srv.Request(K=r.K, B=r.B, arrival=float(a), tag=r.tag) for r, a in zip(rs, t)]
+
+    loop = srv.ServingLoop(diag=ct.MF_LDLT, max_batch=8, streams=4, hbm_budget=96 << 30)
+    for K0, _ in pats:
+        loop.prepare(K0, ks=(1, 2, 4, 8))
+    # warm every variant once
+    loop.run(fresh(reqs[:len(pats) * 2], np.zeros(len(pats) * 2)), realtime=False)
+    # one-at-a-time capacity: all requests visible, one stream, no batching
+    one = loop.run(fresh(reqs, np.zeros(len(reqs))), realtime=False, batching=False)
+    s_one = srv.latency_stats(one)
+    rate = 1.5 * len(reqs) / s_one["makespan_s"]      # offered load above one-at-a-time capacity
+    t_arr = srv.poisson_arrivals(len(reqs), rate, seed=5)
+    mix = loop.run(fresh(reqs, t_arr), realtime=True, batching=True)
+    seq = loop.run(fresh(reqs, t_arr), realtime=True, batching=False)
+    s_mix, s_seq = srv.latency_stats(mix), srv.latency_stats(seq)
+    worst = max(np.linalg.norm(r.K @ r.X - r.B) / np.linalg.norm(r.B) for r in mix)
+    diff = max(np.linalg.norm(a.X - b.X) / np.linalg.norm(b.X) for a, b in zip(mix, seq))
+    return {
+        "metric": "serving solves/s at <=1e-10 rel residual; p50/p99 ms",
+        "value": round(s_mix["solves_per_s"], 2), "unit": "solves/s", "n_gpus": 1,
+        "steps": len(reqs), "warmup": len(pats) * 2, "ms_per_step": round(s_mix["mean_ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"KKT [H A'; A -1e-6 I] LDLt, {npat} patterns x {updates} updates, "
+                               f"n_H log-uniform in [1e3, 3e4], MMD, Poisson arrivals at "
+                               f"{rate:.1f}/s, 1 RHS",
+                   "n_H": [int(x) for x in nhs], "analysis_s": round(loop.analysis_s, 2)},
+        "p50_ms": round(s_mix["p50_ms"], 3), "p99_ms": round(s_mix["p99_ms"], 3),
+        "one_at_a_time": {"p50_ms": round(s_seq["p50_ms"], 3), "p99_ms": round(s_seq["p99_ms"], 3),
+                          "solves_per_s": round(s_seq["solves_per_s"], 2),
+                          "capacity_solves_per_s": round(s_one["solves_per_s"], 2)},
+        "worst_rel_residual": float(worst), "max_rel_diff_batched_vs_alone": float(diff),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,135 +431,17 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs "
-              f"on a single MI355X)", file=sys.stderr)
+              f"on a single MI355X and has no multi-GPU path)", file=sys.stderr)
         sys.exit(2)
 
     import torch
-    import ctypes
     ct = load_package()
-    from cliquetrees_amd import _native
-    lib = _native.hip()
-
-    W = workload(ct, args.workload)
-    A = W["A"]
-    n = A.shape[0]
-    t0 = time.perf_counter()
-    P, S = ct.symbolic(A, alg=W["order"])
-    t_sym = time.perf_counter() - t0
-    plan = ct.device_plan(S)
-    stats = plan.stats()
-    fac = ct.DeviceFactor(plan, W["precision"])
-    h = fac.handle
-    flat = ct.flatindices(S, A)
-    _c = lambda rc: ct.factorization._check(rc)
-    _c(lib.mf_factor_set_pattern(h, len(flat), flat.ctypes.data_as(_native.i64p)))
-
-    dev = torch.device("cuda:0")
-    # a dedicated (non-null, default-priority) stream shared by torch and the engine
-    stream = torch.cuda.Stream(dev)
-    torch.cuda.set_stream(stream)
-    _c(lib.mf_factor_set_stream(h, ctypes.c_void_p(stream.cuda_stream)))
-    nz = torch.tensor(np.ascontiguousarray(A.data), dtype=torch.float64, device=dev)
-    rng = np.random.default_rng(W["seed"])
-    B0 = rng.standard_normal((n, W["nrhs"]))
-    Bdev0 = torch.tensor(np.asfortranarray(B0).T.copy(), dtype=torch.float64, device=dev)  # (nrhs, n)
-    Bdev = Bdev0.clone()
-    perm = np.ascontiguousarray(P.perm, dtype=np.int64)
-    perm_p = perm.ctypes.data_as(_native.i64p)
-    nrhs = W["nrhs"]
-
-    def step():
-        _c(lib.mf_factor_refresh(h, ctypes.c_void_p(nz.data_ptr()), 1))
-        _c(lib.mf_factorize_async(h, ct.MF_CHOLESKY, None, 0, 0.0, 0.0))
-        Bdev.copy_(Bdev0)
-        _c(lib.mf_solve(h, perm_p, ctypes.c_void_p(Bdev.data_ptr()), nrhs, n, 1))
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    info = ctypes.c_int64(0)
-    _c(lib.mf_factor_info(h, ctypes.byref(info)))
-    if info.value != 0:
-        raise SystemExit(f"factorization failed, info={info.value}")
-
-    _c(lib.mf_timing(h, 1))
-    if args.dag_profile:
-        _c(lib.mf_dag_profile(h, 1))
-    nkc = len(ct.KC_NAMES)
-    times = np.zeros(nkc)
-    cnts = np.zeros(nkc, dtype=np.int64)
-    work = np.zeros(nkc)
-    _c(lib.mf_timing_read(h, times.ctypes.data_as(_native.f64p), cnts.ctypes.data_as(_native.i64p),
-                          work.ctypes.data_as(_native.f64p), 1))
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evs[0].record(stream)
-    for k in range(args.steps):
-        step()
-        evs[k + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    step_ms = np.array([evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)])
-    _c(lib.mf_timing_read(h, times.ctypes.data_as(_native.f64p), cnts.ctypes.data_as(_native.i64p),
-                          work.ctypes.data_as(_native.f64p), 1))
-    _c(lib.mf_timing(h, 0))
-    if args.dag_profile:
-        prof = np.zeros(18)
-        _c(lib.mf_dag_profile_read(h, prof.ctypes.data_as(_native.f64p), 1))
-        for k, name in enumerate(["ea", "small", "potrf", "trsm", "upd", "syrk"]):
-            w, b, c = prof[3 * k:3 * k + 3] / args.steps
-            print(f"dag {name:6s} tasks/step {c:9.0f}  wait {w / 1e3:9.2f} ms  work {b / 1e3:9.2f} ms"
-                  f"  work/task {b / max(c, 1):8.2f} us", file=sys.stderr)
-    _c(lib.mf_factor_info(h, ctypes.byref(info)))
-
-    # accuracy of the last step
-    X = Bdev.T.contiguous().cpu().numpy()
-    res = np.linalg.norm(A @ X - B0) / np.linalg.norm(B0)
-
-    ms_per_step = wall * 1e3 / args.steps
-    ref_flops = float(stats["ref_flops"])
-    kc = {name: i for i, name in enumerate(ct.KC_NAMES)}
-    # dominant kernel: the one with the largest share of the timed region
-    g = int(np.argmax(times))
-    dom_ms = times[g] / max(cnts[g], 1)
-    dom_flops_per_launch = work[g] / max(cnts[g], 1)
-    achieved = dom_flops_per_launch / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
-    peak = FP64_PEAK if W["precision"] == ct.MF_FP64 else FP32_PEAK
-    solve_ms = times[kc["solve"]] / max(cnts[kc["solve"]], 1)
-    factor_ms = sum(times[kc[c]] for c in ["small", "ea", "diag", "gemm", "copyout", "dag"]) / args.steps
-    out = {
-        "metric": "factor TFLOP/s + solves/s at <=1e-10 rel residual vs plain FP64; p50/p99 ms",
-        "value": round(ref_flops / (ms_per_step * 1e-3) / 1e12, 4),
-        "unit": "TFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if W["precision"] == ct.MF_FP64 else "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": W["desc"], "n": n, "nnz": int(A.nnz), "nrhs": nrhs,
-                   "fronts": stats["nfr"], "internal_fronts": stats["internal_fronts"],
-                   "max_front": stats["max_nj"], "ref_flops": ref_flops,
-                   "engine_flops": float(stats["engine_flops"]), "symbolic_s": round(t_sym, 3)},
-        "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
-        "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
-        "solves_per_s": round(nrhs / (solve_ms * 1e-3), 2) if solve_ms > 0 else None,
-        "factor_tflops": round(ref_flops / (factor_ms * 1e-3) / 1e12, 4) if factor_ms > 0 else None,
-        "rel_residual": float(res),
-        "info": int(info.value),
-        "kernel_ms_per_step": {name: round(times[i] / args.steps, 4) for i, name in enumerate(ct.KC_NAMES)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "k_" + ct.KC_NAMES[g], "launches": int(cnts[g]),
-                     "flops_per_launch": float(dom_flops_per_launch),
-                     "avg_launch_ms": round(float(dom_ms), 5)},
-    }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ct, A, P, S)
+    if args.workload == "cfg4":
+        out = run_batched(args, ct, torch)
+    elif args.workload == "cfg5":
+        out = run_serving(args, ct, torch)
+    else:
+        out = run_single(args, ct, torch, args.workload)
     print(json.dumps(out))
 
 

@@ -31,6 +31,10 @@ SIGNATURES = {
     "mf_timing": ([vp, ci], ci),
     "mf_timing_read": ([vp, f64p, i64p, f64p, ci], ci),
     "mf_dag_profile": ([vp, ci], ci),
+    "mf_factor_set_matrix": ([vp, i64, i64p, i64p, vp, ci], ci),
+    "mf_refine": ([vp, i64p, vp, i64, vp, i64, i64, ci, cd, ci, ctypes.POINTER(ci), f64p], ci),
+    "mf_refine_segmented": ([vp, i64p, vp, i64, vp, i64, i64, ci, cd, ci, i64, i64p,
+                             ctypes.POINTER(ci), f64p], ci),
     "mf_dag_profile_read": ([vp, f64p, ci], ci),
 }
 

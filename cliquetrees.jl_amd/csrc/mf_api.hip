@@ -91,6 +91,13 @@ struct mf_factor {
     int32_t* cnt = nullptr;     // tile-DAG counters
     int32_t* eacnt = nullptr;
     unsigned long long* prof = nullptr;  // task-kind profile of the tile-DAG (NULL: off)
+    // FP64 matrix of the refinement residual (full symmetric CSC, original order)
+    int64_t A_n = -1, A_nnz = 0;
+    int64_t* A_colptr = nullptr;
+    int64_t* A_rowval = nullptr;
+    double* A_val = nullptr;
+    void* ir_ws = nullptr;     // B copy, X, R (n x nrhs each) + norms
+    size_t ir_ws_bytes = 0;
     unsigned long long* prof_buf = nullptr;
     int64_t* tgt_panel = nullptr;
     int64_t* tgt_ref = nullptr;
@@ -122,6 +129,10 @@ struct mf_factor {
         for (auto e : evpool) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         if (Mv) (void)hipFree(Mv);
+        if (ir_ws) (void)hipFree(ir_ws);
+        if (A_colptr) (void)hipFree(A_colptr);
+        if (A_rowval) (void)hipFree(A_rowval);
+        if (A_val) (void)hipFree(A_val);
         if (Bws) (void)hipFree(Bws);
         if (stage) (void)hipFree(stage);
         if (own) (void)hipStreamDestroy(own);
@@ -515,6 +526,8 @@ int mf_factor_refresh(mf_factor* f, const double* nzval, int device) {
                                    f->Dref, H.ndz, f->Lref, f->stream);
         }
     });
+    if (f->A_val && f->A_nnz == f->nnz && f->nnz > 0)
+        HIPCHK(hipMemcpyAsync(f->A_val, src, 8 * f->nnz, hipMemcpyDeviceToDevice, f->stream));
     HIPCHK(hipGetLastError());
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     f->factored = false;
@@ -673,6 +686,107 @@ int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t
     HIPCHK(hipGetLastError());
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     return MF_OK;
+}
+
+int mf_factor_set_matrix(mf_factor* f, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                         const double* nzval, int device) {
+    if (!f || !colptr || n != f->plan->H.n) return fail(MF_EARG, "bad argument");
+    const int64_t nnz = colptr[n] - 1;
+    if (colptr[0] != 1 || nnz < 0 || (nnz > 0 && !rowval)) return fail(MF_EARG, "bad CSC pattern");
+    for (int64_t j = 0; j < n; ++j)
+        if (colptr[j + 1] < colptr[j]) return fail(MF_EARG, "bad CSC pattern");
+    for (int64_t p = 0; p < nnz; ++p)
+        if (rowval[p] < 1 || rowval[p] > n) return fail(MF_EARG, "row index out of range");
+    if (f->A_colptr) (void)hipFree(f->A_colptr);
+    if (f->A_rowval) (void)hipFree(f->A_rowval);
+    if (f->A_val) (void)hipFree(f->A_val);
+    f->A_colptr = nullptr;
+    f->A_rowval = nullptr;
+    f->A_val = nullptr;
+    f->A_n = -1;
+    if (hipMalloc((void**)&f->A_colptr, 8 * (n + 1)) != hipSuccess ||
+        hipMalloc((void**)&f->A_rowval, 8 * std::max<int64_t>(nnz, 1)) != hipSuccess ||
+        hipMalloc((void**)&f->A_val, 8 * std::max<int64_t>(nnz, 1)) != hipSuccess)
+        return fail(MF_ENOMEM, "hipMalloc failed (matrix)");
+    HIPCHK(hipMemcpyAsync(f->A_colptr, colptr, 8 * (n + 1), hipMemcpyHostToDevice, f->stream));
+    if (nnz > 0)
+        HIPCHK(hipMemcpyAsync(f->A_rowval, rowval, 8 * nnz, hipMemcpyHostToDevice, f->stream));
+    if (nzval && nnz > 0)
+        HIPCHK(hipMemcpyAsync(f->A_val, nzval, 8 * nnz,
+                              device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, f->stream));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    f->A_n = n;
+    f->A_nnz = nnz;
+    return MF_OK;
+}
+
+int mf_refine_segmented(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb,
+                        double* X, int64_t ldx, int64_t nrhs, int device, double tol, int maxit,
+                        int64_t nseg, const int64_t* segptr, int* iters, double* relres) {
+    if (!f || !perm || nrhs < 0 || (nrhs > 0 && (!B || !X)) || maxit < 0 || nseg < 1 || !segptr)
+        return fail(MF_EARG, "bad argument");
+    if (f->A_n < 0) return fail(MF_ESTATE, "no matrix registered (mf_factor_set_matrix)");
+    if (!f->factored) return fail(MF_ESTATE, "factor not computed");
+    const int64_t n = f->plan->H.n;
+    if (ldb < n || ldx < n) return fail(MF_EARG, "leading dimension < n");
+    if (segptr[0] != 0 || segptr[nseg] != n) return fail(MF_EARG, "segments must cover 0..n");
+    for (int64_t g = 0; g < nseg; ++g)
+        if (segptr[g + 1] < segptr[g]) return fail(MF_EARG, "segments must be increasing");
+    if (nseg * nrhs > (int64_t)1 << 30 || nrhs > 65535) return fail(MF_EARG, "too many segments or columns");
+    if (iters) *iters = 0;
+    if (relres) *relres = 0;
+    if (nrhs == 0 || n == 0) return MF_OK;
+    const size_t cols = 8 * (size_t)n * nrhs, nn = (size_t)nseg * nrhs;
+    if (ensure(&f->ir_ws, &f->ir_ws_bytes, 3 * cols + 16 * nn + 8 * (nseg + 1) + 64)) return MF_ENOMEM;
+    double* Bd = (double*)f->ir_ws;
+    double* Xd = Bd + (size_t)n * nrhs;
+    double* Rd = Xd + (size_t)n * nrhs;
+    double* nrm = Rd + (size_t)n * nrhs;          // |b|^2 then |r|^2, nseg x nrhs each
+    int64_t* segd = (int64_t*)(nrm + 2 * nn);
+    hipStream_t s = f->stream;
+    HIPCHK(hipMemcpyAsync(segd, segptr, 8 * (nseg + 1), hipMemcpyHostToDevice, s));
+    const hipMemcpyKind in = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    for (int64_t c = 0; c < nrhs; ++c)
+        HIPCHK(hipMemcpyAsync(Bd + c * n, B + c * ldb, 8 * n, in, s));
+    launch_colnorm2(segd, (int)nseg, Bd, n, (int)nrhs, nrm, s);
+    launch_copy_cols(n, Bd, n, Xd, n, (int)nrhs, s);
+    int rc = mf_solve(f, perm, Xd, nrhs, n, 1);
+    if (rc) return rc;
+    std::vector<double> h(2 * nn);
+    double worst = 0;
+    int it = 0;
+    for (;;) {
+        launch_residual(n, f->A_colptr, f->A_rowval, f->A_val, Xd, n, Bd, n, Rd, n, (int)nrhs, s);
+        launch_colnorm2(segd, (int)nseg, Rd, n, (int)nrhs, nrm + nn, s);
+        HIPCHK(hipMemcpyAsync(h.data(), nrm, 16 * nn, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        worst = 0;
+        for (size_t k = 0; k < nn; ++k) {
+            const double b2 = h[k], r2 = h[nn + k];
+            const double rr = b2 > 0 ? std::sqrt(r2 / b2) : std::sqrt(r2);
+            worst = std::max(worst, rr);
+        }
+        if (!(worst > tol) || it >= maxit) break;
+        if ((rc = mf_solve(f, perm, Rd, nrhs, n, 1))) return rc;
+        launch_axpy(n, Rd, n, Xd, n, (int)nrhs, s);
+        ++it;
+    }
+    const hipMemcpyKind out = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    for (int64_t c = 0; c < nrhs; ++c)
+        HIPCHK(hipMemcpyAsync(X + c * ldx, Xd + c * n, 8 * n, out, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (iters) *iters = it;
+    if (relres) *relres = worst;
+    return MF_OK;
+}
+
+int mf_refine(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb, double* X,
+              int64_t ldx, int64_t nrhs, int device, double tol, int maxit, int* iters,
+              double* relres) {
+    if (!f) return fail(MF_EARG, "null factor");
+    const int64_t seg[2] = {0, f->plan->H.n};
+    return mf_refine_segmented(f, perm, B, ldb, X, ldx, nrhs, device, tol, maxit, 1, seg, iters,
+                               relres);
 }
 
 int mf_chol_impl(mf_plan* plan, double* Dval, double* Lval, double* d, const double* signs,

@@ -110,4 +110,15 @@ void launch_permute(const double* src, int64_t lds, TB* dst, int64_t ldd, const 
 void launch_permute_back(const void* src, int64_t lds, bool src_f32, double* dst, int64_t ldd,
                          const int64_t* perm, int n, int nrhs, hipStream_t s);
 
+// ---- iterative refinement (mf_refine.hip), FP64 ----
+void launch_residual(int64_t n, const int64_t* colptr, const int64_t* rowval, const double* val,
+                     const double* X, int64_t ldx, const double* B, int64_t ldb, double* R,
+                     int64_t ldr, int nrhs, hipStream_t s);
+void launch_colnorm2(const int64_t* seg, int nseg, const double* M, int64_t ld, int nrhs,
+                     double* out, hipStream_t s);
+void launch_axpy(int64_t n, const double* D, int64_t ldd, double* X, int64_t ldx, int nrhs,
+                 hipStream_t s);
+void launch_copy_cols(int64_t n, const double* S, int64_t lds, double* D, int64_t ldd, int nrhs,
+                      hipStream_t s);
+
 }  // namespace mf

@@ -163,6 +163,7 @@ class ChordalFactorization:
         self._plan = device_plan(S)
         self._f = DeviceFactor(self._plan, precision)
         self._pattern = None
+        self._matrix_key = None
         self.copyto_(A)
 
     # copyto!(F, A) with a pattern reuse fast path (flatindices + setflatindex!)
@@ -178,6 +179,7 @@ class ChordalFactorization:
             self._pattern = key
         vals = np.ascontiguousarray(A.data, dtype=np.float64)
         _check(lib.mf_factor_refresh(self._f.handle, _ptr(vals), 0))
+        self._A = A
         self.info = 0
         return self
 
@@ -283,6 +285,42 @@ def solve(F: ChordalFactorization, B):
     """F \\ B."""
     X = np.array(B, dtype=np.float64, copy=True)
     return ldiv_(F, X)
+
+
+def refine(F: ChordalFactorization, B, tol=1e-10, maxit=20, segments=None):
+    """X = A \\ B by iterative refinement around F (typically FP32 fronts): the
+    residual B - A X and the update are FP64 on the GPU.  Returns X (float64,
+    shape of B); F.ir_iters / F.ir_relres hold the corrections done and the
+    final worst relative residual.  segments (0-based row offsets, first 0,
+    last n) makes the stopping test hold per stacked system.  Raises MFError
+    if tol is not reached."""
+    lib = _native.hip()
+    A = F._A
+    key = F._pattern
+    if F._matrix_key != key:
+        colptr = np.ascontiguousarray(A.indptr, dtype=np.int64) + 1
+        rowval = np.ascontiguousarray(A.indices, dtype=np.int64) + 1
+        vals = np.ascontiguousarray(A.data, dtype=np.float64)
+        _check(lib.mf_factor_set_matrix(F._f.handle, F.n, colptr.ctypes.data_as(_native.i64p),
+                                        rowval.ctypes.data_as(_native.i64p), _ptr(vals), 0))
+        F._matrix_key = key
+    Bm = np.asarray(B, dtype=np.float64)
+    vec = Bm.ndim == 1
+    Bf = np.asfortranarray(Bm.reshape(-1, 1) if vec else Bm)
+    if Bf.shape[0] != F.n:
+        raise ValueError("dimension mismatch")
+    X = np.empty_like(Bf, order="F")
+    perm = np.ascontiguousarray(F.perm, dtype=np.int64)
+    iters, rel = ctypes.c_int(0), ctypes.c_double(0.0)
+    seg = np.ascontiguousarray([0, F.n] if segments is None else segments, dtype=np.int64)
+    _check(lib.mf_refine_segmented(F._f.handle, perm.ctypes.data_as(_native.i64p), _ptr(Bf), F.n,
+                                   _ptr(X), F.n, Bf.shape[1], 0, float(tol), int(maxit),
+                                   len(seg) - 1, seg.ctypes.data_as(_native.i64p),
+                                   ctypes.byref(iters), ctypes.byref(rel)))
+    F.ir_iters, F.ir_relres = int(iters.value), float(rel.value)
+    if not (F.ir_relres <= tol):
+        raise MFError(f"refinement stopped at relative residual {F.ir_relres:.3e} > {tol:.1e}")
+    return X.reshape(-1) if vec else X
 
 
 def chol_impl(S: ChordalSymbolic, Dval, Lval, diag=MF_CHOLESKY, signs=None, reg=None):

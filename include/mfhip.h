@@ -131,6 +131,28 @@ int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t
              int device);
 
 /*
+ * Iterative refinement (mixed precision: FP32 fronts, FP64 residual and
+ * update; no counterpart in the reference, which factors in one precision).
+ * mf_factor_set_matrix registers A (full symmetric CSC, 1-based, original
+ * order, the same nonzero order as mf_factor_set_pattern so that
+ * mf_factor_refresh also updates its values).  mf_refine: X <- A \ B by
+ * X = F \ B, then X += F \ (B - A X) until max_j ||B_j - A X_j|| / ||B_j||
+ * <= tol or maxit corrections; *iters = corrections done, *relres = final
+ * worst relative residual.  device != 0: B and X are device pointers.
+ */
+int mf_factor_set_matrix(mf_factor* f, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                         const double* nzval, int device);
+int mf_refine(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb, double* X,
+              int64_t ldx, int64_t nrhs, int device, double tol, int maxit, int* iters,
+              double* relres);
+/* Same for a batch of independent systems stacked in one factor: rows
+ * [segptr[g], segptr[g+1]) (0-based, segptr[0] = 0, segptr[nseg] = n) are
+ * system g, and the stopping test holds for every system and column. */
+int mf_refine_segmented(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb,
+                        double* X, int64_t ldx, int64_t nrhs, int device, double tol, int maxit,
+                        int64_t nseg, const int64_t* segptr, int* iters, double* relres);
+
+/*
  * Drop-in forms with chol_impl!'s own arguments: Dval/Lval in (assembled A)
  * and out (factor), d out (LDLt pivots), host memory, one call.
  */

@@ -244,6 +244,114 @@ def test_fp32_fronts(ct, oracle, cases, name):
     assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1e-3
 
 
+@pytest.mark.parametrize("name", ["lap3d_16", "st27_10", "lap3d_20x18x9"])
+def test_mixed_precision_refinement(ct, oracle, cases, name):
+    """FP32 fronts + FP64 iterative refinement reach 1e-10 relative residual
+    for 16 right-hand sides and agree with the FP64 oracle's solution; the
+    value refresh also updates the refinement matrix."""
+    A, order, snd = cases[name]
+    P, S = ct.symbolic(A, alg=order, snd=snd)
+    Dv, Lv = oracle.assemble(S, P, A)
+    D0, L0, d0, _ = oracle.chol_impl(S, Dv, Lv)
+    B = np.random.default_rng(5).standard_normal((A.shape[0], 16))
+    X0 = oracle.solve(S, P, D0, L0, d0, B)
+    F = ct.ChordalCholesky(A, P, S, precision=ct.MF_FP32)
+    ct.cholesky_(F)
+    X = ct.refine(F, B, tol=1e-10)
+    res = np.linalg.norm(A @ X - B, axis=0) / np.linalg.norm(B, axis=0)
+    assert res.max() <= 1e-10 and 1 <= F.ir_iters <= 10
+    assert rel(X, X0) <= 1e-9
+    # new values on the same pattern: refresh, refactor, refine again
+    A2 = (A + sp.diags(np.full(A.shape[0], 0.5))).tocsc()
+    F.copyto_(A2)
+    ct.cholesky_(F)
+    X2 = ct.refine(F, B[:, :3], tol=1e-10)
+    res2 = np.linalg.norm(A2 @ X2 - B[:, :3], axis=0) / np.linalg.norm(B[:, :3], axis=0)
+    assert res2.max() <= 1e-10
+
+
+def test_refinement_fp64_and_vector(ct, cases):
+    """With FP64 fronts one solve already meets the tolerance (0 corrections);
+    a vector right-hand side keeps its shape."""
+    A, order, snd = cases["lap3d_16"]
+    P, S = ct.symbolic(A, alg=order, snd=snd)
+    F = ct.ChordalCholesky(A, P, S)
+    ct.cholesky_(F)
+    b = np.random.default_rng(9).standard_normal(A.shape[0])
+    x = ct.refine(F, b, tol=1e-10)
+    assert x.shape == b.shape and F.ir_iters == 0
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1e-12
+
+
+def _batch(ct, count, seed, nmax=400):
+    specs = ct.problems.batch_specs(count, seed, nmin=20, nmax=nmax)
+    mats = [ct.problems.grid_laplacian_random(d, seed + k) for k, d in enumerate(specs)]
+    orders = [ct.grid_nd(*(d if len(d) == 3 else (d[0], d[1], 1))) for d in specs]
+    return mats, orders
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batched_forest(ct, oracle, precision):
+    """A batch of uneven 2-D/3-D systems factored as one forest: every
+    instance matches its own oracle factorization and solve."""
+    mats, orders = _batch(ct, 24, 40)
+    prec = ct.MF_FP64 if precision == "fp64" else ct.MF_FP32
+    Bt = ct.BatchedFactorization(mats, orders, precision=prec).factorize()
+    rng = np.random.default_rng(1)
+    bs = [rng.standard_normal(m.shape[0]) for m in mats]
+    xs = Bt.solve(bs) if prec == ct.MF_FP64 else Bt.refine(bs, tol=1e-10)
+    for A, o, b, x in zip(mats, orders, bs, xs):
+        P, S = ct.symbolic(A, alg=o)
+        Dv, Lv = oracle.assemble(S, P, A)
+        D0, L0, d0, info0 = oracle.chol_impl(S, Dv, Lv)
+        x0 = oracle.solve(S, P, D0, L0, d0, b)
+        assert info0 == 0
+        assert rel(x, x0) <= (1e-11 if prec == ct.MF_FP64 else 1e-9)
+        assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1e-10
+
+
+def test_batched_failure_names_instance(ct):
+    mats, orders = _batch(ct, 6, 41, nmax=120)
+    bad = mats[3].tolil()
+    bad[5, 5] = -50.0
+    mats[3] = bad.tocsc()
+    with pytest.raises(ct.BatchFailure) as ei:
+        ct.BatchedFactorization(mats, orders).factorize()
+    assert ei.value.instance == 3
+
+
+def test_serving_loop_same_alone_and_in_the_mix(ct):
+    """Continuous batching of KKT LDLt requests (3 patterns x 6 value updates,
+    Poisson arrivals): every result equals the request solved alone, and the
+    solves meet the residual bound."""
+    srv = ct.serving
+    loop = srv.ServingLoop(diag=ct.MF_LDLT, max_batch=4, streams=3, hbm_budget=2 << 30)
+    reqs = []
+    rng = np.random.default_rng(11)
+    for p, nh in enumerate([300, 700, 1500]):
+        K0, nh_, m = ct.problems.kkt_matrix(ct.problems.near_cubic_dims(nh, 3), seed=20 + p)
+        for u in range(6):
+            K = ct.problems.kkt_value_update(K0, nh_, seed=100 * p + u)
+            reqs.append(srv.Request(K=K, B=rng.standard_normal((K.shape[0], 1)), tag=(p, u)))
+    t = srv.poisson_arrivals(len(reqs), rate=2000.0, seed=3)
+    order = rng.permutation(len(reqs))
+    for r, k in zip(reqs, order):
+        r.arrival = float(t[k])
+    out = loop.run(reqs)
+    assert max(r.batch for r in out) > 1          # some requests were batched
+    for r in out:
+        assert r.info == 0
+        perm = ct.mmd(r.K)
+        P, S = ct.symbolic(r.K, alg=perm)
+        F = ct.ChordalLDLt(r.K, P, S)
+        ct.ldlt_(F)
+        x0 = ct.solve(F, r.B)
+        assert rel(r.X, x0) <= 1e-12
+        assert np.linalg.norm(r.K @ r.X - r.B) / np.linalg.norm(r.B) <= 1e-10
+    st = srv.latency_stats(out)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
 def test_edge_cases(ct, oracle):
     # 1x1, diagonal matrix (forest of single-column fronts), disconnected blocks
     for A in [sp.csc_matrix(np.array([[2.0]])), sp.diags(np.arange(1.0, 40.0)).tocsc(),
