@@ -71,6 +71,13 @@ def hip():
     if _hip is None:
         if not os.path.exists(HIP_PATH):
             raise OSError(f"{HIP_PATH} is not built; run __graft_entry__.build()")
+        # PyTorch ships its own HIP runtime; load it first so that the engine
+        # binds to that same runtime (one runtime per process, whichever of the
+        # two the caller touches first)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(HIP_PATH)
         from . import _hipsig
         _hipsig.declare(lib)

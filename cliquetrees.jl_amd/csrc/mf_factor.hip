@@ -97,8 +97,7 @@ constexpr int LDT = 64 + 4;        // leading dimension of the GEMM LDS stages
 // Returns after a failed pivot (recorded in F.info).
 template <typename T>
 __device__ void factor_lds(T* M, int nn, int nj, int gc0, const DevFactor<T>& F,
-                           T* s_piv, T* s_inv, int* s_fail) {
-    const int tid = threadIdx.x;
+                           T* s_piv, T* s_inv, int* s_fail, const int tid) {
     for (int k = 0; k < nn; ++k) {
         if (tid == 0) {
             double akk = (double)M[k + k * LDS];
@@ -137,8 +136,7 @@ __device__ void factor_lds(T* M, int nn, int nj, int gc0, const DevFactor<T>& F,
 // write-out of the panel and of U.  The caller has made the children final.
 template <typename T>
 __device__ void small_front(const DevPlan& P, const DevFactor<T>& F, int J, T* M, T* s_piv,
-                            T* s_inv, int* s_fail) {
-    const int tid = threadIdx.x;
+                            T* s_inv, int* s_fail, const int tid) {
     const Front f = P.fronts[J];
     const int nn = f.nn, na = f.na, nj = nn + na;
     const int r = tid & 63, cg = tid >> 6;
@@ -159,7 +157,7 @@ __device__ void small_front(const DevPlan& P, const DevFactor<T>& F, int J, T* M
     if (r < nj)
         for (int c = cg; c <= r && c < nn; c += 4) M[r + c * LDS] += Pj[r + c * nj];
     __syncthreads();
-    factor_lds<T>(M, nn, nj, f.c0, F, s_piv, s_inv, s_fail);
+    factor_lds<T>(M, nn, nj, f.c0, F, s_piv, s_inv, s_fail, tid);
     if (r < nj)
         for (int c = cg; c <= r && c < nn; c += 4) Pj[r + c * nj] = M[r + c * LDS];
     if (na > 0 && r < na) {
@@ -194,7 +192,7 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
             }
         }
         __syncthreads();
-        small_front<T>(P, F, J, M, &s_piv, &s_inv, &s_fail);
+        small_front<T>(P, F, J, M, &s_piv, &s_inv, &s_fail, tid);
         drain_stores();
         __syncthreads();
         if (!persistent) break;
@@ -256,8 +254,8 @@ __device__ void ea_column(const DevPlan& P, const DevFactor<T>& F, const EaTask&
 // l_rk l_ck for Cholesky and l_rk d_k l_ck for LDLt.
 template <typename T>
 __device__ void potrf_tile(const DevFactor<T>& F, const Front& f, int k0, bool invert_only,
-                           T* Ms, T* cb) {
-    const int nj = f.nn + f.na, kb = min(NB, f.nn - k0), tid = threadIdx.x;
+                           T* Ms, T* cb, const int tid) {
+    const int nj = f.nn + f.na, kb = min(NB, f.nn - k0);
     const int r = tid & 63, cg = tid >> 6, gc0 = f.c0 + k0;
     T* blk = F.panel + f.poff + k0 + (int64_t)k0 * nj;
     T m[16];
@@ -346,7 +344,7 @@ __global__ __launch_bounds__(256) void k_diag(DevPlan P, DevFactor<T> F, int t0,
     __shared__ T Ms[NB * LDS];
     __shared__ T cb[2 * NB];
     const DiagTask dt = P.diag[t0 + blockIdx.x];
-    potrf_tile<T>(F, P.fronts[dt.front], dt.k0, invert_only != 0, Ms, cb);
+    potrf_tile<T>(F, P.fronts[dt.front], dt.k0, invert_only != 0, Ms, cb, (int)threadIdx.x);
 }
 
 template <typename T>
@@ -361,13 +359,13 @@ __device__ __forceinline__ T* arr_base(const DevFactor<T>& F, int a) {
 // MFMAs consume the current chunk from LDS (As/Bs: KC x LDT each).  For
 // C -= A B' the C tile is fetched before the k-loop.
 template <typename T>
-__device__ void gemm_tile(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs) {
+__device__ void gemm_tile(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs, const int tid) {
     typedef typename Mfma<T>::v4 v4;
     constexpr int PER = KC * 64 / 256;  // elements of A (and of B) per thread and chunk
     const T* A = arr_base(F, g.arrA) + g.a_off;
     const T* B = arr_base(F, g.arrB) + g.b_off;
     T* C = arr_base(F, g.arrC) + g.c_off;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wave = tid >> 6, lane = tid & 63;
     const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
     const bool sub = g.mode == G_SUB;
@@ -454,28 +452,131 @@ __device__ void gemm_tile(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs
     }
 }
 
+constexpr int KW = 16;        // k-chunk of the wide tile
+constexpr int LDW = 128 + 4;  // leading dimension of its LDS stages
+
+// 128 x 128 update-matrix tile, C -= A diag(s) B' (lower part only on the
+// diagonal): twice the flops per byte of the 64 x 64 tile, for the long-k
+// update matrices that carry most of the flops.  Wave w owns the 64 x 64
+// quarter (w & 1, w >> 1) as 4 x 4 MFMA blocks (the strictly upper quarter
+// of a diagonal tile does no MFMA work).  The k-loop stages KW columns of A
+// and B through LDS (As/Bs: KW x LDW) with the next chunk prefetched into
+// registers; C is updated after the k-loop in four row-block rounds.
+template <typename T>
+__device__ void gemm_tile128(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs, const int tid) {
+    typedef typename Mfma<T>::v4 v4;
+    constexpr int PER = KW * 128 / 256;
+    const T* A = arr_base(F, g.arrA) + g.a_off;
+    const T* B = arr_base(F, g.arrB) + g.b_off;
+    T* C = arr_base(F, g.arrC) + g.c_off;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
+    const int li = lane & 15, lk = lane >> 4;
+    const bool idle = g.lower && wc > wr;  // strictly above the diagonal
+    const T* s = (F.diag != 0 && g.s_col >= 0) ? F.d + g.s_col : nullptr;
+    v4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4{0, 0, 0, 0};
+    T ra[PER], rb[PER];
+    const int r = tid & 127, kq = tid >> 7;  // element (r, kq + 2i) of each chunk
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int gk = k0 + kq + 2 * i;
+            T av = (T)0, bv = (T)0;
+            if (gk < g.k) {
+                if (r < g.m) {
+                    av = A[r + (int64_t)gk * g.lda];
+                    if (s) av *= s[gk];
+                }
+                if (r < g.n) bv = B[r + (int64_t)gk * g.ldb];
+            }
+            ra[i] = av;
+            rb[i] = bv;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < g.k; k0 += KW) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            As[(kq + 2 * i) * LDW + r] = ra[i];
+            Bs[(kq + 2 * i) * LDW + r] = rb[i];
+        }
+        __syncthreads();
+        if (k0 + KW < g.k) fetch(k0 + KW);
+        if (!idle) {
+#pragma unroll
+            for (int kk = 0; kk < KW; kk += 4) {
+                T a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = As[(kk + lk) * LDW + wr + 16 * i + li];
+                    b[i] = Bs[(kk + lk) * LDW + wc + 16 * i + li];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+    if (idle) return;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        T cv[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = wr + 16 * i + Mfma<T>::row(lk, v), col = wc + 16 * j + li;
+                cv[j][v] = (row < g.m && col < g.n && (!g.lower || row >= col))
+                               ? C[row + (int64_t)col * g.ldc] : (T)0;
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = wr + 16 * i + Mfma<T>::row(lk, v), col = wc + 16 * j + li;
+                if (row < g.m && col < g.n && (!g.lower || row >= col))
+                    C[row + (int64_t)col * g.ldc] = cv[j][v] - acc[i][j][v];
+            }
+    }
+}
+
 // Persistent tile-DAG factorization of the big region.  Workgroups take
 // tickets in the planner's order; a task waits (bounded) only on counters
-// that tasks with smaller tickets set, so the order is deadlock-free on any
+// that tasks with smaller tickets set.  A workgroup takes its next ticket
+// when it starts a task (hiding that latency behind the task); the smallest
+// unfinished ticket is then always some workgroup's current task, whose
+// dependencies are all finished, so the order is deadlock-free on any
 // number of resident workgroups.  Counters:
 //   F.flags[J]   finished update-matrix tiles of front J (1 for a small front)
 //   D.eacnt[J]   finished extend-add tasks of J (1 for a small front)
 //   D.cnt[toff_J + K*R_J + I]  panel updates applied to tile (I, K); K+1
 //                once the tile is final
-// Synchronization per task: thread 0 polls with relaxed loads and issues one
-// agent-scope acquire; at the end every wave drains its stores, then thread 0
-// publishes with an agent-scope release.
+// Synchronization per task: the lanes of wave 0 poll one dependency each
+// with relaxed loads, then thread 0 issues one agent-scope acquire; at the
+// end every wave drains its stores, then thread 0 publishes with an
+// agent-scope release.
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDag D) {
-    constexpr int SMEM = (NB * LDS + 2 * NB) > (2 * KC * LDT) ? (NB * LDS + 2 * NB) : (2 * KC * LDT);
+    constexpr int S1 = NB * LDS + 2 * NB, S2 = 2 * KC * LDT, S3 = 2 * KW * LDW;
+    constexpr int SMEM = S1 > S2 ? (S1 > S3 ? S1 : S3) : (S2 > S3 ? S2 : S3);
     __shared__ T smem[SMEM];
     __shared__ T s_piv, s_inv;
-    __shared__ int s_fail, s_t;
-    const int tid = threadIdx.x;
+    __shared__ int s_fail, s_next;
+    if (threadIdx.x == 0) s_next = (int)atomicAdd(F.ticket, 1u);
     for (;;) {
-        if (tid == 0) s_t = (int)atomicAdd(F.ticket, 1u);
+        // an opaque copy of the thread index: values derived from it are
+        // recomputed per task instead of being held (and spilled) across the
+        // whole persistent loop
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
         __syncthreads();
-        const int t = s_t;
+        const int t = s_next;
         __syncthreads();
         if (t >= D.ntask) break;
         const DagTask d = D.task[t];
@@ -483,43 +584,69 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
         const DagFront g = D.df[J];
         int32_t* cnt = D.cnt + g.toff;
         unsigned long long tw0 = 0, tw1 = 0;
-        if (tid == 0) {
-            s_fail = 0;
-            if (D.prof) tw0 = wall_clock64();
+        if (tid < 64) {
+            if (tid == 0) {
+                s_fail = 0;
+                s_next = (int)atomicAdd(F.ticket, 1u);
+                if (D.prof) tw0 = wall_clock64();
+            }
+            int nw, nchd = 0, c0 = 0, u0 = 0;
             switch (d.kind) {
                 case DT_EA:
                 case DT_SMALL: {
                     const Front f = P.fronts[J];
-                    for (int q = f.chd0; q < f.chd1; ++q) {
-                        const int K = P.chd[q];
-                        wait_ge(F.flags + K, D.df[K].n_done, F.hang);
-                    }
-                    for (int q = D.uwait_ptr[J]; q < D.uwait_ptr[J + 1]; ++q) {
-                        const int c = D.uwait[q];
-                        wait_ge(D.eacnt + c, D.df[c].n_ea, F.hang);
-                    }
+                    c0 = f.chd0;
+                    nchd = f.chd1 - f.chd0;
+                    u0 = D.uwait_ptr[J];
+                    nw = nchd + D.uwait_ptr[J + 1] - u0;
                     break;
                 }
-                case DT_POTRF:
-                    if (d.a == 0) wait_ge(D.eacnt + J, g.n_ea, F.hang);
-                    else wait_ge(cnt + d.a * g.R + d.a, d.a, F.hang);
-                    break;
-                case DT_TRSM:
-                    wait_ge(cnt + d.c * g.R + d.c, d.c + 1, F.hang);
-                    if (d.c > 0) wait_ge(cnt + d.c * g.R + d.b, d.c, F.hang);
-                    break;
-                case DT_UPD:
-                    wait_ge(cnt + d.d * g.R + d.b, d.d + 1, F.hang);
-                    wait_ge(cnt + d.d * g.R + d.c, d.d + 1, F.hang);
-                    if (d.d > 0) wait_ge(cnt + d.c * g.R + d.b, d.d, F.hang);
-                    break;
-                default:  // DT_SYRK
-                    wait_ge(cnt + (g.p - 1) * g.R + d.b, g.p, F.hang);
-                    wait_ge(cnt + (g.p - 1) * g.R + d.c, g.p, F.hang);
-                    break;
+                case DT_POTRF: nw = 1; break;
+                case DT_TRSM: nw = d.c > 0 ? 2 : 1; break;
+                case DT_UPD: nw = d.d > 0 ? 3 : 2; break;
+                default: nw = 2 + (d.d & 1) + ((d.d >> 1) & 1); break;
             }
-            acquire_agent();
-            if (D.prof) tw1 = wall_clock64();
+            for (int q = tid; q < nw; q += 64) {
+                int32_t* w;
+                int32_t target;
+                switch (d.kind) {
+                    case DT_EA:
+                    case DT_SMALL:
+                        if (q < nchd) {
+                            const int K = P.chd[c0 + q];
+                            w = F.flags + K;
+                            target = D.df[K].n_done;
+                        } else {
+                            const int c = D.uwait[u0 + q - nchd];
+                            w = D.eacnt + c;
+                            target = D.df[c].n_ea;
+                        }
+                        break;
+                    case DT_POTRF:
+                        w = d.a == 0 ? D.eacnt + J : cnt + d.a * g.R + d.a;
+                        target = d.a == 0 ? g.n_ea : d.a;
+                        break;
+                    case DT_TRSM:
+                        w = cnt + d.c * g.R + (q == 0 ? d.c : d.b);
+                        target = q == 0 ? d.c + 1 : d.c;
+                        break;
+                    case DT_UPD:
+                        w = q == 2 ? cnt + d.c * g.R + d.b : cnt + d.d * g.R + (q == 0 ? d.b : d.c);
+                        target = q == 2 ? d.d : d.d + 1;
+                        break;
+                    default: {  // DT_SYRK: the tile's row blocks of the panel are final
+                        const int blk = q == 0 ? d.b : q == 1 ? d.c : (q == 2 && (d.d & 1)) ? d.b + 1 : d.c + 1;
+                        w = cnt + (g.p - 1) * g.R + blk;
+                        target = g.p;
+                        break;
+                    }
+                }
+                wait_ge(w, target, F.hang);
+            }
+            if (tid == 0) {
+                acquire_agent();
+                if (D.prof) tw1 = wall_clock64();
+            }
         }
         __syncthreads();
         switch (d.kind) {
@@ -530,14 +657,17 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                 break;
             }
             case DT_SMALL:
-                small_front<T>(P, F, J, smem, &s_piv, &s_inv, &s_fail);
+                small_front<T>(P, F, J, smem, &s_piv, &s_inv, &s_fail, tid);
                 break;
             case DT_POTRF:
-                potrf_tile<T>(F, P.fronts[J], d.a * NB, false, smem, smem + NB * LDS);
+                potrf_tile<T>(F, P.fronts[J], d.a * NB, false, smem, smem + NB * LDS, tid);
                 break;
-            default:
-                gemm_tile<T>(F, P.gemm[d.a], smem, smem + KC * LDT);
+            default: {
+                const GemmTask gt = P.gemm[d.a];
+                if (gt.wide) gemm_tile128<T>(F, gt, smem, smem + KW * LDW, tid);
+                else gemm_tile<T>(F, gt, smem, smem + KC * LDT, tid);
                 break;
+            }
         }
         drain_stores();
         __syncthreads();

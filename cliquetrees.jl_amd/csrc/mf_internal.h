@@ -25,6 +25,7 @@ constexpr int NB = 64;          // panel width and GEMM tile edge of big fronts
 constexpr int SMALL_NJ = 64;    // fronts with nJ <= SMALL_NJ use the fused LDS kernel
 constexpr int EA_WIN = 256;     // parent rows per extend-add window (one wave)
 constexpr int EA_ROWS = 8192;   // parent rows per extend-add task (about)
+constexpr int WIDE_K = 128;     // update-matrix tiles are 128 x 128 when nn >= WIDE_K
 
 struct Front {
     int64_t poff;    // panel offset (elements) in the panel pool
@@ -65,7 +66,8 @@ struct GemmTask {
     int32_t s_col;  // global column of s[0] (LDLt pivots), -1: none
     int8_t arrA, arrB, arrC, mode;
     int8_t lower;   // store only r >= c (diagonal tile)
-    int8_t pad[3];
+    int8_t wide;    // 128 x 128 tile (m, n <= 2 NB), else 64 x 64
+    int8_t pad[2];
 };
 
 // Factor one diagonal block of a big front and invert it.
@@ -84,7 +86,8 @@ struct DagTask {
     int32_t a;   // DT_EA: first EaTask; DT_POTRF: panel block K; else: GemmTask index
     int32_t b;   // DT_EA: end EaTask; DT_TRSM/UPD/SYRK: row block I
     int32_t c;   // DT_TRSM: panel block K; DT_UPD/SYRK: column block Jc
-    int32_t d;   // DT_UPD: panel block K
+    int32_t d;   // DT_UPD: panel block K; DT_SYRK: bit 0 / bit 1: the tile also
+                 // covers row block I+1 / column block Jc+1 (wide tile)
 };
 
 // Per-front constants of the DAG.

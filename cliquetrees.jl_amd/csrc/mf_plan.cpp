@@ -367,8 +367,9 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
 
     auto add_gemm = [&](int8_t mode, int8_t aA, int64_t ao, int32_t lda, int8_t aB, int64_t bo,
                         int32_t ldb, int8_t aC, int64_t co, int32_t ldc, int32_t m, int32_t nn_,
-                        int32_t k, int32_t scol, int8_t lower) {
+                        int32_t k, int32_t scol, int8_t lower, int8_t wide = 0) {
         GemmTask t{};
+        t.wide = wide;
         t.a_off = ao;
         t.b_off = bo;
         t.c_off = co;
@@ -483,8 +484,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 D.R = D.p + (F.na + NB - 1) / NB;
                 D.toff = (int32_t)toff;
                 toff += (int64_t)D.R * D.p;
-                D.n_done = 0;
-                for (int32_t I = D.p; I < D.R; ++I) D.n_done += I - D.p + 1;
+                D.n_done = 0;  // counted with the update-matrix tiles below
             }
             // extend-add groups: consecutive columns of one front, about
             // EA_ROWS parent rows (each wave of the task takes every fourth column)
@@ -533,17 +533,25 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                             }
                     }
             }
+            // update-matrix tiles: 128 x 128 (two row blocks by two column
+            // blocks) when the k-extent nn is long enough to amortize them
             for (int32_t J : L.big_list) {
                 const Front& F = P.fronts[J];
-                const DagFront& D = P.dagf[J];
+                DagFront& D = P.dagf[J];
                 int32_t nj = F.nn + F.na;
-                for (int32_t Jc = D.p; Jc < D.R; ++Jc)
-                    for (int32_t I = Jc; I < D.R; ++I) {
-                        int32_t r = row0(F, D.p, I) - F.nn, c = row0(F, D.p, Jc) - F.nn;
-                        P.dag.push_back(DagTask{DT_SYRK, J, (int32_t)P.gemm.size(), I, Jc, 0});
+                const int32_t step = (F.nn >= WIDE_K && F.na > NB) ? 2 : 1;
+                auto urow = [&](int32_t I) { return I >= D.R ? F.na : row0(F, D.p, I) - F.nn; };
+                D.n_done = 0;
+                for (int32_t Jc = D.p; Jc < D.R; Jc += step)
+                    for (int32_t I = Jc; I < D.R; I += step) {
+                        const int32_t i2 = std::min(I + step, D.R), j2 = std::min(Jc + step, D.R);
+                        const int32_t r = urow(I), c = urow(Jc), m = urow(i2) - r, nc = urow(j2) - c;
+                        const int32_t fl = (i2 - I > 1 ? 1 : 0) | (j2 - Jc > 1 ? 2 : 0);
+                        P.dag.push_back(DagTask{DT_SYRK, J, (int32_t)P.gemm.size(), I, Jc, fl});
                         add_gemm(G_SUB, ARR_PANEL, F.poff + F.nn + r, nj, ARR_PANEL, F.poff + F.nn + c, nj,
-                                 ARR_U, F.uoff + r + (int64_t)c * F.na, F.na, rlen(F, D.p, I),
-                                 rlen(F, D.p, Jc), F.nn, F.c0, I == Jc ? 1 : 0);
+                                 ARR_U, F.uoff + r + (int64_t)c * F.na, F.na, m, nc, F.nn, F.c0,
+                                 I == Jc ? 1 : 0, step == 2 ? 1 : 0);
+                        D.n_done++;
                     }
             }
         }
@@ -620,7 +628,9 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             return off + (rows - 1) + (int64_t)(cols - 1) * ld < pool(a);
         };
         for (const GemmTask& t : P.gemm) {
-            if (t.m <= 0 || t.n <= 0 || t.k <= 0 || t.m > NB || t.n > NB) return "gemm tile shape";
+            const int32_t edge = t.wide ? 2 * NB : NB;
+            if (t.m <= 0 || t.n <= 0 || t.k <= 0 || t.m > edge || t.n > edge) return "gemm tile shape";
+            if (t.wide && (t.mode != G_SUB || t.arrC != ARR_U)) return "wide tile kind";
             if (!inside(t.arrA, t.a_off, t.m, t.k, t.lda) || !inside(t.arrB, t.b_off, t.n, t.k, t.ldb) ||
                 !inside(t.arrC, t.c_off, t.m, t.n, t.ldc))
                 return "gemm task out of bounds";
@@ -781,9 +791,13 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 }
             }
             if (d.kind == DT_SYRK) {
-                int32_t p = P.dagf[d.front].p;
-                if (need(d.b, p - 1) < 0 || need(d.b, p - 1) >= t || need(d.c, p - 1) < 0 || need(d.c, p - 1) >= t)
-                    return "dag: syrk before its rows";
+                int32_t p = P.dagf[d.front].p, R = P.dagf[d.front].R;
+                int32_t rows[4] = {d.b, d.c, (d.d & 1) ? d.b + 1 : d.b, (d.d & 2) ? d.c + 1 : d.c};
+                for (int32_t I : rows)
+                    if (I >= R || need(I, p - 1) < 0 || need(I, p - 1) >= t) return "dag: syrk before its rows";
+                const GemmTask& gt = P.gemm[d.a];
+                if ((gt.m > NB) != ((d.d & 1) != 0) || (gt.n > NB) != ((d.d & 2) != 0))
+                    return "dag: syrk tile and its row blocks disagree";
             }
         }
     }
