@@ -26,6 +26,7 @@ constexpr int SMALL_NJ = 64;    // fronts with nJ <= SMALL_NJ use the fused LDS 
 constexpr int EA_WIN = 256;     // parent rows per extend-add window (one wave)
 constexpr int EA_ROWS = 8192;   // parent rows per extend-add task (about)
 constexpr int WIDE_K = 128;     // update-matrix tiles are 128 x 128 when nn >= WIDE_K
+constexpr int SIM_WORKERS = 512;  // resident workgroups assumed when ordering the DAG tickets
 
 struct Front {
     int64_t poff;    // panel offset (elements) in the panel pool

@@ -13,8 +13,11 @@
 #include "mf_plan.h"
 
 #include <algorithm>
+#include <deque>
+#include <functional>
 #include <map>
 #include <numeric>
+#include <queue>
 
 namespace mf {
 
@@ -727,6 +730,262 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         }
     }
 
+    // ---- dataflow form of the DAG ------------------------------------------------
+    // Predecessor edges per tile: POTRF(K) <- last update of (K,K) (or every
+    // extend-add task of the front for K = 0); TRSM(I,K) <- POTRF(K), last
+    // update of (I,K); UPD(I,Jc,K) <- the final tiles (I,K), (Jc,K) and the
+    // previous update of (I,Jc); SYRK <- the final last-column tiles of its row
+    // blocks.  A host run of the dataflow checks that every task executes.
+    {
+        const int32_t NT = (int32_t)P.dag.size();
+        std::vector<int32_t> fin(P.dag_counters + 1, -1), lastw(P.dag_counters + 1, -1);
+        auto tile = [&](int32_t J, int32_t I, int32_t K) {
+            return (int64_t)P.dagf[J].toff + (int64_t)K * P.dagf[J].R + I;
+        };
+        P.gtask0.assign(NJ, 0);
+        P.gtask1.assign(NJ, 0);
+        std::vector<uint8_t> in_dag(NJ, 0);
+        for (int32_t t = 0; t < NT; ++t) {
+            const DagTask& d = P.dag[t];
+            in_dag[d.front] = 1;
+            if (d.kind == DT_EA || d.kind == DT_SMALL) {
+                if (P.gtask1[d.front] == 0) P.gtask0[d.front] = t;
+                else if (P.gtask1[d.front] != t) return "dataflow: gate tasks of a front not contiguous";
+                P.gtask1[d.front] = t + 1;
+            }
+        }
+        std::vector<std::pair<int32_t, int32_t>> edges;  // (successor, predecessor)
+        edges.reserve(4 * (size_t)NT);
+        auto edge = [&](int32_t pred, int32_t t) -> bool {
+            if (pred < 0 || pred >= t) return false;
+            edges.push_back({t, pred});
+            return true;
+        };
+        for (int32_t t = 0; t < NT; ++t) {
+            const DagTask& d = P.dag[t];
+            const int32_t J = d.front;
+            switch (d.kind) {
+                case DT_POTRF:
+                    if (d.a == 0) {
+                        for (int32_t e = P.gtask0[J]; e < P.gtask1[J]; ++e)
+                            if (!edge(e, t)) return "dataflow: potrf edge";
+                    } else if (!edge(lastw[tile(J, d.a, d.a)], t)) {
+                        return "dataflow: potrf without its update";
+                    }
+                    fin[tile(J, d.a, d.a)] = t;
+                    break;
+                case DT_TRSM:
+                    if (!edge(fin[tile(J, d.c, d.c)], t)) return "dataflow: trsm without potrf";
+                    if (d.c > 0 && !edge(lastw[tile(J, d.b, d.c)], t)) return "dataflow: trsm without its update";
+                    fin[tile(J, d.b, d.c)] = t;
+                    break;
+                case DT_UPD:
+                    if (!edge(fin[tile(J, d.b, d.d)], t) || !edge(fin[tile(J, d.c, d.d)], t))
+                        return "dataflow: update without its operands";
+                    if (d.d > 0 && !edge(lastw[tile(J, d.b, d.c)], t)) return "dataflow: update order";
+                    lastw[tile(J, d.b, d.c)] = t;
+                    break;
+                case DT_SYRK: {
+                    const int32_t p = P.dagf[J].p;
+                    const int32_t rows[4] = {d.b, d.c, (d.d & 1) ? d.b + 1 : d.b, (d.d & 2) ? d.c + 1 : d.c};
+                    for (int32_t I : rows)
+                        if (!edge(fin[tile(J, I, p - 1)], t)) return "dataflow: syrk without its rows";
+                    break;
+                }
+                default: break;
+            }
+        }
+        std::sort(edges.begin(), edges.end());
+        edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+        P.dep0.assign(NT, 0);
+        std::vector<int32_t> nsucc(NT + 1, 0);
+        for (auto& e : edges) {
+            P.dep0[e.first]++;
+            nsucc[e.second + 1]++;
+        }
+        P.succ_ptr.assign(NT + 1, 0);
+        for (int32_t t = 0; t < NT; ++t) P.succ_ptr[t + 1] = P.succ_ptr[t] + nsucc[t + 1];
+        P.succ.assign(edges.size(), 0);
+        {
+            std::vector<int32_t> pos(P.succ_ptr.begin(), P.succ_ptr.end() - 1);
+            for (auto& e : edges) P.succ[pos[e.second]++] = e.first;
+        }
+        // gates
+        P.gate0.assign(NJ, 0);
+        for (int32_t J = 0; J < NJ; ++J) {
+            if (!in_dag[J] || P.gtask1[J] == 0) continue;
+            const Front& F = P.fronts[J];
+            for (int32_t q = F.chd0; q < F.chd1; ++q)
+                if (in_dag[P.chd[q]]) P.gate0[J]++;
+            P.gate0[J] += P.uwait_ptr[J + 1] - P.uwait_ptr[J];
+        }
+        for (int32_t J = 0; J < NJ; ++J)
+            for (int32_t q = P.uwait_ptr[J]; q < P.uwait_ptr[J + 1]; ++q)
+                if (!in_dag[P.uwait[q]]) return "dataflow: U reuse waits on a front outside the DAG";
+        P.uwaiters_ptr.assign(NJ + 1, 0);
+        for (int32_t J = 0; J < NJ; ++J)
+            for (int32_t q = P.uwait_ptr[J]; q < P.uwait_ptr[J + 1]; ++q) P.uwaiters_ptr[P.uwait[q] + 1]++;
+        for (int32_t J = 0; J < NJ; ++J) P.uwaiters_ptr[J + 1] += P.uwaiters_ptr[J];
+        P.uwaiters.assign(P.uwait.size(), 0);
+        {
+            std::vector<int32_t> pos(P.uwaiters_ptr.begin(), P.uwaiters_ptr.end() - 1);
+            for (int32_t J = 0; J < NJ; ++J)
+                for (int32_t q = P.uwait_ptr[J]; q < P.uwait_ptr[J + 1]; ++q) P.uwaiters[pos[P.uwait[q]]++] = J;
+        }
+        // priority: the panel's critical path (extend-add, diagonal tiles,
+        // panel solves, the look-ahead column) before trailing and update-matrix tiles
+        P.hi.assign(NT, 0);
+        P.n_hi = P.n_lo = 0;
+        for (int32_t t = 0; t < NT; ++t) {
+            const DagTask& d = P.dag[t];
+            const bool h = d.kind == DT_EA || d.kind == DT_SMALL || d.kind == DT_POTRF ||
+                           d.kind == DT_TRSM || (d.kind == DT_UPD && d.c == d.d + 1);
+            P.hi[t] = h ? 1 : 0;
+            (h ? P.n_hi : P.n_lo)++;
+        }
+        auto gated = [&](int32_t t) { return P.dag[t].kind == DT_EA || P.dag[t].kind == DT_SMALL; };
+        P.init_hi.clear();
+        P.init_lo.clear();
+        for (int32_t t = 0; t < NT; ++t) {
+            const bool ready = gated(t) ? P.gate0[P.dag[t].front] == 0 : P.dep0[t] == 0;
+            if (ready) (P.hi[t] ? P.init_hi : P.init_lo).push_back(t);
+        }
+        // host run of the dataflow (the kernel's completion rules)
+        std::vector<int32_t> dep(P.dep0), gate(P.gate0), eacnt(NJ, 0), done(NJ, 0), seen(NT, 0);
+        std::deque<int32_t> qh(P.init_hi.begin(), P.init_hi.end()), ql(P.init_lo.begin(), P.init_lo.end());
+        int64_t executed = 0;
+        auto push = [&](int32_t s) { (P.hi[s] ? qh : ql).push_back(s); };
+        std::function<void(int32_t)> gate_dec = [&](int32_t G) {
+            if (--gate[G] == 0)
+                for (int32_t e = P.gtask0[G]; e < P.gtask1[G]; ++e) push(e);
+        };
+        auto consumed = [&](int32_t c) {
+            for (int32_t q = P.uwaiters_ptr[c]; q < P.uwaiters_ptr[c + 1]; ++q) gate_dec(P.uwaiters[q]);
+        };
+        auto front_done = [&](int32_t J) {
+            const int32_t par = P.fronts[J].parent;
+            if (par >= 0) {
+                if (!in_dag[par] || P.gtask1[par] == 0) return false;
+                gate_dec(par);
+            }
+            return true;
+        };
+        while (!qh.empty() || !ql.empty()) {
+            std::deque<int32_t>& q = qh.empty() ? ql : qh;
+            const int32_t t = q.front();
+            q.pop_front();
+            if (seen[t]++) return "dataflow: task released twice";
+            ++executed;
+            for (int32_t k = P.succ_ptr[t]; k < P.succ_ptr[t + 1]; ++k)
+                if (--dep[P.succ[k]] == 0) push(P.succ[k]);
+            const DagTask& d = P.dag[t];
+            if (d.kind == DT_EA) {
+                if (++eacnt[d.front] == P.dagf[d.front].n_ea) consumed(d.front);
+            } else if (d.kind == DT_SMALL) {
+                consumed(d.front);
+                if (!front_done(d.front)) return "dataflow: parent of a small front has no gate";
+            } else if (d.kind == DT_SYRK) {
+                if (++done[d.front] == P.dagf[d.front].n_done && !front_done(d.front))
+                    return "dataflow: parent of a front has no gate";
+            }
+        }
+        if (executed != NT) return "dataflow: not every task runs (cycle or missing release)";
+        if ((int64_t)P.init_hi.size() > P.n_hi || (int64_t)P.init_lo.size() > P.n_lo) return "dataflow: queue sizes";
+    }
+
+    // ---- ticket order by list scheduling ------------------------------------------
+    // The persistent kernel hands out tasks in ticket order and a task waits
+    // for its predecessors, so the order decides how much the workgroups wait.
+    // Simulate the launch on the host: SIM_WORKERS workgroups, an estimated
+    // duration per task, ready tasks taken by longest remaining path to the end
+    // (critical path first).  The simulated start order is a topological order
+    // (a task starts after its predecessors finish) and becomes the tickets.
+    {
+        const int32_t NT = (int32_t)P.dag.size();
+        std::vector<double> cost(NT), bl(NT, 0.0), bgate(NJ, 0.0);
+        for (int32_t t = 0; t < NT; ++t) {
+            const DagTask& d = P.dag[t];
+            double c = 10.0;
+            switch (d.kind) {
+                case DT_EA: {
+                    double rows = 0;
+                    for (int32_t e = d.a; e < d.b; ++e) rows += P.ea[e].row1 - P.ea[e].row0;
+                    c = 3.0 + 24.0 * rows / 40000.0;
+                    break;
+                }
+                case DT_SMALL: c = 25.0; break;
+                case DT_POTRF: c = 40.0; break;
+                default: {
+                    const GemmTask& g = P.gemm[d.a];
+                    c = 8.0 + 2.0 * g.m * g.n * (double)g.k / (g.wide ? 1.2e5 : 1.0e5);
+                    break;
+                }
+            }
+            cost[t] = c;
+        }
+        // longest path to the end, in reverse topological (static) order
+        for (int32_t t = NT - 1; t >= 0; --t) {
+            const DagTask& d = P.dag[t];
+            double m = 0;
+            for (int32_t k = P.succ_ptr[t]; k < P.succ_ptr[t + 1]; ++k) m = std::max(m, bl[P.succ[k]]);
+            const int32_t J = d.front;
+            if (d.kind == DT_SYRK || d.kind == DT_SMALL) {
+                const int32_t par = P.fronts[J].parent;
+                if (par >= 0) m = std::max(m, bgate[par]);
+            }
+            if (d.kind == DT_EA || d.kind == DT_SMALL)
+                for (int32_t q = P.uwaiters_ptr[J]; q < P.uwaiters_ptr[J + 1]; ++q)
+                    m = std::max(m, bgate[P.uwaiters[q]]);
+            bl[t] = cost[t] + m;
+            if (d.kind == DT_EA || d.kind == DT_SMALL) bgate[J] = std::max(bgate[J], bl[t]);
+        }
+        typedef std::pair<double, int32_t> Item;
+        std::priority_queue<Item> ready;                                     // (bl, -t)
+        std::priority_queue<Item, std::vector<Item>, std::greater<Item>> run;  // (finish, t)
+        std::vector<int32_t> dep(P.dep0), gate(P.gate0), eacnt(NJ, 0), done(NJ, 0);
+        auto push = [&](int32_t s) { ready.push({bl[s], -s}); };
+        for (int32_t t : P.init_hi) push(t);
+        for (int32_t t : P.init_lo) push(t);
+        auto gate_dec = [&](int32_t G) {
+            if (--gate[G] == 0)
+                for (int32_t e = P.gtask0[G]; e < P.gtask1[G]; ++e) push(e);
+        };
+        std::vector<int32_t> order;
+        order.reserve(NT);
+        double now = 0;
+        int32_t busy = 0;
+        while ((int32_t)order.size() < NT) {
+            while (busy < SIM_WORKERS && !ready.empty()) {
+                const int32_t t = -ready.top().second;
+                ready.pop();
+                order.push_back(t);
+                run.push({now + cost[t], t});
+                ++busy;
+            }
+            if (run.empty()) return "list schedule: no runnable task";
+            const Item fin = run.top();
+            run.pop();
+            now = fin.first;
+            --busy;
+            const int32_t t = fin.second;
+            for (int32_t k = P.succ_ptr[t]; k < P.succ_ptr[t + 1]; ++k)
+                if (--dep[P.succ[k]] == 0) push(P.succ[k]);
+            const DagTask& d = P.dag[t];
+            const int32_t J = d.front;
+            bool consumed = false, fdone = false;
+            if (d.kind == DT_EA) consumed = ++eacnt[J] == P.dagf[J].n_ea;
+            else if (d.kind == DT_SMALL) consumed = fdone = true;
+            else if (d.kind == DT_SYRK) fdone = ++done[J] == P.dagf[J].n_done;
+            if (consumed)
+                for (int32_t q = P.uwaiters_ptr[J]; q < P.uwaiters_ptr[J + 1]; ++q) gate_dec(P.uwaiters[q]);
+            if (fdone && P.fronts[J].parent >= 0) gate_dec(P.fronts[J].parent);
+        }
+        P.sim_makespan_us = now;
+        std::vector<DagTask> dag2(NT);
+        for (int32_t i = 0; i < NT; ++i) dag2[i] = P.dag[order[i]];
+        P.dag.swap(dag2);
+    }
+
     // ---- DAG order validation: every wait targets an earlier ticket ------------
     {
         const int32_t NT = (int32_t)P.dag.size();
@@ -830,10 +1089,18 @@ extern "C" int mf_plan_check(int64_t n, int64_t nfr, const int64_t* resptr, cons
         std::copy(err.begin(), err.begin() + k, msg);
         msg[k] = 0;
     }
+    int64_t ea_u = 0, ea_p = 0, ea_src_rows = 0;
+    for (const mf::EaTask& e : P.ea) {
+        const mf::Front& F = P.fronts[e.front];
+        (e.col >= F.nn ? ea_u : ea_p) += e.row1 - e.row0;
+        for (int32_t q = e.src0; q < e.src1; ++q)
+            ea_src_rows += P.fronts[P.ea_src[2 * q]].na - P.ea_src[2 * q + 1];
+    }
     int64_t v[] = {(int64_t)P.fronts.size(), P.n_small_region, P.n_big, (int64_t)P.levels.size(),
                    P.panel_elems, P.u_elems, P.w_elems, (int64_t)P.gemm.size(),
                    (int64_t)P.ea.size(), (int64_t)(P.ftask.size() + P.btask.size()), (int64_t)P.ref_flops,
-                   (int64_t)P.engine_flops, P.max_nj, (int64_t)P.dag.size(), P.dag_counters};
+                   (int64_t)P.engine_flops, P.max_nj, (int64_t)P.dag.size(), P.dag_counters,
+                   (int64_t)P.sim_makespan_us, ea_u, ea_p, ea_src_rows};
     for (int64_t i = 0; stats && i < nstats && i < (int64_t)(sizeof(v) / sizeof(v[0])); ++i)
         stats[i] = v[i];
     return err.empty() ? 0 : -1;

@@ -53,6 +53,18 @@ struct HostPlan {
     int64_t dag_counters = 0;          // tile counters
     std::vector<int32_t> uwait_ptr, uwait;  // per front: fronts whose extend-add must end
                                             // before the front's U slot is written
+    // dataflow form of the DAG: a task runs once its predecessors are done;
+    // the extend-add (or small-front) tasks of a front are released together
+    // by the front's gate (its children done, reused U slots consumed)
+    std::vector<int32_t> dep0;              // per task: number of predecessor tasks
+    std::vector<int32_t> succ_ptr, succ;    // successors (CSR over tasks)
+    std::vector<int32_t> gate0;             // per front: children in the DAG + U-slot waits
+    std::vector<int32_t> gtask0, gtask1;    // per front: the tasks its gate releases
+    std::vector<int32_t> uwaiters_ptr, uwaiters;  // reverse of uwait
+    std::vector<int8_t> hi;                 // per task: 1 = high-priority queue
+    std::vector<int32_t> init_hi, init_lo;  // tasks ready at the start, per queue
+    int32_t n_hi = 0, n_lo = 0;             // tasks per queue
+    double sim_makespan_us = 0;             // list-scheduled length of the DAG (estimate)
     std::vector<int32_t> ntf, ntb;     // tasks per front (completion counts)
     std::vector<int64_t> mvoff1;       // per internal front: offset of Mv (na x 1)
     // copy-out

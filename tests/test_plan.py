@@ -11,7 +11,7 @@ def plan_check(ct, S):
     lib = _native.sym()
     arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in
             [S.resptr, S.sepptr, S.septgt, S.reltgt, S.chdptr, S.chdtgt, S.Dptr, S.Lptr]]
-    st = np.zeros(15, dtype=np.int64)
+    st = np.zeros(19, dtype=np.int64)
     msg = ctypes.create_string_buffer(256)
     rc = lib.mf_plan_check(S.n, S.nfr, *[a.ctypes.data_as(_native.i64p) for a in arrs],
                            st.ctypes.data_as(_native.i64p), len(st), msg, 256)
