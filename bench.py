@@ -31,6 +31,7 @@ Other workloads (--workload):
   cfg5    serving: Poisson stream of KKT LDLt requests, patterns reused over
           20 value updates, continuous batching; p50/p99 latency, solves/s
           against one-at-a-time serving
+  copy    a 1 GiB device copy (calibration of the HBM traffic counters)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -245,12 +246,22 @@ def run_single(args, ct, torch, name):
     times, cnts, work = read_timing(ct, lib, _native, h)
     E.c(lib.mf_timing(h, 0))
     if args.dag_profile:
-        prof = np.zeros(18)
+        prof = np.zeros(21)
         E.c(lib.mf_dag_profile_read(h, prof.ctypes.data_as(_native.f64p), 1))
         for k, kname in enumerate(["ea", "small", "potrf", "trsm", "upd", "syrk"]):
             w, b, c = prof[3 * k:3 * k + 3] / args.steps
             print(f"dag {kname:6s} tasks/step {c:9.0f}  wait {w / 1e3:9.2f} ms  work {b / 1e3:9.2f} ms"
                   f"  work/task {b / max(c, 1):8.2f} us", file=sys.stderr)
+        if args.trace_out:
+            nt = int(E.stats["dag_tasks"])
+            tasks = np.zeros(6 * nt, dtype=np.int32)
+            tr = np.zeros(2 * nt, dtype=np.uint64)
+            E.c(E.lib.mf_plan_dag(E.plan.handle, tasks.ctypes.data_as(ctypes.c_void_p), len(tasks)))
+            E.c(E.lib.mf_dag_trace_read(h, tr.ctypes.data_as(ctypes.c_void_p), len(tr)))
+            np.savez_compressed(args.trace_out, tasks=tasks.reshape(-1, 6), trace=tr.reshape(-1, 2))
+        rel, acq, tick = prof[18:21] / args.steps
+        print(f"dag sync: release {rel / 1e3:.2f} ms  acquire {acq / 1e3:.2f} ms  ticket {tick / 1e3:.2f} ms"
+              f" (summed over workgroups, per step)", file=sys.stderr)
     info = E.info()
     X = E.solution()
     res = np.linalg.norm(A @ X - B0) / np.linalg.norm(B0)
@@ -419,6 +430,23 @@ def run_serving(args, ct, torch):
     }
 
 
+def run_copy(args, ct, torch):
+    """A plain device copy of 1 GiB (read 1 GiB, write 1 GiB): the reference
+    kernel whose FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic
+    counters of the engine's kernels."""
+    n = (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    ms, _ = time_steps(torch, stream, lambda: dst.copy_(src), args.steps, args.warmup)
+    return {"metric": "device copy GB/s (read + write)", "value": round(2 * (1 << 30) / (ms * 1e-3) / 1e9, 1),
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "torch copy_ of 1 GiB float64 (copy-kernel calibration)",
+                       "bytes_read": 1 << 30, "bytes_written": 1 << 30}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -428,6 +456,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dag-profile", action="store_true",
                     help="per task kind wait/work time of the tile-DAG kernel (stderr)")
+    ap.add_argument("--trace-out", default=None,
+                    help="with --dag-profile: save the per-ticket start/end times (npz)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs "
@@ -436,7 +466,9 @@ def main():
 
     import torch
     ct = load_package()
-    if args.workload == "cfg4":
+    if args.workload == "copy":
+        out = run_copy(args, ct, torch)
+    elif args.workload == "cfg4":
         out = run_batched(args, ct, torch)
     elif args.workload == "cfg5":
         out = run_serving(args, ct, torch)

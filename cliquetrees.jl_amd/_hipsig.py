@@ -36,6 +36,8 @@ SIGNATURES = {
     "mf_refine_segmented": ([vp, i64p, vp, i64, vp, i64, i64, ci, cd, ci, i64, i64p,
                              ctypes.POINTER(ci), f64p], ci),
     "mf_dag_profile_read": ([vp, f64p, ci], ci),
+    "mf_dag_trace_read": ([vp, vp, i64], ci),
+    "mf_plan_dag": ([vp, vp, i64], ci),
 }
 
 # Names of the C-ABI entry points declared in include/mfhip.h.

@@ -99,6 +99,7 @@ struct mf_factor {
     void* ir_ws = nullptr;     // B copy, X, R (n x nrhs each) + norms
     size_t ir_ws_bytes = 0;
     unsigned long long* prof_buf = nullptr;
+    unsigned long long* trace_buf = nullptr;   // per DAG ticket: start, end
     int64_t* tgt_panel = nullptr;
     int64_t* tgt_ref = nullptr;
     int64_t nnz = -1;
@@ -230,7 +231,7 @@ void enqueue_factor(mf_factor* f) {
         launch_fill<int32_t>(f->cnt, H.dag_counters, 0, s);
         launch_fill<int32_t>(f->eacnt, std::max<int>(P.nJ, 1), 0, s);
         launch_fill<uint32_t>(f->ticket, 1, 0u, s);
-        DevDag D{pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->prof,
+        DevDag D{pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->prof, f->prof ? f->trace_buf : nullptr,
                  (int32_t)H.dag.size()};
         timed(f, MF_KC_DAG, H.engine_flops - pl->small_flops, 1, [&] { launch_dag<T>(P, F, D, s); });
     }
@@ -358,7 +359,7 @@ int mf_plan_stats(const mf_plan* pl, int64_t* st, int64_t nst) {
                    H.w_elems, (int64_t)H.gemm.size(), (int64_t)H.ea.size(),
                    (int64_t)H.diag.size(), (int64_t)llround(H.ref_flops),
                    (int64_t)llround(H.engine_flops), (int64_t)llround(H.gemm_flops), H.ndz,
-                   H.nlz};
+                   H.nlz, (int64_t)H.dag.size()};
     int64_t m = (int64_t)(sizeof(v) / sizeof(v[0]));
     for (int64_t i = 0; i < nst && i < m; ++i) st[i] = v[i];
     return MF_OK;
@@ -833,32 +834,59 @@ int mf_dag_profile(mf_factor* f, int enable) {
     if (!f) return fail(MF_EARG, "null factor");
     if (enable && !f->prof) {
         void* p = nullptr;
-        if (hipMalloc(&p, 8 * MF_DT_KINDS * 3) != hipSuccess) return fail(MF_ENOMEM, "hipMalloc failed (profile)");
+        if (hipMalloc(&p, 8 * (MF_DT_KINDS + 1) * 3) != hipSuccess) return fail(MF_ENOMEM, "hipMalloc failed (profile)");
         f->allocs.push_back(p);
         f->prof_buf = (unsigned long long*)p;
-        launch_fill<unsigned long long>(f->prof_buf, MF_DT_KINDS * 3, 0ull, f->stream);
+        launch_fill<unsigned long long>(f->prof_buf, (MF_DT_KINDS + 1) * 3, 0ull, f->stream);
+        const size_t nt = f->plan->H.dag.size();
+        if (hipMalloc(&p, 16 * (nt + 1)) != hipSuccess) return fail(MF_ENOMEM, "hipMalloc failed (trace)");
+        f->allocs.push_back(p);
+        f->trace_buf = (unsigned long long*)p;
+        launch_fill<unsigned long long>(f->trace_buf, 2 * (nt + 1), 0ull, f->stream);
     }
     f->prof = enable ? f->prof_buf : nullptr;
     return MF_OK;
 }
 
+int mf_dag_trace_read(mf_factor* f, unsigned long long* out, int64_t n) {
+    if (!f || !out) return fail(MF_EARG, "null argument");
+    const int64_t nt = (int64_t)f->plan->H.dag.size();
+    if (n < 2 * nt) return fail(MF_EARG, "trace buffer too small (2 x DAG tasks)");
+    if (!f->trace_buf) return fail(MF_ESTATE, "profiling is off");
+    HIPCHK(hipMemcpyAsync(out, f->trace_buf, 16 * nt, hipMemcpyDeviceToHost, f->stream));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    return MF_OK;
+}
+
+int mf_plan_dag(const mf_plan* pl, int32_t* tasks, int64_t n) {
+    if (!pl || !tasks) return fail(MF_EARG, "null argument");
+    const HostPlan& H = pl->H;
+    if (n < 6 * (int64_t)H.dag.size()) return fail(MF_EARG, "buffer too small (6 x DAG tasks)");
+    for (size_t t = 0; t < H.dag.size(); ++t) {
+        const DagTask& d = H.dag[t];
+        const int32_t v[6] = {d.kind, d.front, d.a, d.b, d.c, d.d};
+        std::copy(v, v + 6, tasks + 6 * t);
+    }
+    return MF_OK;
+}
+
 int mf_dag_profile_read(mf_factor* f, double* out, int reset) {
     if (!f || !out) return fail(MF_EARG, "null argument");
-    for (int i = 0; i < MF_DT_KINDS * 3; ++i) out[i] = 0;
+    for (int i = 0; i < (MF_DT_KINDS + 1) * 3; ++i) out[i] = 0;
     if (!f->prof_buf) return MF_OK;
-    unsigned long long v[MF_DT_KINDS * 3];
+    unsigned long long v[(MF_DT_KINDS + 1) * 3];
     HIPCHK(hipMemcpyAsync(v, f->prof_buf, sizeof(v), hipMemcpyDeviceToHost, f->stream));
     HIPCHK(hipStreamSynchronize(f->stream));
     int dev = 0, khz = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     const double us = khz > 0 ? 1e3 / khz : 0.0;
-    for (int k = 0; k < MF_DT_KINDS; ++k) {
+    for (int k = 0; k <= MF_DT_KINDS; ++k) {
         out[3 * k] = v[3 * k] * us;
         out[3 * k + 1] = v[3 * k + 1] * us;
-        out[3 * k + 2] = (double)v[3 * k + 2];
+        out[3 * k + 2] = k < MF_DT_KINDS ? (double)v[3 * k + 2] : v[3 * k + 2] * us;
     }
-    if (reset) launch_fill<unsigned long long>(f->prof_buf, MF_DT_KINDS * 3, 0ull, f->stream);
+    if (reset) launch_fill<unsigned long long>(f->prof_buf, (MF_DT_KINDS + 1) * 3, 0ull, f->stream);
     return MF_OK;
 }
 

@@ -69,7 +69,20 @@ __device__ __forceinline__ double regularize(const DevFactor<T>& F, double s, do
 // L2.  Before the closing barrier of a task every wave drains, so that thread
 // 0's agent-scope release (an L2 write-back) covers the whole workgroup's data.
 __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0); }
-__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// Agent-scope acquire (L1/L2 invalidate) and release (L2 write-back), each
+// followed by an explicit wait: the compiler omits the wait after the
+// write-back when its own counters are already zero (after drain_stores), and
+// a flag stored before the write-back completes can be seen by a workgroup on
+// another XCD ahead of the data.  The wait after the invalidate makes it
+// complete before the barrier releases the workgroup's other waves.
+__device__ __forceinline__ void acquire_agent() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
+}
+__device__ __forceinline__ void release_agent() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
+}
 
 // Bounded wait until *flag >= target; gives up (and records it) after ~1 s.
 __device__ __forceinline__ void wait_ge(int32_t* flag, int32_t target, int32_t* hang) {
@@ -88,6 +101,10 @@ __device__ __forceinline__ void wait_ge(int32_t* flag, int32_t target, int32_t* 
     }
 }
 
+__device__ __forceinline__ int ld_rel(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 constexpr int LDS = SMALL_NJ + 1;  // leading dimension of LDS fronts and tiles
 constexpr int KC = 32;             // k-chunk of the GEMM tile
 constexpr int LDT = 64 + 4;        // leading dimension of the GEMM LDS stages
@@ -96,7 +113,7 @@ constexpr int LDT = 64 + 4;        // leading dimension of the GEMM LDS stages
 // nj x nj front M (lower triangle, leading dimension LDS) in LDS.
 // Returns after a failed pivot (recorded in F.info).
 template <typename T>
-__device__ void factor_lds(T* M, int nn, int nj, int gc0, const DevFactor<T>& F,
+__device__ __forceinline__ void factor_lds(T* M, int nn, int nj, int gc0, const DevFactor<T>& F,
                            T* s_piv, T* s_inv, int* s_fail, const int tid) {
     for (int k = 0; k < nn; ++k) {
         if (tid == 0) {
@@ -135,7 +152,7 @@ __device__ void factor_lds(T* M, int nn, int nj, int gc0, const DevFactor<T>& F,
 // matrices, assembly with the panel's A entries, partial factorization,
 // write-out of the panel and of U.  The caller has made the children final.
 template <typename T>
-__device__ void small_front(const DevPlan& P, const DevFactor<T>& F, int J, T* M, T* s_piv,
+__device__ __forceinline__ void small_front(const DevPlan& P, const DevFactor<T>& F, int J, T* M, T* s_piv,
                             T* s_inv, int* s_fail, const int tid) {
     const Front f = P.fronts[J];
     const int nn = f.nn, na = f.na, nj = nn + na;
@@ -196,7 +213,10 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
         drain_stores();
         __syncthreads();
         if (!persistent) break;
-        if (tid == 0) __hip_atomic_store(F.flags + J, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) {
+            release_agent();
+            __hip_atomic_store(F.flags + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         // closes the thread-0 block before the back-edge: every thread of the
         // workgroup reaches the next ticket together
         __syncthreads();
@@ -211,7 +231,7 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
 // [wstart[w], wstart[w+1]) (rel is increasing), so no search is needed; a
 // wavefront's LDS operations complete in order, so no barrier is needed.
 template <typename T>
-__device__ void ea_column(const DevPlan& P, const DevFactor<T>& F, const EaTask& e, T* acc,
+__device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& F, const EaTask& e, T* acc,
                           int lane) {
     const Front f = P.fronts[e.front];
     const int nj = f.nn + f.na, pc = e.col;
@@ -339,12 +359,168 @@ __device__ void potrf_tile(const DevFactor<T>& F, const Front& f, int k0, bool i
     }
 }
 
+constexpr int PB = 16;                          // block size inside the diagonal tile
+constexpr int PTILE = NB * LDS + 3 * PB * PB + PB + NB;  // LDS elements of potrf_blocked
+
+// Diagonal tile, blocked: the 64 x 64 tile lives in LDS and is factored in
+// four 16-column block steps (potrf2!'s right-looking recursion cut at 16):
+// (1) one wavefront factors the 16 x 16 diagonal block in registers with the
+// pivot rule, (2) one lane per row solves the rows below against it, (3) all
+// threads apply the rank-16 update to the trailing lower triangle.  Then
+// W = L^-1 in place: the four diagonal blocks are inverted one per
+// wavefront, and the block rows below follow in three stages of
+// W_ij = -W_ii (sum_k L_ik W_kj).  About 20 barriers instead of one per
+// column.  L goes to the panel and W (ld = nn, zeros above the diagonal) to
+// the W pool.  LDS: PTILE elements.
+template <typename T>
+__device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front& f, int k0, bool invert_only,
+                              T* lds, const int tid) {
+    T* A = lds;                       // 64 x LDS, column-major
+    T* Tm = lds + NB * LDS;           // 3 blocks of 16 x 16 (stage products)
+    T* colv = Tm + 3 * PB * PB;       // one column of the diagonal block
+    T* pv = colv + PB;                // pivots of the tile (LDLt scaling)
+    const int nj = f.nn + f.na, kb = min(NB, f.nn - k0), gc0 = f.c0 + k0;
+    T* blk = F.panel + f.poff + k0 + (int64_t)k0 * nj;
+    const int r = tid & 63, cg = tid >> 6, lane = tid & 63;
+    const bool chol = F.diag == 0;
+    for (int c = cg; c < NB; c += 4) {
+        const T* colp = blk + (int64_t)c * nj;
+        A[r + c * LDS] = (r < kb && c < kb && c <= r) ? colp[r] : (r == c ? (T)1 : (T)0);
+    }
+    if (tid < NB) pv[tid] = (T)1;
+    __syncthreads();
+    if (!invert_only) {
+        for (int b = 0; b < NB / PB; ++b) {
+            const int c0 = b * PB;
+            if (c0 >= kb) break;  // uniform
+            // (1) diagonal block by wavefront 0: lane i < 16 holds row c0 + i
+            if (tid < 64) {
+                T x[PB];
+#pragma unroll
+                for (int j = 0; j < PB; ++j) x[j] = lane < PB ? A[(c0 + lane) + (c0 + j) * LDS] : (T)0;
+#pragma unroll
+                for (int k = 0; k < PB; ++k) {
+                    if (c0 + k >= kb) continue;  // uniform
+                    if (lane < PB) colv[lane] = x[k];
+                    const T akk = colv[k];
+                    double piv = regularize(F, sign_of(F, gc0 + c0 + k), (double)akk);
+                    const bool bad = (piv == 0.0);
+                    if (bad) piv = 1.0;  // keeps the rest finite after a failure
+                    if (lane == 0) {
+                        if (bad) atomicMin(F.info, (unsigned long long)(gc0 + c0 + k + 1));
+                        else if (!chol) F.d[gc0 + c0 + k] = (T)piv;
+                        pv[c0 + k] = (T)piv;
+                    }
+                    const T ip = (T)(1.0 / piv), ark = x[k], ar = ark * ip;
+#pragma unroll
+                    for (int j = k + 1; j < PB; ++j) x[j] -= ar * colv[j];
+                    if (lane >= k) {
+                        if (chol) {
+                            const T sq = (T)sqrt(piv);
+                            x[k] = (lane == k) ? sq : ark * ((T)1 / sq);
+                        } else if (lane > k) {
+                            x[k] = ar;  // LDLt keeps the unregularized pivot on the diagonal
+                        }
+                    }
+                }
+                if (lane < PB)
+#pragma unroll
+                    for (int j = 0; j < PB; ++j)
+                        if (j <= lane) A[(c0 + lane) + (c0 + j) * LDS] = x[j];
+            }
+            __syncthreads();
+            // (2) rows below: y L_D' = a (unit L_D for LDLt), then l = y / d for LDLt
+            const int rb = c0 + PB;
+            if (tid < 64 && rb + lane < NB) {
+                const int i = rb + lane;
+                T y[PB];
+#pragma unroll
+                for (int j = 0; j < PB; ++j) y[j] = A[i + (c0 + j) * LDS];
+#pragma unroll
+                for (int k = 0; k < PB; ++k) {
+                    if (chol) y[k] *= (T)1 / A[(c0 + k) + (c0 + k) * LDS];
+#pragma unroll
+                    for (int j = k + 1; j < PB; ++j) y[j] -= y[k] * A[(c0 + j) + (c0 + k) * LDS];
+                }
+#pragma unroll
+                for (int k = 0; k < PB; ++k) A[i + (c0 + k) * LDS] = chol ? y[k] : y[k] / pv[c0 + k];
+            }
+            __syncthreads();
+            // (3) trailing update of the lower triangle: A_ij -= sum_k L_ik s_k L_jk
+            const int nr = NB - rb;
+            for (int e = tid; e < nr * nr; e += 256) {
+                const int i = rb + e % nr, j = rb + e / nr;
+                if (j > i) continue;
+                T acc = (T)0;
+#pragma unroll
+                for (int k = 0; k < PB; ++k)
+                    acc += A[i + (c0 + k) * LDS] * (chol ? A[j + (c0 + k) * LDS]
+                                                        : A[j + (c0 + k) * LDS] * pv[c0 + k]);
+                A[i + j * LDS] -= acc;
+            }
+            __syncthreads();
+        }
+        for (int c = cg; c < kb; c += 4) {
+            T* colp = blk + (int64_t)c * nj;
+            if (r < kb && c <= r) colp[r] = A[r + c * LDS];
+        }
+        __syncthreads();  // L is read out before the inverse overwrites it
+    }
+    // W = L^-1 in place.  Diagonal blocks: wavefront w inverts block w; lane
+    // c < 16 keeps column c of W_ww (forward substitution, unit for LDLt).
+    {
+        const int bw = cg, c0 = bw * PB;
+        T w[PB];
+        if (lane < PB) {
+#pragma unroll
+            for (int i = 0; i < PB; ++i) w[i] = (i == lane) ? (T)1 : (T)0;
+#pragma unroll
+            for (int k = 0; k < PB; ++k) {
+                if (chol) w[k] *= (T)1 / A[(c0 + k) + (c0 + k) * LDS];
+#pragma unroll
+                for (int i = k + 1; i < PB; ++i) w[i] -= A[(c0 + i) + (c0 + k) * LDS] * w[k];
+            }
+        }
+        // a wavefront's LDS reads complete before its writes: only this
+        // wavefront touches block (bw, bw) here
+        if (lane < PB)
+#pragma unroll
+            for (int i = 0; i < PB; ++i) A[(c0 + i) + (c0 + lane) * LDS] = (i >= lane) ? w[i] : (T)0;
+    }
+    __syncthreads();
+    for (int bi = 1; bi < NB / PB; ++bi) {
+        // T_ij = sum_{k=j}^{i-1} L_ik W_kj for j < i (W_kj rows done, L_ik intact)
+        for (int e = tid; e < bi * PB * PB; e += 256) {
+            const int bj = e / (PB * PB), rr = e % PB, cc = (e / PB) % PB;
+            const int i = bi * PB + rr, j = bj * PB + cc;
+            T acc = (T)0;
+            for (int m = bj * PB; m < bi * PB; ++m) acc += A[i + m * LDS] * A[m + j * LDS];
+            Tm[bj * PB * PB + rr + cc * PB] = acc;
+        }
+        __syncthreads();
+        // W_ij = -W_ii T_ij
+        for (int e = tid; e < bi * PB * PB; e += 256) {
+            const int bj = e / (PB * PB), rr = e % PB, cc = (e / PB) % PB;
+            const int i = bi * PB + rr, j = bj * PB + cc;
+            T acc = (T)0;
+#pragma unroll
+            for (int m = 0; m < PB; ++m) acc += A[i + (bi * PB + m) * LDS] * Tm[bj * PB * PB + m + cc * PB];
+            A[i + j * LDS] = -acc;
+        }
+        __syncthreads();
+    }
+    T* W = F.W + f.woff + k0;
+    for (int c = cg; c < kb; c += 4) {
+        T* colp = W + (int64_t)c * f.nn;
+        if (r < kb) colp[r] = (c <= r) ? A[r + c * LDS] : (T)0;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag(DevPlan P, DevFactor<T> F, int t0, int invert_only) {
-    __shared__ T Ms[NB * LDS];
-    __shared__ T cb[2 * NB];
+    __shared__ T lds[PTILE];
     const DiagTask dt = P.diag[t0 + blockIdx.x];
-    potrf_tile<T>(F, P.fronts[dt.front], dt.k0, invert_only != 0, Ms, cb, (int)threadIdx.x);
+    potrf_blocked<T>(F, P.fronts[dt.front], dt.k0, invert_only != 0, lds, (int)threadIdx.x);
 }
 
 template <typename T>
@@ -359,7 +535,7 @@ __device__ __forceinline__ T* arr_base(const DevFactor<T>& F, int a) {
 // MFMAs consume the current chunk from LDS (As/Bs: KC x LDT each).  For
 // C -= A B' the C tile is fetched before the k-loop.
 template <typename T>
-__device__ void gemm_tile(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs, const int tid) {
+__device__ __forceinline__ void gemm_tile(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs, const int tid) {
     typedef typename Mfma<T>::v4 v4;
     constexpr int PER = KC * 64 / 256;  // elements of A (and of B) per thread and chunk
     const T* A = arr_base(F, g.arrA) + g.a_off;
@@ -463,7 +639,7 @@ constexpr int LDW = 128 + 4;  // leading dimension of its LDS stages
 // and B through LDS (As/Bs: KW x LDW) with the next chunk prefetched into
 // registers; C is updated after the k-loop in four row-block rounds.
 template <typename T>
-__device__ void gemm_tile128(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs, const int tid) {
+__device__ __forceinline__ void gemm_tile128(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs, const int tid) {
     typedef typename Mfma<T>::v4 v4;
     constexpr int PER = KW * 128 / 256;
     const T* A = arr_base(F, g.arrA) + g.a_off;
@@ -546,109 +722,178 @@ __device__ void gemm_tile128(const DevFactor<T>& F, const GemmTask& g, T* As, T*
     }
 }
 
-// Persistent tile-DAG factorization of the big region.  Workgroups take
-// tickets in the planner's order; a task waits (bounded) only on counters
-// that tasks with smaller tickets set.  A workgroup takes its next ticket
-// when it starts a task (hiding that latency behind the task); the smallest
-// unfinished ticket is then always some workgroup's current task, whose
-// dependencies are all finished, so the order is deadlock-free on any
-// number of resident workgroups.  Counters:
+// Dependency q of DAG task d: the counter w that must reach target.  Returns
+// false when d has no dependency q; *nw is the number of dependencies.
+template <typename T>
+__device__ __forceinline__ bool dag_dep(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, const DagTask& d,
+                        int q, int32_t** w, int* target, int* nw) {
+    const int J = d.front;
+    const DagFront g = D.df[J];
+    int32_t* cnt = D.cnt + g.toff;
+    switch (d.kind) {
+        case DT_EA:
+        case DT_SMALL: {  // children complete, earlier users of the U slot consumed
+            const Front f = P.fronts[J];
+            const int nchd = f.chd1 - f.chd0, u0 = D.uwait_ptr[J];
+            *nw = nchd + D.uwait_ptr[J + 1] - u0;
+            if (q >= *nw) return false;
+            if (q < nchd) {
+                const int K = P.chd[f.chd0 + q];
+                *w = F.flags + K;
+                *target = D.df[K].n_done;
+            } else {
+                const int c = D.uwait[u0 + q - nchd];
+                *w = D.eacnt + c;
+                *target = D.df[c].n_ea;
+            }
+            return true;
+        }
+        case DT_POTRF:
+            *nw = 1;
+            if (q >= 1) return false;
+            *w = d.a == 0 ? D.eacnt + J : cnt + d.a * g.R + d.a;
+            *target = d.a == 0 ? g.n_ea : d.a;
+            return true;
+        case DT_TRSM:
+            *nw = d.c > 0 ? 2 : 1;
+            if (q >= *nw) return false;
+            *w = cnt + d.c * g.R + (q == 0 ? d.c : d.b);
+            *target = q == 0 ? d.c + 1 : d.c;
+            return true;
+        case DT_UPD: {  // columns [K0, K1): rows I, Jc final through K1 - 1
+            const int K0 = d.d & 0xffff, K1 = d.d >> 16;
+            *nw = K0 > 0 ? 3 : 2;
+            if (q >= *nw) return false;
+            *w = q == 2 ? cnt + d.c * g.R + d.b : cnt + (K1 - 1) * g.R + (q == 0 ? d.b : d.c);
+            *target = q == 2 ? K0 : K1;
+            return true;
+        }
+        default: {  // DT_SYRK: the tile's row blocks of the panel are final
+            *nw = 2 + (d.d & 1) + ((d.d >> 1) & 1);
+            if (q >= *nw) return false;
+            const int blk = q == 0 ? d.b : q == 1 ? d.c : (q == 2 && (d.d & 1)) ? d.b + 1 : d.c + 1;
+            *w = cnt + (g.p - 1) * g.R + blk;
+            *target = g.p;
+            return true;
+        }
+    }
+}
+
+constexpr int WIN = 4;    // tickets a workgroup holds at once
+constexpr int WDEP = 16;  // lanes of wave 0 per held ticket (one dependency each)
+
+// Persistent tile-DAG factorization of the big region.  Tickets are handed
+// out in the planner's order (a list schedule, critical path first); a task
+// depends only on tasks with smaller tickets.  Each workgroup holds WIN
+// tickets and runs whichever of them is ready (the oldest first), so it
+// blocks only when none of its tickets is ready.  The smallest unfinished
+// ticket always belongs to some workgroup and is ready (its dependencies all
+// have smaller tickets), so the launch is deadlock-free on any number of
+// resident workgroups.  Counters:
 //   F.flags[J]   finished update-matrix tiles of front J (1 for a small front)
 //   D.eacnt[J]   finished extend-add tasks of J (1 for a small front)
-//   D.cnt[toff_J + K*R_J + I]  panel updates applied to tile (I, K); K+1
+//   D.cnt[toff_J + K*R_J + I]  panel columns applied to tile (I, K); K+1
 //                once the tile is final
-// Synchronization per task: the lanes of wave 0 poll one dependency each
-// with relaxed loads, then thread 0 issues one agent-scope acquire; at the
-// end every wave drains its stores, then thread 0 publishes with an
-// agent-scope release.
+// Synchronization per task: lanes of wave 0 poll one dependency each with
+// relaxed loads (16 lanes per held ticket), then thread 0 issues one
+// agent-scope acquire; at the end every wave drains its stores, then thread
+// 0 publishes with an agent-scope release.
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDag D) {
-    constexpr int S1 = NB * LDS + 2 * NB, S2 = 2 * KC * LDT, S3 = 2 * KW * LDW;
+    constexpr int S1 = PTILE, S2 = 2 * KC * LDT, S3 = 2 * KW * LDW;
     constexpr int SMEM = S1 > S2 ? (S1 > S3 ? S1 : S3) : (S2 > S3 ? S2 : S3);
     __shared__ T smem[SMEM];
     __shared__ T s_piv, s_inv;
-    __shared__ int s_fail, s_next;
-    if (threadIdx.x == 0) s_next = (int)atomicAdd(F.ticket, 1u);
+    __shared__ int s_fail, s_sel;
+    __shared__ int s_win[WIN];
+    if (threadIdx.x == 0)
+        for (int i = 0; i < WIN; ++i) {
+            const int t = (int)atomicAdd(F.ticket, 1u);
+            s_win[i] = t < D.ntask ? t : -1;
+        }
+    __syncthreads();
+    // wave 0, lane (slot, q): dependency q of the ticket held in slot
+    int my_t = -3, my_tgt = 0, my_extra = 0;
+    bool my_has = false;
+    int32_t* my_w = nullptr;
+    unsigned long long t_mark = 0;
+    if (D.prof && threadIdx.x == 0) t_mark = wall_clock64();
     for (;;) {
         // an opaque copy of the thread index: values derived from it are
         // recomputed per task instead of being held (and spilled) across the
         // whole persistent loop
         int tid = (int)threadIdx.x;
         asm volatile("" : "+v"(tid));
+        if (tid < 64) {
+            const int slot = tid >> 4, q = tid & (WDEP - 1);
+            int sel = -1;
+            for (int spins = 0;; ++spins) {
+                const int t = s_win[slot];
+                if (t != my_t) {  // a new ticket in this slot
+                    my_t = t;
+                    my_has = false;
+                    my_extra = 0;
+                    if (t >= 0) {
+                        const DagTask dd = D.task[t];
+                        int nw = 0;
+                        my_has = dag_dep<T>(P, F, D, dd, q, &my_w, &my_tgt, &nw);
+                        if (q == 0 && nw > WDEP) my_extra = nw;
+                    }
+                }
+                bool ok = !my_has || ld_rel(my_w) >= my_tgt;
+                if (my_extra) {  // more dependencies than lanes (many children): checked here
+                    const DagTask dd = D.task[my_t];
+                    for (int k = WDEP; k < my_extra && ok; ++k) {
+                        int32_t* w2;
+                        int tg2, nw2;
+                        if (dag_dep<T>(P, F, D, dd, k, &w2, &tg2, &nw2)) ok = ld_rel(w2) >= tg2;
+                    }
+                }
+                const unsigned long long okm = __ballot(ok);
+                if (tid == 0) {
+                    int best = -1, bt = 0x7fffffff;
+                    bool any = false;
+                    for (int i = 0; i < WIN; ++i) {
+                        const int ti = s_win[i];
+                        if (ti < 0) continue;
+                        any = true;
+                        if (((okm >> (WDEP * i)) & 0xffffull) == 0xffffull && ti < bt) {
+                            bt = ti;
+                            best = i;
+                        }
+                    }
+                    sel = any ? best : -2;
+                    if (sel == -1) {
+                        if (spins > (1 << 24)) {
+                            atomicExch(F.hang, 1);
+                            sel = -2;
+                        } else if ((spins & 255) == 255 && ld_rel(F.hang)) {
+                            sel = -2;
+                        }
+                    }
+                }
+                sel = __shfl(sel, 0);
+                if (sel != -1) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (tid == 0) {
+                s_sel = sel;
+                s_fail = 0;
+                const unsigned long long a0 = D.prof ? wall_clock64() : 0;
+                if (sel >= 0) acquire_agent();
+                if (D.prof) atomicAdd(D.prof + 3 * DT_KINDS + 1, wall_clock64() - a0);
+            }
+        }
         __syncthreads();
-        const int t = s_next;
-        __syncthreads();
-        if (t >= D.ntask) break;
+        const int sel = s_sel;
+        if (sel < 0) break;
+        const int t = s_win[sel];
         const DagTask d = D.task[t];
         const int J = d.front;
         const DagFront g = D.df[J];
         int32_t* cnt = D.cnt + g.toff;
-        unsigned long long tw0 = 0, tw1 = 0;
-        if (tid < 64) {
-            if (tid == 0) {
-                s_fail = 0;
-                s_next = (int)atomicAdd(F.ticket, 1u);
-                if (D.prof) tw0 = wall_clock64();
-            }
-            int nw, nchd = 0, c0 = 0, u0 = 0;
-            switch (d.kind) {
-                case DT_EA:
-                case DT_SMALL: {
-                    const Front f = P.fronts[J];
-                    c0 = f.chd0;
-                    nchd = f.chd1 - f.chd0;
-                    u0 = D.uwait_ptr[J];
-                    nw = nchd + D.uwait_ptr[J + 1] - u0;
-                    break;
-                }
-                case DT_POTRF: nw = 1; break;
-                case DT_TRSM: nw = d.c > 0 ? 2 : 1; break;
-                case DT_UPD: nw = d.d > 0 ? 3 : 2; break;
-                default: nw = 2 + (d.d & 1) + ((d.d >> 1) & 1); break;
-            }
-            for (int q = tid; q < nw; q += 64) {
-                int32_t* w;
-                int32_t target;
-                switch (d.kind) {
-                    case DT_EA:
-                    case DT_SMALL:
-                        if (q < nchd) {
-                            const int K = P.chd[c0 + q];
-                            w = F.flags + K;
-                            target = D.df[K].n_done;
-                        } else {
-                            const int c = D.uwait[u0 + q - nchd];
-                            w = D.eacnt + c;
-                            target = D.df[c].n_ea;
-                        }
-                        break;
-                    case DT_POTRF:
-                        w = d.a == 0 ? D.eacnt + J : cnt + d.a * g.R + d.a;
-                        target = d.a == 0 ? g.n_ea : d.a;
-                        break;
-                    case DT_TRSM:
-                        w = cnt + d.c * g.R + (q == 0 ? d.c : d.b);
-                        target = q == 0 ? d.c + 1 : d.c;
-                        break;
-                    case DT_UPD:
-                        w = q == 2 ? cnt + d.c * g.R + d.b : cnt + d.d * g.R + (q == 0 ? d.b : d.c);
-                        target = q == 2 ? d.d : d.d + 1;
-                        break;
-                    default: {  // DT_SYRK: the tile's row blocks of the panel are final
-                        const int blk = q == 0 ? d.b : q == 1 ? d.c : (q == 2 && (d.d & 1)) ? d.b + 1 : d.c + 1;
-                        w = cnt + (g.p - 1) * g.R + blk;
-                        target = g.p;
-                        break;
-                    }
-                }
-                wait_ge(w, target, F.hang);
-            }
-            if (tid == 0) {
-                acquire_agent();
-                if (D.prof) tw1 = wall_clock64();
-            }
-        }
-        __syncthreads();
+        unsigned long long t_begin = 0;
+        if (D.prof && tid == 0) t_begin = wall_clock64();
         switch (d.kind) {
             case DT_EA: {
                 const int wave = tid >> 6, lane = tid & 63;
@@ -660,7 +905,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                 small_front<T>(P, F, J, smem, &s_piv, &s_inv, &s_fail, tid);
                 break;
             case DT_POTRF:
-                potrf_tile<T>(F, P.fronts[J], d.a * NB, false, smem, smem + NB * LDS, tid);
+                potrf_blocked<T>(F, P.fronts[J], d.a * NB, false, smem, tid);
                 break;
             default: {
                 const GemmTask gt = P.gemm[d.a];
@@ -673,12 +918,19 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
         __syncthreads();
         if (tid == 0) {
             if (D.prof) {
-                const unsigned long long tw2 = wall_clock64();
-                atomicAdd(D.prof + 3 * d.kind, tw1 - tw0);
-                atomicAdd(D.prof + 3 * d.kind + 1, tw2 - tw1);
+                const unsigned long long now = wall_clock64();
+                if (D.trace) {
+                    D.trace[2 * t] = t_begin | ((unsigned long long)blockIdx.x << 48);
+                    D.trace[2 * t + 1] = now;
+                }
+                atomicAdd(D.prof + 3 * d.kind, t_begin - t_mark);
+                atomicAdd(D.prof + 3 * d.kind + 1, now - t_begin);
                 atomicAdd(D.prof + 3 * d.kind + 2, 1ull);
+                t_mark = now;
             }
-            constexpr int R = __ATOMIC_RELEASE, S = __HIP_MEMORY_SCOPE_AGENT;
+            const unsigned long long r0 = D.prof ? wall_clock64() : 0;
+            release_agent();
+            constexpr int R = __ATOMIC_RELAXED, S = __HIP_MEMORY_SCOPE_AGENT;
             switch (d.kind) {
                 case DT_EA: __hip_atomic_fetch_add(D.eacnt + J, 1, R, S); break;
                 case DT_SMALL:
@@ -687,8 +939,15 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                     break;
                 case DT_POTRF: __hip_atomic_store(cnt + d.a * g.R + d.a, d.a + 1, R, S); break;
                 case DT_TRSM: __hip_atomic_store(cnt + d.c * g.R + d.b, d.c + 1, R, S); break;
-                case DT_UPD: __hip_atomic_store(cnt + d.c * g.R + d.b, d.d + 1, R, S); break;
+                case DT_UPD: __hip_atomic_store(cnt + d.c * g.R + d.b, d.d >> 16, R, S); break;
                 default: __hip_atomic_fetch_add(F.flags + J, 1, R, S); break;
+            }
+            const unsigned long long r1 = D.prof ? wall_clock64() : 0;
+            const int nt = (int)atomicAdd(F.ticket, 1u);
+            s_win[sel] = nt < D.ntask ? nt : -1;
+            if (D.prof) {
+                atomicAdd(D.prof + 3 * DT_KINDS, r1 - r0);
+                atomicAdd(D.prof + 3 * DT_KINDS + 2, wall_clock64() - r1);
             }
         }
         // closes the thread-0 block before the back-edge

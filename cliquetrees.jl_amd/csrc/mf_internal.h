@@ -26,6 +26,7 @@ constexpr int SMALL_NJ = 64;    // fronts with nJ <= SMALL_NJ use the fused LDS 
 constexpr int EA_WIN = 256;     // parent rows per extend-add window (one wave)
 constexpr int EA_ROWS = 8192;   // parent rows per extend-add task (about)
 constexpr int WIDE_K = 128;     // update-matrix tiles are 128 x 128 when nn >= WIDE_K
+constexpr int UPD_GROUP = 4;    // panel steps applied by one trailing-update tile
 constexpr int SIM_WORKERS = 512;  // resident workgroups assumed when ordering the DAG tickets
 
 struct Front {
@@ -80,15 +81,16 @@ struct DiagTask {
 };
 
 // Task of the persistent tile-DAG factorization of the big region.
-enum : int32_t { DT_EA = 0, DT_SMALL = 1, DT_POTRF = 2, DT_TRSM = 3, DT_UPD = 4, DT_SYRK = 5 };
+enum : int32_t { DT_EA = 0, DT_SMALL = 1, DT_POTRF = 2, DT_TRSM = 3, DT_UPD = 4, DT_SYRK = 5,
+                 DT_KINDS = 6 };
 struct DagTask {
     int32_t kind;
     int32_t front;
     int32_t a;   // DT_EA: first EaTask; DT_POTRF: panel block K; else: GemmTask index
     int32_t b;   // DT_EA: end EaTask; DT_TRSM/UPD/SYRK: row block I
     int32_t c;   // DT_TRSM: panel block K; DT_UPD/SYRK: column block Jc
-    int32_t d;   // DT_UPD: panel block K; DT_SYRK: bit 0 / bit 1: the tile also
-                 // covers row block I+1 / column block Jc+1 (wide tile)
+    int32_t d;   // DT_UPD: panel blocks [K0, K1) as K0 | K1 << 16; DT_SYRK: bit 0 /
+                 // bit 1: the tile also covers row block I+1 / column block Jc+1
 };
 
 // Per-front constants of the DAG.

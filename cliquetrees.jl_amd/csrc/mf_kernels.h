@@ -64,6 +64,7 @@ struct DevDag {
     int32_t* cnt;      // tile counters
     int32_t* eacnt;    // finished extend-add tasks per front
     unsigned long long* prof;  // NULL, or per task kind: wait, body (wall-clock ticks), count
+    unsigned long long* trace; // NULL, or per ticket: start | workgroup << 48, end (wall clock)
     int32_t ntask;
 };
 

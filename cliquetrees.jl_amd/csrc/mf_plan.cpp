@@ -460,7 +460,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     // ---- tile-DAG factorization of the big region -----------------------------
     // Row blocks of a big front: panel blocks [64K, 64K+kb_K) for K < p, then
     // update rows [nn + 64m, ...).  Tile (I, K) of the panel carries a counter
-    // = number of panel updates applied; K+1 once it is final (factored or
+    // = number of panel columns applied; K+1 once it is final (factored or
     // solved).  Order per level: small fronts, extend-add, then per panel step
     // K: POTRF(K), TRSM(., K), the look-ahead column updates (., K+1, K), the
     // other updates; finally the update-matrix tiles (SYRK, K = nn).
@@ -519,20 +519,26 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                                  rlen(F, D.p, I), kb, kb, F.c0 + k0, 0);
                     }
                 }
+                // trailing updates grouped by UPD_GROUP panel steps: tile
+                // (I, Jc) receives the columns [K0, K1) in one product once
+                // block column K1 - 1 is final; the group ending right before
+                // Jc (the look-ahead column) is issued first
+                const int32_t K1 = K + 1, K0 = (K / UPD_GROUP) * UPD_GROUP;
                 for (int pass = 0; pass < 2; ++pass)
                     for (int32_t J : L.big_list) {
                         const Front& F = P.fronts[J];
                         const DagFront& D = P.dagf[J];
                         if (K >= D.p) continue;
-                        int32_t nj = F.nn + F.na, k0 = K * NB, kb = rlen(F, D.p, K);
+                        const int32_t nj = F.nn + F.na, k0 = K0 * NB, kk = (K1 - K0) * NB;
                         int32_t jc0 = pass == 0 ? K + 1 : K + 2, jc1 = pass == 0 ? std::min(K + 2, D.p) : D.p;
+                        if (pass == 1 && K1 % UPD_GROUP != 0) continue;  // group not complete yet
                         for (int32_t Jc = jc0; Jc < jc1; ++Jc)
                             for (int32_t I = Jc; I < D.R; ++I) {
-                                P.dag.push_back(DagTask{DT_UPD, J, (int32_t)P.gemm.size(), I, Jc, K});
+                                P.dag.push_back(DagTask{DT_UPD, J, (int32_t)P.gemm.size(), I, Jc, K0 | (K1 << 16)});
                                 add_gemm(G_SUB, ARR_PANEL, F.poff + row0(F, D.p, I) + (int64_t)k0 * nj, nj,
                                          ARR_PANEL, F.poff + row0(F, D.p, Jc) + (int64_t)k0 * nj, nj,
                                          ARR_PANEL, F.poff + row0(F, D.p, I) + (int64_t)(Jc * NB) * nj, nj,
-                                         rlen(F, D.p, I), rlen(F, D.p, Jc), kb, F.c0 + k0, I == Jc ? 1 : 0);
+                                         rlen(F, D.p, I), rlen(F, D.p, Jc), kk, F.c0 + k0, I == Jc ? 1 : 0);
                             }
                     }
             }
@@ -779,12 +785,14 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     if (d.c > 0 && !edge(lastw[tile(J, d.b, d.c)], t)) return "dataflow: trsm without its update";
                     fin[tile(J, d.b, d.c)] = t;
                     break;
-                case DT_UPD:
-                    if (!edge(fin[tile(J, d.b, d.d)], t) || !edge(fin[tile(J, d.c, d.d)], t))
+                case DT_UPD: {
+                    const int32_t K0 = d.d & 0xffff, K1 = d.d >> 16;
+                    if (!edge(fin[tile(J, d.b, K1 - 1)], t) || !edge(fin[tile(J, d.c, K1 - 1)], t))
                         return "dataflow: update without its operands";
-                    if (d.d > 0 && !edge(lastw[tile(J, d.b, d.c)], t)) return "dataflow: update order";
+                    if (K0 > 0 && !edge(lastw[tile(J, d.b, d.c)], t)) return "dataflow: update order";
                     lastw[tile(J, d.b, d.c)] = t;
                     break;
+                }
                 case DT_SYRK: {
                     const int32_t p = P.dagf[J].p;
                     const int32_t rows[4] = {d.b, d.c, (d.d & 1) ? d.b + 1 : d.b, (d.d & 2) ? d.c + 1 : d.c};
@@ -839,7 +847,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         for (int32_t t = 0; t < NT; ++t) {
             const DagTask& d = P.dag[t];
             const bool h = d.kind == DT_EA || d.kind == DT_SMALL || d.kind == DT_POTRF ||
-                           d.kind == DT_TRSM || (d.kind == DT_UPD && d.c == d.d + 1);
+                           d.kind == DT_TRSM || (d.kind == DT_UPD && d.c == (d.d >> 16));
             P.hi[t] = h ? 1 : 0;
             (h ? P.n_hi : P.n_lo)++;
         }
@@ -905,19 +913,22 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         std::vector<double> cost(NT), bl(NT, 0.0), bgate(NJ, 0.0);
         for (int32_t t = 0; t < NT; ++t) {
             const DagTask& d = P.dag[t];
-            double c = 10.0;
+            // microseconds per task on one workgroup (measured on MI355X with
+            // two workgroups per CU): dispatch + extend-add rows / panel tile /
+            // MFMA work at the rate each tile shape reaches
+            double c = 5.0;
             switch (d.kind) {
                 case DT_EA: {
                     double rows = 0;
                     for (int32_t e = d.a; e < d.b; ++e) rows += P.ea[e].row1 - P.ea[e].row0;
-                    c = 3.0 + 24.0 * rows / 40000.0;
+                    c += rows * 0.0066;
                     break;
                 }
-                case DT_SMALL: c = 25.0; break;
-                case DT_POTRF: c = 40.0; break;
+                case DT_SMALL: c += 20.0; break;
+                case DT_POTRF: c += 48.0; break;
                 default: {
                     const GemmTask& g = P.gemm[d.a];
-                    c = 8.0 + 2.0 * g.m * g.n * (double)g.k / (g.wide ? 1.2e5 : 1.0e5);
+                    c += 2.0 * g.m * g.n * (double)g.k / (g.wide ? 1.3e5 : 7.5e4);
                     break;
                 }
             }
@@ -1003,7 +1014,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 case DT_SMALL: last_done[d.front] = t; break;
                 case DT_POTRF: setter[cnt_id(d.front, d.a, d.a)] = t; first_panel[d.front] = std::min(first_panel[d.front], t); break;
                 case DT_TRSM: setter[cnt_id(d.front, d.b, d.c)] = t; break;
-                case DT_UPD: upd_at[cnt_id(d.front, d.b, d.c) * 4096 + d.d] = t; break;
+                case DT_UPD: upd_at[cnt_id(d.front, d.b, d.c) * 4096 + (d.d >> 16) - 1] = t; break;
                 case DT_SYRK: last_done[d.front] = t; break;
                 default: return "dag task kind";
             }
@@ -1042,10 +1053,13 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 }
             }
             if (d.kind == DT_UPD) {
-                if (need(d.b, d.d) < 0 || need(d.b, d.d) >= t || need(d.c, d.d) < 0 || need(d.c, d.d) >= t)
+                const int32_t K0 = d.d & 0xffff, K1 = d.d >> 16;
+                if (K1 <= K0 || K1 > d.c || K1 - K0 > UPD_GROUP) return "dag: update group";
+                if (need(d.b, K1 - 1) < 0 || need(d.b, K1 - 1) >= t || need(d.c, K1 - 1) < 0 ||
+                    need(d.c, K1 - 1) >= t)
                     return "dag: update before its operands";
-                if (d.d > 0) {
-                    auto it = upd_at.find(cnt_id(d.front, d.b, d.c) * 4096 + d.d - 1);
+                if (K0 > 0) {
+                    auto it = upd_at.find(cnt_id(d.front, d.b, d.c) * 4096 + K0 - 1);
                     if (it == upd_at.end() || it->second >= t) return "dag: update order";
                 }
             }

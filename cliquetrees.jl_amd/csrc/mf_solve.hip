@@ -18,8 +18,9 @@
 //   PT_ROWS   64 update rows of a big front: M2 rows = gather - L21 X.
 // Completion: per-front task counters (children complete / parent complete)
 // and per-front block progress counters.  Thread 0 polls with relaxed loads
-// and issues one agent-scope acquire; after the closing barrier (which drains
-// every wave's stores to L2) thread 0's release atomics publish the task.
+// and issues one agent-scope acquire; after the closing barrier (every wave
+// first drains its stores to L2) thread 0 writes L2 back, waits for the
+// write-back, and only then publishes the task with relaxed atomics.
 // All waits are bounded.  Right-hand sides are processed 16 at a time.
 #include <hip/hip_runtime.h>
 
@@ -39,7 +40,20 @@ constexpr int LDP = NB + 1;
 // L2.  Before the closing barrier of a task every wave drains, so that thread
 // 0's agent-scope release (an L2 write-back) covers the whole workgroup's data.
 __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0); }
-__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// Agent-scope acquire (L1/L2 invalidate) and release (L2 write-back), each
+// followed by an explicit wait: the compiler omits the wait after the
+// write-back when its own counters are already zero (after drain_stores), and
+// a flag stored before the write-back completes can be seen by a workgroup on
+// another XCD ahead of the data.  The wait after the invalidate makes it
+// complete before the barrier releases the workgroup's other waves.
+__device__ __forceinline__ void acquire_agent() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
+}
+__device__ __forceinline__ void release_agent() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
+}
 
 // Bounded wait until *p >= target; gives up (and records it) after ~1 s.
 __device__ __forceinline__ void wait_geq(int32_t* p, int32_t target, int32_t* hang) {
@@ -391,9 +405,10 @@ __global__ __launch_bounds__(256) void k_fsolve(DevPlan P, DevFactor<T> F, DevSo
         drain_stores();
         __syncthreads();
         if (tid == 0) {
+            release_agent();
             if (tk.kind == PT_DIAG)
-                __hip_atomic_store(Y_.prog + J, tk.a / NB + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(Y_.done + J, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(Y_.prog + J, tk.a / NB + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(Y_.done + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         // closes the thread-0 block before the back-edge: every thread of the
         // workgroup reaches the next ticket together
@@ -428,11 +443,12 @@ __global__ __launch_bounds__(256) void k_bsolve(DevPlan P, DevFactor<T> F, DevSo
         drain_stores();
         __syncthreads();
         if (tid == 0) {
+            release_agent();
             if (tk.kind == PT_DIAG) {
                 const int p = (f.nn + NB - 1) / NB;
-                __hip_atomic_store(Y_.prog + J, p - tk.a / NB, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(Y_.prog + J, p - tk.a / NB, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            __hip_atomic_fetch_add(Y_.done + J, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(Y_.done + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         // closes the thread-0 block before the back-edge: every thread of the
         // workgroup reaches the next ticket together

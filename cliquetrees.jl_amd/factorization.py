@@ -33,7 +33,8 @@ MF_REG_NONE, MF_REG_DYNAMIC = 0, 1
 
 STAT_NAMES = ["n", "nfr", "internal_fronts", "small_region", "big_fronts", "big_levels",
               "small_levels", "max_nj", "panel_elems", "u_elems", "w_elems", "gemm_tasks",
-              "ea_tasks", "diag_tasks", "ref_flops", "engine_flops", "gemm_flops", "ndz", "nlz"]
+              "ea_tasks", "diag_tasks", "ref_flops", "engine_flops", "gemm_flops", "ndz", "nlz",
+              "dag_tasks"]
 KC_NAMES = ["small", "ea", "diag", "gemm", "copyout", "solve", "assemble", "dag"]
 
 

@@ -173,11 +173,18 @@ int mf_timing_read(mf_factor* f, double* times_ms, int64_t* counts, double* work
 /* Task-kind profile of the big-front tile-DAG kernel (no counterpart in the
  * reference; a diagnostic).  Kinds: extend-add, small front, diagonal tile,
  * panel-solve tile, trailing-update tile, update-matrix tile.  out[3k..3k+2]
- * = microseconds spent waiting on dependencies, microseconds of work (summed
- * over workgroups), task count. */
+ * = microseconds from the previous task's end to this task's start (ticket
+ * selection and dependency waits), microseconds of work (summed over
+ * workgroups), task count; out[18..20] = microseconds in the publishing
+ * (release) step, in the acquire step and in the ticket fetch. */
 enum { MF_DT_KINDS = 6 };
 int mf_dag_profile(mf_factor* f, int enable);
 int mf_dag_profile_read(mf_factor* f, double* out, int reset);
+/* With the profile on, the last launch's per-ticket times: out[2t] = start
+ * (wall-clock ticks) | workgroup << 48, out[2t+1] = end; n >= 2 x tasks.
+ * mf_plan_dag: the tickets' tasks, 6 int32 each (kind, front, a, b, c, d). */
+int mf_dag_trace_read(mf_factor* f, unsigned long long* out, int64_t n);
+int mf_plan_dag(const mf_plan* plan, int32_t* tasks, int64_t n);
 
 #ifdef __cplusplus
 }

@@ -371,3 +371,63 @@ def test_edge_cases(ct, oracle):
         assert rel(F.Dval, D0) <= RTOL64 and rel(F.Lval, L0) <= RTOL64
         b = np.arange(1.0, A.shape[0] + 1)
         assert rel(ct.solve(F, b), oracle.solve(S, P, D0, L0, d0, b)) <= 1e-12
+
+
+def test_repeat_factorization_bitwise(ct):
+    """The schedule is dynamic but every tile's arithmetic is fixed (update
+    groups in order, extend-add sums in child order), so repeated
+    factorizations of the 64^3 Laplacian must agree bit for bit.  The loop
+    is the device-resident step (value refresh, asynchronous factorization,
+    16-RHS device solve, all on one stream without host synchronization);
+    every step's factor and residual are checked."""
+    import ctypes
+    import torch
+    from cliquetrees_amd import _native
+    A = ct.problems.laplacian3d_7pt(64, 64, 64, seed=2)
+    n = A.shape[0]
+    P, S = ct.symbolic(A, alg=ct.grid_nd(64, 64, 64))
+    F = ct.ChordalCholesky(A, P, S)
+    lib = _native.hip()
+    h = F._f.handle
+    stream = torch.cuda.Stream()
+    assert lib.mf_factor_set_stream(h, ctypes.c_void_p(stream.cuda_stream)) == 0
+    flat = ct.flatindices(S, A)
+    assert lib.mf_factor_set_pattern(h, len(flat), flat.ctypes.data_as(_native.i64p)) == 0
+    perm = np.ascontiguousarray(P.perm, dtype=np.int64)
+    B0 = np.random.default_rng(4).standard_normal((n, 16))
+    with torch.cuda.stream(stream):
+        nz = torch.tensor(A.data, dtype=torch.float64, device="cuda")
+        Bd0 = torch.tensor(B0.T.copy(), dtype=torch.float64, device="cuda")
+        reps = 8
+        Ds = [torch.empty(S.ndz, dtype=torch.float64, device="cuda") for _ in range(reps)]
+        Ls = [torch.empty(S.nlz, dtype=torch.float64, device="cuda") for _ in range(reps)]
+        Xs = [torch.empty_like(Bd0) for _ in range(reps)]
+        for rep in range(reps):
+            assert lib.mf_factor_refresh(h, ctypes.c_void_p(nz.data_ptr()), 1) == 0
+            assert lib.mf_factorize_async(h, ct.MF_CHOLESKY, None, 0, 0.0, 0.0) == 0
+            Xs[rep].copy_(Bd0)
+            assert lib.mf_solve(h, perm.ctypes.data_as(_native.i64p), ctypes.c_void_p(Xs[rep].data_ptr()),
+                                16, n, 1) == 0
+            assert lib.mf_factor_get(h, ctypes.c_void_p(Ds[rep].data_ptr()),
+                                     ctypes.c_void_p(Ls[rep].data_ptr()), None, 1) == 0
+    torch.cuda.synchronize()
+    info = ctypes.c_int64(-1)
+    assert lib.mf_factor_info(h, ctypes.byref(info)) == 0 and info.value == 0
+    for rep in range(reps):
+        X = Xs[rep].T.cpu().numpy()
+        res = np.linalg.norm(A @ X - B0) / np.linalg.norm(B0)
+        same = torch.equal(Ds[rep], Ds[0]) and torch.equal(Ls[rep], Ls[0])
+        assert same and res <= 1e-12, (rep, same, res)
+        assert torch.equal(Xs[rep], Xs[0]), rep
+    # back-to-back solves with the last factor: the persistent solve kernels
+    # publish diagonal-block progress across workgroups, and every solve must
+    # give the same bits
+    with torch.cuda.stream(stream):
+        Ys = [torch.empty_like(Bd0) for _ in range(32)]
+        for Y in Ys:
+            Y.copy_(Bd0)
+            assert lib.mf_solve(h, perm.ctypes.data_as(_native.i64p), ctypes.c_void_p(Y.data_ptr()),
+                                16, n, 1) == 0
+    torch.cuda.synchronize()
+    bad = [i for i, Y in enumerate(Ys) if not torch.equal(Y, Xs[0])]
+    assert not bad, bad
