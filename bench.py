@@ -1,6 +1,6 @@
 """Benchmark of the multifrontal numeric engine on one MI355X.
 
-Default workload (cfg2): plain FP64 sparse Cholesky of the 3-D 7-point
+Default workload (lap7_64): plain FP64 sparse Cholesky of the 3-D 7-point
 weighted Laplacian on a 64^3 grid (n = 262,144; edge weights i.i.d.
 U[0.5, 1.5], seed 2, plus 1e-3 I), geometric nested-dissection order, 16
 right-hand sides N(0,1) (seed 2).  One step = assembly of A's values into the
@@ -20,18 +20,19 @@ numeric factorization + a 16-RHS solve, all on the GPU.
                  factorization, timed on the host cores.
 
 Other workloads (--workload):
-  cfg2mp  the cfg2 matrix with FP32 fronts + FP64 iterative refinement to
-          1e-10, against the plain FP64 step on the same matrix
-  cfg3    3-D 27-point stencil 48^3 (seed 3), FP32 fronts + FP64 refinement
-          to 1e-10, error vs the FP64 solution, plain FP64 step alongside
-  cfg3fp  the cfg3 matrix in plain FP64
-  cfg4    4,096 independent grid Laplacians, n ~ U{50..2000} (seed 4), half
-          2-D / half 3-D, factored and solved as one forest (FP64 and
-          FP32 + refinement); value = systems per second
-  cfg5    serving: Poisson stream of KKT LDLt requests, patterns reused over
-          20 value updates, continuous batching; p50/p99 latency, solves/s
-          against one-at-a-time serving
-  copy    a 1 GiB device copy (calibration of the HBM traffic counters)
+  lap7_64_mixed  the lap7_64 matrix with FP32 fronts + FP64 iterative
+                 refinement to 1e-10, against the plain FP64 step on it
+  st27_48_mixed  3-D 27-point stencil 48^3 (seed 3), FP32 fronts + FP64
+                 refinement to 1e-10, error vs the FP64 solution, plain FP64
+                 step alongside
+  st27_48        the same 27-point matrix in plain FP64
+  batch4096      4,096 independent grid Laplacians, n ~ U{50..2000} (seed 4),
+                 half 2-D / half 3-D, factored and solved as one forest (FP64
+                 and FP32 + refinement); value = systems per second
+  kkt_serving    Poisson stream of KKT LDLt requests, patterns reused over 20
+                 value updates, continuous batching; p50/p99 latency, solves/s
+                 against one-at-a-time serving
+  copy           a 1 GiB device copy (calibration of the HBM traffic counters)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -54,6 +55,25 @@ HBM_PEAK = 8000.0  # GB/s
 METRIC = "factor TFLOP/s + solves/s at <=1e-10 rel residual vs plain FP64; p50/p99 ms"
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profile(kernel, tag):
+    """HBM bytes per launch of ``kernel`` from the committed counter summary
+    ``profiles/<kernel>_<tag>_hbm_traffic.md`` (FETCH_SIZE and WRITE_SIZE
+    passes of rocprofv3, each scaled by what the same counter reads for a
+    1 GiB device copy); None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{tag}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def load_package():
     if "cliquetrees_amd" in sys.modules:
         return sys.modules["cliquetrees_amd"]
@@ -67,18 +87,18 @@ def load_package():
 
 
 def workload(ct, name):
-    if name in ("cfg2", "cfg2mp"):
+    if name in ("lap7_64", "lap7_64_mixed"):
         A = ct.problems.laplacian3d_7pt(64, 64, 64, seed=2)
-        mp = name == "cfg2mp"
-        return dict(A=A, order=ct.grid_nd(64, 64, 64), nrhs=16, seed=2,
+        mp = name == "lap7_64_mixed"
+        return dict(A=A, order=ct.grid_nd(64, 64, 64), nrhs=16, seed=2, tag=name,
                     precision=ct.MF_FP32 if mp else ct.MF_FP64, refine=mp,
                     desc="3D 7-point Laplacian 64^3, w~U[0.5,1.5] + 1e-3 I, geometric ND, 16 RHS, "
                          + ("FP32 fronts + FP64 iterative refinement to 1e-10" if mp
                             else "plain FP64 Cholesky"))
-    if name in ("cfg3", "cfg3fp"):
+    if name in ("st27_48", "st27_48_mixed"):
         A = ct.problems.stencil3d_27pt(48, 48, 48, seed=3)
-        mp = name == "cfg3"
-        return dict(A=A, order=ct.grid_nd(48, 48, 48), nrhs=16, seed=3,
+        mp = name == "st27_48_mixed"
+        return dict(A=A, order=ct.grid_nd(48, 48, 48), nrhs=16, seed=3, tag=name,
                     precision=ct.MF_FP32 if mp else ct.MF_FP64, refine=mp,
                     desc="3D 27-point stencil 48^3, w~U[0.5,1.5], diagonally dominant, geometric ND, "
                          "16 RHS, " + ("FP32 fronts + FP64 iterative refinement to 1e-10" if mp
@@ -219,6 +239,7 @@ def run_single(args, ct, torch, name):
     t0 = time.perf_counter()
     P, S = ct.symbolic(A, alg=W["order"])
     t_sym = time.perf_counter() - t0
+    log(f"{name}: n={n}, symbolic {t_sym:.2f} s")
     dev = torch.device("cuda:0")
     # a dedicated (non-null, default-priority) stream shared by torch and the engine
     stream = torch.cuda.Stream(dev)
@@ -231,9 +252,11 @@ def run_single(args, ct, torch, name):
     if W["refine"]:
         E.register_matrix()
 
+    log("device plan and factor object ready")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    log(f"{args.warmup} warmup steps done")
     if E.info() != 0:
         raise SystemExit(f"factorization failed, info={E.info()}")
     E.iters.clear()
@@ -243,6 +266,7 @@ def run_single(args, ct, torch, name):
         E.c(lib.mf_dag_profile(h, 1))
     read_timing(ct, lib, _native, h)
     ms_per_step, step_ms = time_steps(torch, stream, step, args.steps, 0)
+    log(f"{args.steps} timed steps: {ms_per_step:.3f} ms/step")
     times, cnts, work = read_timing(ct, lib, _native, h)
     E.c(lib.mf_timing(h, 0))
     if args.dag_profile:
@@ -304,7 +328,8 @@ def run_single(args, ct, torch, name):
         "info": info,
         "kernel_ms_per_step": {k: round(times[i] / args.steps, 4) for i, k in enumerate(ct.KC_NAMES)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "frac": round(achieved / peak, 4),
+                     "traffic": traffic_from_profile("k_" + ct.KC_NAMES[g], W["tag"]),
                      "kernel": "k_" + ct.KC_NAMES[g], "launches": int(cnts[g]),
                      "flops_per_launch": float(dom_flops_per_launch),
                      "avg_launch_ms": round(float(dom_ms), 5)},
@@ -313,6 +338,7 @@ def run_single(args, ct, torch, name):
         out["refinement"] = {"iters_mean": float(np.mean(E.iters)), "iters_max": int(np.max(E.iters)),
                              "relres_max": float(np.max(E.relres))}
         # the plain FP64 step on the same matrix, and the error against its solution
+        log("plain FP64 step on the same matrix")
         fac64 = Engine(ct, torch, A, P, S, ct.MF_FP64, W["nrhs"], B0, stream)
         ms64, _ = time_steps(torch, stream, fac64.step_plain, max(3, args.steps // 2), 1)
         X64 = fac64.solution()
@@ -322,12 +348,14 @@ def run_single(args, ct, torch, name):
         out["error_vs_fp64"] = float(np.linalg.norm(X - X64) / np.linalg.norm(X64))
         out["speedup_vs_plain_fp64"] = round(ms64 / ms_per_step, 3)
     if not args.no_cpu and not fp32:
+        log("CPU baseline (oracle + host BLAS on a subtree sample)")
         out["cpu_baseline"] = cpu_baseline(ct, A, P, S)
+        log("CPU baseline done")
     return out
 
 
 def run_batched(args, ct, torch):
-    """cfg4: 4,096 uneven systems as one forest; FP64 and FP32 + refinement."""
+    """4,096 uneven systems as one forest; FP64 and FP32 + refinement."""
     specs = ct.problems.batch_specs(4096, 4)
     mats = [ct.problems.grid_laplacian_random(d, 4000 + k) for k, d in enumerate(specs)]
     orders = [ct.grid_nd(*(d if len(d) == 3 else (d[0], d[1], 1))) for d in specs]
@@ -336,8 +364,10 @@ def run_batched(args, ct, torch):
     B = np.concatenate(bs)
     res = {}
     t0 = time.perf_counter()
+    log("batch4096: 4096 systems generated")
     Bf64 = ct.BatchedFactorization(mats, orders, precision=ct.MF_FP64)
     t_setup = time.perf_counter() - t0
+    log(f"batch4096: forest analysed and FP64 factor object ready ({t_setup:.1f} s)")
     Bf32 = ct.BatchedFactorization(mats, orders, precision=ct.MF_FP32)
     for label, Bf in [("fp64", Bf64), ("fp32_ir", Bf32)]:
         def step():
@@ -353,6 +383,7 @@ def run_batched(args, ct, torch):
             torch.cuda.synchronize()
             t.append(time.perf_counter() - t1)
         t = np.array(t)
+        log(f"batch4096 {label}: median {np.median(t) * 1e3:.2f} ms per batch")
         worst = max(np.linalg.norm(m @ X[o:o + m.shape[0]] - b) / np.linalg.norm(b)
                     for m, b, o in zip(mats, bs, Bf.offsets[:-1]))
         res[label] = {"ms_per_batch": round(float(np.median(t) * 1e3), 3),
@@ -375,7 +406,7 @@ def run_batched(args, ct, torch):
 
 
 def run_serving(args, ct, torch):
-    """cfg5: Poisson stream of KKT LDLt requests with pattern reuse."""
+    """Poisson stream of KKT LDLt requests with pattern reuse."""
     srv = ct.serving
     rng = np.random.default_rng(5)
     npat, updates = 6, 20
@@ -400,14 +431,17 @@ def run_serving(args, ct, torch):
     loop = srv.ServingLoop(diag=ct.MF_LDLT, max_batch=8, streams=4, hbm_budget=96 << 30)
     for K0, _ in pats:
         loop.prepare(K0, ks=(1, 2, 4, 8))
+    log(f"kkt_serving: {npat} patterns analysed ({loop.analysis_s:.1f} s)")
     # warm every variant once
     loop.run(fresh(reqs[:len(pats) * 2], np.zeros(len(pats) * 2)), realtime=False)
     # one-at-a-time capacity: all requests visible, one stream, no batching
     one = loop.run(fresh(reqs, np.zeros(len(reqs))), realtime=False, batching=False)
     s_one = srv.latency_stats(one)
+    log(f"kkt_serving: one-at-a-time pass {s_one['makespan_s']:.2f} s")
     rate = 1.5 * len(reqs) / s_one["makespan_s"]      # offered load above one-at-a-time capacity
     t_arr = srv.poisson_arrivals(len(reqs), rate, seed=5)
     mix = loop.run(fresh(reqs, t_arr), realtime=True, batching=True)
+    log("kkt_serving: batched Poisson pass done")
     seq = loop.run(fresh(reqs, t_arr), realtime=True, batching=False)
     s_mix, s_seq = srv.latency_stats(mix), srv.latency_stats(seq)
     worst = max(np.linalg.norm(r.K @ r.X - r.B) / np.linalg.norm(r.B) for r in mix)
@@ -452,7 +486,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--workload", default="lap7_64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dag-profile", action="store_true",
                     help="per task kind wait/work time of the tile-DAG kernel (stderr)")
@@ -468,9 +502,9 @@ def main():
     ct = load_package()
     if args.workload == "copy":
         out = run_copy(args, ct, torch)
-    elif args.workload == "cfg4":
+    elif args.workload == "batch4096":
         out = run_batched(args, ct, torch)
-    elif args.workload == "cfg5":
+    elif args.workload == "kkt_serving":
         out = run_serving(args, ct, torch)
     else:
         out = run_single(args, ct, torch, args.workload)
