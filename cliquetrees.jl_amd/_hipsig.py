@@ -35,6 +35,7 @@ SIGNATURES = {
     "mf_refine": ([vp, i64p, vp, i64, vp, i64, i64, ci, cd, ci, ctypes.POINTER(ci), f64p], ci),
     "mf_refine_segmented": ([vp, i64p, vp, i64, vp, i64, i64, ci, cd, ci, i64, i64p,
                              ctypes.POINTER(ci), f64p], ci),
+    "mf_refine_steps": ([vp, i64p, vp, i64, vp, i64, i64, ci, i64, i64p, vp], ci),
     "mf_dag_profile_read": ([vp, f64p, ci], ci),
     "mf_dag_trace_read": ([vp, vp, i64], ci),
     "mf_plan_dag": ([vp, vp, i64], ci),

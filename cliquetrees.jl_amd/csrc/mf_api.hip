@@ -781,6 +781,42 @@ int mf_refine_segmented(mf_factor* f, const int64_t* perm, const double* B, int6
     return MF_OK;
 }
 
+int mf_refine_steps(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb, double* X,
+                    int64_t ldx, int64_t nrhs, int steps, int64_t nseg, const int64_t* segptr,
+                    double* nrm2) {
+    if (!f || !perm || nrhs < 0 || (nrhs > 0 && (!B || !X || !nrm2)) || steps < 0 || nseg < 1 ||
+        !segptr)
+        return fail(MF_EARG, "bad argument");
+    if (f->A_n < 0) return fail(MF_ESTATE, "no matrix registered (mf_factor_set_matrix)");
+    if (!f->factored) return fail(MF_ESTATE, "factor not computed");
+    const int64_t n = f->plan->H.n;
+    if (ldb < n || ldx < n) return fail(MF_EARG, "leading dimension < n");
+    if (segptr[0] != 0 || segptr[nseg] != n) return fail(MF_EARG, "segments must cover 0..n");
+    for (int64_t g = 0; g < nseg; ++g)
+        if (segptr[g + 1] < segptr[g]) return fail(MF_EARG, "segments must be increasing");
+    if (nseg * nrhs > (int64_t)1 << 30 || nrhs > 65535) return fail(MF_EARG, "too many segments or columns");
+    if (nrhs == 0 || n == 0) return MF_OK;
+    const size_t cols = 8 * (size_t)n * nrhs;
+    if (ensure(&f->ir_ws, &f->ir_ws_bytes, 3 * cols + 16 * (size_t)nseg * nrhs + 8 * (nseg + 1) + 64))
+        return MF_ENOMEM;
+    double* Rd = (double*)f->ir_ws + 2 * (size_t)n * nrhs;
+    int64_t* segd = (int64_t*)(Rd + (size_t)n * nrhs + 2 * (size_t)nseg * nrhs);
+    hipStream_t s = f->stream;
+    HIPCHK(hipMemcpyAsync(segd, segptr, 8 * (nseg + 1), hipMemcpyHostToDevice, s));
+    launch_colnorm2(segd, (int)nseg, B, ldb, (int)nrhs, nrm2, s);
+    launch_copy_cols(n, B, ldb, X, ldx, (int)nrhs, s);
+    int rc = mf_solve(f, perm, X, nrhs, ldx, 1);
+    for (int it = 0; !rc && it < steps; ++it) {
+        launch_residual(n, f->A_colptr, f->A_rowval, f->A_val, X, ldx, B, ldb, Rd, n, (int)nrhs, s);
+        if (!(rc = mf_solve(f, perm, Rd, nrhs, n, 1))) launch_axpy(n, Rd, n, X, ldx, (int)nrhs, s);
+    }
+    if (rc) return rc;
+    launch_residual(n, f->A_colptr, f->A_rowval, f->A_val, X, ldx, B, ldb, Rd, n, (int)nrhs, s);
+    launch_colnorm2(segd, (int)nseg, Rd, n, (int)nrhs, nrm2 + (size_t)nseg * nrhs, s);
+    HIPCHK(hipGetLastError());
+    return MF_OK;
+}
+
 int mf_refine(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb, double* X,
               int64_t ldx, int64_t nrhs, int device, double tol, int maxit, int* iters,
               double* relres) {

@@ -41,6 +41,7 @@ class Request:
     arrival: float = 0.0        # seconds, relative to the start of run()
     tag: object = None
     X: np.ndarray | None = None
+    relres: float = float("nan")    # max_j ||B_j - K X_j|| / ||B_j||, from the device
     info: int = 0
     start: float = 0.0
     done: float = 0.0
@@ -73,7 +74,7 @@ class _Pattern:
             flat = flatindices(S, Kk)
             plan = device_plan(S)
             st = plan.stats()
-            v = (P, S, flat, plan, st)
+            v = (P, S, flat, plan, st, Kk)
             self.variants[k] = v
         return v
 
@@ -90,12 +91,21 @@ class _Slot:
     """A resident factor object for (pattern, k) with its own stream and buffers."""
 
     def __init__(self, pat, k, precision, torch):
-        P, S, flat, plan, st = pat.variant(k)
+        P, S, flat, plan, st, Kk = pat.variant(k)
         self.pat, self.k = pat, k
         self.f = DeviceFactor(plan, precision)
         self.perm = np.ascontiguousarray(P.perm, dtype=np.int64)
         lib = _native.hip()
         _check(lib.mf_factor_set_pattern(self.f.handle, len(flat), flat.ctypes.data_as(_native.i64p)))
+        # A itself (same nonzero order as the pattern, so that every value
+        # refresh updates it too) for the FP64 residuals of the refinement
+        colptr = np.ascontiguousarray(Kk.indptr, dtype=np.int64) + 1
+        rowval = np.ascontiguousarray(Kk.indices, dtype=np.int64) + 1
+        vals = np.ascontiguousarray(Kk.data, dtype=np.float64)
+        _check(lib.mf_factor_set_matrix(self.f.handle, Kk.shape[0], colptr.ctypes.data_as(_native.i64p),
+                                        rowval.ctypes.data_as(_native.i64p),
+                                        vals.ctypes.data_as(ctypes.c_void_p), 0))
+        self.seg = np.arange(k + 1, dtype=np.int64) * pat.n
         self.stream = torch.cuda.Stream()
         _check(lib.mf_factor_set_stream(self.f.handle, ctypes.c_void_p(self.stream.cuda_stream)))
         n = S.n
@@ -104,7 +114,7 @@ class _Slot:
         self.vals_d = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
         self.n = n
         self.nrhs = 0
-        self.B_h = self.B_d = None
+        self.B_h = self.B_d = self.X_d = self.nrm_h = self.nrm_d = None
         self.bytes = _slot_bytes(st, n, self.nnz, precision)
         self.busy = False
         self.event = None
@@ -115,6 +125,9 @@ class _Slot:
         if self.nrhs != nrhs:
             self.B_h = torch.empty((nrhs, self.n), dtype=torch.float64, pin_memory=True)
             self.B_d = torch.empty((nrhs, self.n), dtype=torch.float64, device="cuda")
+            self.X_d = torch.empty((nrhs, self.n), dtype=torch.float64, device="cuda")
+            self.nrm_h = torch.empty(2 * self.k * nrhs, dtype=torch.float64, pin_memory=True)
+            self.nrm_d = torch.empty(2 * self.k * nrhs, dtype=torch.float64, device="cuda")
             self.nrhs = nrhs
 
     def release(self):
@@ -124,13 +137,17 @@ class _Slot:
 class ServingLoop:
     """Continuous-batching factor + solve server (see the module docstring).
 
+    Each dispatch solves with ``refine_steps`` FP64 corrections
+    X += F \\ (B - K X) (mf_refine_steps, no host synchronization) and
+    reports every request's final relative residual in ``Request.relres``.
+
     diag:       MF_LDLT (quasi-definite KKT systems, default signs) or MF_CHOLESKY
     ordering:   "mmd" or a callable K -> 1-based order
     hbm_budget: bytes of device memory the resident factor objects may use
     """
 
     def __init__(self, diag=MF_LDLT, precision=MF_FP64, ordering="mmd", hbm_budget=64 << 30,
-                 max_batch=8, streams=4):
+                 max_batch=8, streams=4, refine_steps=1):
         import torch
 
         self.torch = torch
@@ -138,6 +155,7 @@ class ServingLoop:
         self.ordering = ordering
         self.budget = int(hbm_budget)
         self.max_batch = max(1, int(max_batch))
+        self.refine_steps = max(0, int(refine_steps))
         self.nstreams = max(1, int(streams))
         self.patterns = {}
         self.slots = OrderedDict()    # (pattern key, k, serial) -> _Slot, LRU order
@@ -173,7 +191,7 @@ class ServingLoop:
             if key[0] == pat.key and key[1] == k and not s.busy:
                 self.slots.move_to_end(key)
                 return s
-        P, S, flat, plan, st = pat.variant(k)
+        P, S, flat, plan, st, _ = pat.variant(k)
         need = _slot_bytes(st, S.n, len(flat), self.precision)
         if need > self.budget:
             raise MemoryError(f"one factor of this pattern needs {need} bytes > budget {self.budget}")
@@ -209,10 +227,14 @@ class ServingLoop:
             slot.B_d.copy_(slot.B_h, non_blocking=True)
         _check(lib.mf_factor_refresh(slot.f.handle, ctypes.c_void_p(slot.vals_d.data_ptr()), 1))
         _check(lib.mf_factorize_async(slot.f.handle, self.diag, None, MF_REG_NONE, 0.0, 0.0))
-        _check(lib.mf_solve(slot.f.handle, slot.perm.ctypes.data_as(_native.i64p),
-                            ctypes.c_void_p(slot.B_d.data_ptr()), nrhs, slot.n, 1))
+        _check(lib.mf_refine_steps(slot.f.handle, slot.perm.ctypes.data_as(_native.i64p),
+                                   ctypes.c_void_p(slot.B_d.data_ptr()), slot.n,
+                                   ctypes.c_void_p(slot.X_d.data_ptr()), slot.n, nrhs,
+                                   self.refine_steps, slot.k, slot.seg.ctypes.data_as(_native.i64p),
+                                   ctypes.c_void_p(slot.nrm_d.data_ptr())))
         with torch.cuda.stream(slot.stream):
-            slot.B_h.copy_(slot.B_d, non_blocking=True)
+            slot.B_h.copy_(slot.X_d, non_blocking=True)
+            slot.nrm_h.copy_(slot.nrm_d, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(slot.stream)
         slot.busy, slot.event, slot.batch = True, ev, batch
@@ -224,9 +246,13 @@ class ServingLoop:
         info = ctypes.c_int64(0)
         _check(_native.hip().mf_factor_info(slot.f.handle, ctypes.byref(info)))
         X = slot.B_h.numpy()
+        nrm = slot.nrm_h.numpy()
+        k, nrhs = slot.k, slot.nrhs
+        b2, r2 = nrm[:k * nrhs].reshape(nrhs, k), nrm[k * nrhs:].reshape(nrhs, k)
         n1 = slot.pat.n
         for c, r in enumerate(slot.batch):
             r.X = np.array(X[:, c * n1:(c + 1) * n1].T)
+            r.relres = float(np.max(np.sqrt(r2[:, c] / np.where(b2[:, c] > 0, b2[:, c], 1.0))))
             g = int(info.value)
             r.info = 0
             if g:

@@ -151,6 +151,16 @@ int mf_refine(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb, d
 int mf_refine_segmented(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb,
                         double* X, int64_t ldx, int64_t nrhs, int device, double tol, int maxit,
                         int64_t nseg, const int64_t* segptr, int* iters, double* relres);
+/* Asynchronous form for serving loops (device pointers only, nothing waits
+ * on the host): X = F \ B, then exactly `steps` corrections
+ * X += F \ (B - A X), then the final residual.  nrm2 (device, 2 nseg nrhs
+ * doubles) receives sum_i B(i,j)^2 at [j nseg + g] and sum_i R(i,j)^2 at
+ * [nseg nrhs + j nseg + g] over the rows of segment g, so the caller reads
+ * the relative residuals whenever it synchronizes the factor's stream.
+ * segptr is host memory, as for mf_refine_segmented. */
+int mf_refine_steps(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb, double* X,
+                    int64_t ldx, int64_t nrhs, int steps, int64_t nseg, const int64_t* segptr,
+                    double* nrm2);
 
 /*
  * Drop-in forms with chol_impl!'s own arguments: Dval/Lval in (assembled A)
