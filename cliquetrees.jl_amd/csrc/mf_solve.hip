@@ -17,16 +17,17 @@
 //             the product with the inverted diagonal block W;
 //   PT_ROWS   64 update rows of a big front: M2 rows = gather - L21 X.
 // Completion: per-front task counters (children complete / parent complete)
-// and per-front block progress counters.  Thread 0 polls with relaxed loads
-// and issues one agent-scope acquire; after the closing barrier (every wave
-// first drains its stores to L2) thread 0 writes L2 back, waits for the
-// write-back, and only then publishes the task with relaxed atomics.
+// and per-front block progress counters.  Thread 0 polls with relaxed loads;
+// the data those counters guard is only read and written with
+// device-coherent accesses (ldc/stc below), so a task is published by
+// draining every wave's stores, a barrier and one relaxed counter update.
 // All waits are bounded.  Right-hand sides are processed 16 at a time.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 #include "mf_kernels.h"
+#include "mf_mfma.h"
 
 namespace mf {
 
@@ -34,32 +35,33 @@ namespace {
 
 constexpr int RB = 16;  // right-hand sides per pass
 constexpr int LDS = SMALL_NJ + 1;
-constexpr int LDP = NB + 1;
+constexpr int LDX = NB + 1;  // leading dimension of the 64 x 16 LDS tiles
 
-// Waits until this wave's global stores (and loads) have completed, i.e. reached
-// L2.  Before the closing barrier of a task every wave drains, so that thread
-// 0's agent-scope release (an L2 write-back) covers the whole workgroup's data.
+// Waits until this wave's global stores (and loads) have completed.  Every
+// wave drains before the closing barrier of a task, so that the counter
+// update thread 0 makes after the barrier follows all of the task's stores.
 __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0); }
-// Agent-scope acquire (L1/L2 invalidate) and release (L2 write-back), each
-// followed by an explicit wait: the compiler omits the wait after the
-// write-back when its own counters are already zero (after drain_stores), and
-// a flag stored before the write-back completes can be seen by a workgroup on
-// another XCD ahead of the data.  The wait after the invalidate makes it
-// complete before the barrier releases the workgroup's other waves.
-__device__ __forceinline__ void acquire_agent() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __builtin_amdgcn_s_waitcnt(0);
+// Right-hand-side and update-vector entries that other workgroups of the
+// same launch read or write go through device-coherent (sc1) loads and
+// stores, which the caches keep coherent per access.  Publishing a task
+// then needs only its stores to have completed (every wave drains before
+// the closing barrier) and a relaxed flag store; no whole-cache L2
+// write-back or invalidate, which would also evict the factor tiles that
+// other workgroups are reading.  Data written by earlier launches (the
+// factor, W, the permuted input) is read with ordinary cached loads.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void release_agent() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_s_waitcnt(0);
+template <typename T>
+__device__ __forceinline__ void stc(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Bounded wait until *p >= target; gives up (and records it) after ~1 s.
 __device__ __forceinline__ void wait_geq(int32_t* p, int32_t target, int32_t* hang) {
     int spins = 0;
-    // poll with relaxed loads (an acquire per poll would invalidate the caches
-    // every other workgroup is using); the caller issues one acquire fence
+    // poll with relaxed loads (device-coherent; no cache-wide fence)
     while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         if (++spins > (1 << 24)) {
             atomicExch(hang, 1);
@@ -80,12 +82,12 @@ struct Sync {
 };
 
 // Children's update vectors scattered into Y (rows [r0, r1) of front f,
-// column c of the pass at Y[(row - r0) + c * NB]).
+// column c of the pass at Y[(row - r0) + c * LDX]).
 template <typename T>
 __device__ void gather_children(const DevPlan& P, const DevSolve<T>& S, const Front& f, int r0,
                                 int r1, int cb, int nb, T* Y) {
     const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
-    for (int e = tid; e < NB * RB; e += blockDim.x) Y[e] = (T)0;
+    for (int e = tid; e < LDX * RB; e += blockDim.x) Y[e] = (T)0;
     __syncthreads();
     for (int q = f.chd0; q < f.chd1; ++q) {
         const int K = P.chd[q];
@@ -106,10 +108,70 @@ __device__ void gather_children(const DevPlan& P, const DevSolve<T>& S, const Fr
         const T* MvK = S.Mv + P.mvoff1[K] * S.nrhs;
         for (int i = a + r; i < lo; i += 64) {
             const int yr = relK[i] - r0;
-            for (int c = g; c < nb; c += 4) Y[yr + c * NB] += MvK[i + (int64_t)(cb + c) * ma];
+            for (int c = g; c < nb; c += 4) Y[yr + c * LDX] += ldc(MvK + i + (int64_t)(cb + c) * ma);
         }
         __syncthreads();
     }
+}
+
+// ---- 64 x 64 tile products of the big fronts on the matrix cores --------------
+// A wave owns 16 rows of a 64-row block of right-hand sides; lane (li, lk)
+// holds the A operand A[16w + li][4s + lk] for s = 0..15 in registers (so
+// the factor tile of the next step is loaded while the current step waits),
+// the B operand (64 rows x 16 right-hand sides) is staged in LDS.
+
+// a[s] = sgn * base[i * si + j * sj] for i = 16w + li < rows, j = 4s + lk < cols, else 0.
+template <typename T>
+__device__ __forceinline__ void load_frag(const T* base, int64_t si, int64_t sj, int rows, int cols,
+                                          T sgn, T (&a)[NB / 4]) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int i = 16 * (threadIdx.x >> 6) + li;
+#pragma unroll
+    for (int s = 0; s < NB / 4; ++s) {
+        const int j = 4 * s + lk;
+        a[s] = (i < rows && j < cols) ? sgn * base[i * si + j * sj] : (T)0;
+    }
+}
+
+// acc += A (registers) * Xs (64 x 16, LDS, leading dimension LDX)
+template <typename T>
+__device__ __forceinline__ void frag_mma(const T (&a)[NB / 4], const T* Xs,
+                                         typename Mfma<T>::v4& acc) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < NB / 4; ++s) acc = Mfma<T>::op(a[s], Xs[(4 * s + lk) + li * LDX], acc);
+}
+
+// Row of output register v of this lane inside the 64-row block.
+template <typename T>
+__device__ __forceinline__ int acc_row(int v) {
+    return 16 * (threadIdx.x >> 6) + Mfma<T>::row((threadIdx.x & 63) >> 4, v);
+}
+
+// Xs(r, c) = B(row0 + r, cb + c) for r < rows, c < nb (device-coherent loads), else 0.
+template <typename T>
+__device__ __forceinline__ void stage_rows(const T* B, int64_t ldb, int row0, int rows, int cb,
+                                           int nb, T* Xs) {
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = g; c < RB; c += 4)
+        Xs[r + c * LDX] = (r < rows && c < nb) ? ldc(B + row0 + r + (int64_t)(cb + c) * ldb) : (T)0;
+}
+
+// X_k = W acc for the block's rows < kb: acc goes through LDS (Ys) as the B
+// operand; W's fragment is already in registers.
+template <typename T>
+__device__ __forceinline__ typename Mfma<T>::v4 times_w(const T (&wa)[NB / 4], typename Mfma<T>::v4 acc,
+                                                        T* Ys) {
+    typedef typename Mfma<T>::v4 v4;
+    const int li = threadIdx.x & 15;
+    __syncthreads();  // every wave is done reading Ys / Xs of the previous step
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Ys[acc_row<T>(v) + li * LDX] = acc[v];
+    __syncthreads();
+    v4 x = v4{0, 0, 0, 0};
+    frag_mma(wa, Ys, x);
+    return x;
 }
 
 template <typename T>
@@ -133,7 +195,7 @@ __device__ void small_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolv
             if (r < ma) {
                 const int rr = P.rel[kf.reloff + r];
                 const T* MvK = S.Mv + P.mvoff1[K] * S.nrhs;
-                for (int c = g; c < nb; c += 4) X[rr + c * LDS] += MvK[r + (int64_t)(cb + c) * ma];
+                for (int c = g; c < nb; c += 4) X[rr + c * LDS] += ldc(MvK + r + (int64_t)(cb + c) * ma);
             }
             __syncthreads();
         }
@@ -151,8 +213,8 @@ __device__ void small_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolv
         }
         if (r < nj) {
             for (int c = g; c < nb; c += 4) {
-                if (r < nn) S.B[f.c0 + r + (int64_t)(cb + c) * S.ldb] = X[r + c * LDS];
-                else MvJ[(r - nn) + (int64_t)(cb + c) * na] = X[r + c * LDS];
+                if (r < nn) stc(S.B + f.c0 + r + (int64_t)(cb + c) * S.ldb, X[r + c * LDS]);
+                else stc(MvJ + (r - nn) + (int64_t)(cb + c) * na, X[r + c * LDS]);
             }
         }
         __syncthreads();
@@ -172,7 +234,7 @@ __device__ void small_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolv
     for (int cb = 0; cb < S.nrhs; cb += RB) {
         const int nb = min(RB, S.nrhs - cb);
         if (r < nj)
-            for (int c = g; c < nb; c += 4) X[r + c * LDS] = S.B[rows[r] + (int64_t)(cb + c) * S.ldb];
+            for (int c = g; c < nb; c += 4) X[r + c * LDS] = ldc(S.B + rows[r] + (int64_t)(cb + c) * S.ldb);
         __syncthreads();
         if (r < nn) {
             for (int c = g; c < nb; c += 4) {
@@ -193,62 +255,50 @@ __device__ void small_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolv
             __syncthreads();
         }
         if (r < nn)
-            for (int c = g; c < nb; c += 4) S.B[f.c0 + r + (int64_t)(cb + c) * S.ldb] = X[r + c * LDS];
+            for (int c = g; c < nb; c += 4) stc(S.B + f.c0 + r + (int64_t)(cb + c) * S.ldb, X[r + c * LDS]);
         __syncthreads();
     }
 }
 
-// Forward diagonal block [k0, k0+kb) of a big front (rows r = tid & 63,
-// right-hand sides g, g+4, g+8, g+12 of the pass).
+// Forward diagonal block [k0, k0+kb) of a big front: acc = B_k + gathered
+// children - sum_{ib<k} L_{k,ib} X_ib, taking X_ib as the front's progress
+// counter releases them, then X_k = W_k acc.
 template <typename T>
 __device__ void diag_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
-                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* Ps, T* X,
-                         T* Y) {
-    const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
+                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* X, T* Y) {
+    typedef typename Mfma<T>::v4 v4;
+    const int tid = threadIdx.x, li = tid & 15;
     const int nj = f.nn + f.na;
     const T* Pj = F.panel + f.poff;
     const T* W = F.W + f.woff;
     const int kblk = k0 / NB;
     for (int cb = 0; cb < S.nrhs; cb += RB) {
         const int nb = min(RB, S.nrhs - cb);
+        T wa[NB / 4], la[NB / 4];
+        load_frag(W + k0, 1, f.nn, kb, kb, (T)1, wa);
+        if (kblk > 0) load_frag(Pj + k0, 1, nj, kb, NB, (T)-1, la);
         gather_children(P, S, f, k0, k0 + kb, cb, nb, Y);
-        T acc[RB / 4];
+        v4 acc;
 #pragma unroll
-        for (int i = 0; i < RB / 4; ++i) {
-            const int c = g + 4 * i;
-            acc[i] = (r < kb && c < nb) ? S.B[f.c0 + k0 + r + (int64_t)(cb + c) * S.ldb] + Y[r + c * NB]
-                                        : (T)0;
+        for (int v = 0; v < 4; ++v) {
+            const int row = acc_row<T>(v);
+            acc[v] = (row < kb && li < nb)
+                         ? S.B[f.c0 + k0 + row + (int64_t)(cb + li) * S.ldb] + Y[row + li * LDX]
+                         : (T)0;
         }
         for (int ib = 0; ib < kblk; ++ib) {
-            if (tid == 0) {
-                wait_geq(Y_.prog + J, ib + 1, F.hang);
-                acquire_agent();
-            }
+            if (tid == 0) wait_geq(Y_.prog + J, ib + 1, F.hang);
             __syncthreads();
-            for (int c = g; c < nb; c += 4) X[r + c * NB] = S.B[f.c0 + ib * NB + r + (int64_t)(cb + c) * S.ldb];
-            for (int c = g; c < NB; c += 4) Ps[r + c * LDP] = (r < kb) ? Pj[(k0 + r) + (int64_t)(ib * NB + c) * nj] : (T)0;
+            stage_rows(S.B, S.ldb, f.c0 + ib * NB, NB, cb, nb, X);
             __syncthreads();
-            for (int kk = 0; kk < NB; ++kk) {
-                const T lv = Ps[r + kk * LDP];
-#pragma unroll
-                for (int i = 0; i < RB / 4; ++i) acc[i] -= lv * X[kk + (g + 4 * i) * NB];
-            }
-            __syncthreads();
+            frag_mma(la, X, acc);
+            if (ib + 1 < kblk) load_frag(Pj + k0 + (int64_t)(ib + 1) * NB * nj, 1, nj, kb, NB, (T)-1, la);
         }
+        const v4 x = times_w(wa, acc, Y);
 #pragma unroll
-        for (int i = 0; i < RB / 4; ++i) Y[r + (g + 4 * i) * NB] = acc[i];
-        for (int c = g; c < NB; c += 4) Ps[r + c * LDP] = (r < kb && c < kb) ? W[k0 + r + (int64_t)c * f.nn] : (T)0;
-        __syncthreads();
-        if (r < kb) {
-#pragma unroll
-            for (int i = 0; i < RB / 4; ++i) {
-                const int c = g + 4 * i;
-                if (c < nb) {
-                    T s = (T)0;
-                    for (int kk = 0; kk < kb; ++kk) s += Ps[r + kk * LDP] * Y[kk + c * NB];
-                    S.B[f.c0 + k0 + r + (int64_t)(cb + c) * S.ldb] = s;
-                }
-            }
+        for (int v = 0; v < 4; ++v) {
+            const int row = acc_row<T>(v);
+            if (row < kb && li < nb) stc(S.B + f.c0 + k0 + row + (int64_t)(cb + li) * S.ldb, x[v]);
         }
         __syncthreads();
     }
@@ -257,129 +307,114 @@ __device__ void diag_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
 // Forward update rows [r0, r1) (>= nn) of a big front: M2 = gather - L21 X.
 template <typename T>
 __device__ void rows_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
-                         const Front& f, int J, int r0, int r1, const Sync& Y_, T* Ps, T* X,
-                         T* Y) {
-    const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
+                         const Front& f, int J, int r0, int r1, const Sync& Y_, T* X, T* Y) {
+    typedef typename Mfma<T>::v4 v4;
+    const int tid = threadIdx.x, li = tid & 15;
     const int nn = f.nn, na = f.na, nj = nn + na, m = r1 - r0;
     const T* Pj = F.panel + f.poff;
     const int p = (nn + NB - 1) / NB;
     T* MvJ = S.Mv + P.mvoff1[J] * S.nrhs;
     for (int cb = 0; cb < S.nrhs; cb += RB) {
         const int nb = min(RB, S.nrhs - cb);
+        T la[NB / 4];
+        load_frag(Pj + r0, 1, nj, m, min(NB, nn), (T)-1, la);
         gather_children(P, S, f, r0, r1, cb, nb, Y);
-        T acc[RB / 4];
+        v4 acc;
 #pragma unroll
-        for (int i = 0; i < RB / 4; ++i) acc[i] = Y[r + (g + 4 * i) * NB];
+        for (int v = 0; v < 4; ++v) {
+            const int row = acc_row<T>(v);
+            acc[v] = (row < m && li < nb) ? Y[row + li * LDX] : (T)0;
+        }
         for (int ib = 0; ib < p; ++ib) {
             const int kbi = min(NB, nn - ib * NB);
-            if (tid == 0) {
-                wait_geq(Y_.prog + J, ib + 1, F.hang);
-                acquire_agent();
-            }
+            if (tid == 0) wait_geq(Y_.prog + J, ib + 1, F.hang);
             __syncthreads();
-            for (int c = g; c < nb; c += 4)
-                X[r + c * NB] = (r < kbi) ? S.B[f.c0 + ib * NB + r + (int64_t)(cb + c) * S.ldb] : (T)0;
-            for (int c = g; c < NB; c += 4)
-                Ps[r + c * LDP] = (r < m && c < kbi) ? Pj[(r0 + r) + (int64_t)(ib * NB + c) * nj] : (T)0;
+            stage_rows(S.B, S.ldb, f.c0 + ib * NB, kbi, cb, nb, X);
             __syncthreads();
-            for (int kk = 0; kk < kbi; ++kk) {
-                const T lv = Ps[r + kk * LDP];
-#pragma unroll
-                for (int i = 0; i < RB / 4; ++i) acc[i] -= lv * X[kk + (g + 4 * i) * NB];
-            }
-            __syncthreads();
+            frag_mma(la, X, acc);
+            if (ib + 1 < p)
+                load_frag(Pj + r0 + (int64_t)(ib + 1) * NB * nj, 1, nj, m, min(NB, nn - (ib + 1) * NB),
+                          (T)-1, la);
         }
-        if (r < m) {
 #pragma unroll
-            for (int i = 0; i < RB / 4; ++i) {
-                const int c = g + 4 * i;
-                if (c < nb) MvJ[(r0 + r - nn) + (int64_t)(cb + c) * na] = acc[i];
-            }
+        for (int v = 0; v < 4; ++v) {
+            const int row = acc_row<T>(v);
+            if (row < m && li < nb) stc(MvJ + (r0 + row - nn) + (int64_t)(cb + li) * na, acc[v]);
         }
         __syncthreads();
     }
 }
 
-// Backward diagonal block [k0, k0+kb): columns r = tid & 63 of the block.
+// Backward diagonal block [k0, k0+kb): acc = C_k - L21' x_sep - sum_{ib>k}
+// L_{ib,k}' X_ib (x_sep final: the ancestors are complete; X_ib as the
+// progress counter releases them, last block first), then X_k = W_k' acc.
 template <typename T>
 __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
-                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* Ps, T* X,
-                         T* Y) {
-    const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
+                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* X, T* Y) {
+    typedef typename Mfma<T>::v4 v4;
+    const int tid = threadIdx.x, r = tid & 63, g = tid >> 6, li = tid & 15;
     const int nn = f.nn, nj = nn + f.na;
     const T* Pj = F.panel + f.poff;
     const T* W = F.W + f.woff;
     const int32_t* rows = P.rows + f.rowoff;
     const int p = (nn + NB - 1) / NB, kblk = k0 / NB;
+    const int nsep = (nj - nn + NB - 1) / NB, nstep = nsep + (p - 1 - kblk);
+    // A fragment of step q: separator tile q (L21' rows nn + 64q...), then the
+    // blocks below the diagonal block, last first (L_{ib,k}')
+    auto frag = [&](int q, T (&a)[NB / 4]) {
+        if (q < nsep) {
+            const int rt = nn + q * NB;
+            load_frag(Pj + rt + (int64_t)k0 * nj, nj, 1, kb, min(NB, nj - rt), (T)-1, a);
+        } else {
+            const int ib = p - 1 - (q - nsep);
+            load_frag(Pj + ib * NB + (int64_t)k0 * nj, nj, 1, kb, min(NB, nn - ib * NB), (T)-1, a);
+        }
+    };
     for (int cb = 0; cb < S.nrhs; cb += RB) {
         const int nb = min(RB, S.nrhs - cb);
-        T acc[RB / 4];
+        T wa[NB / 4], la[NB / 4];
+        load_frag(W + k0, f.nn, 1, kb, kb, (T)1, wa);
+        if (nstep > 0) frag(0, la);
+        v4 acc;
 #pragma unroll
-        for (int i = 0; i < RB / 4; ++i) {
-            const int c = g + 4 * i;
-            acc[i] = (r < kb && c < nb) ? S.B[f.c0 + k0 + r + (int64_t)(cb + c) * S.ldb] : (T)0;
+        for (int v = 0; v < 4; ++v) {
+            const int row = acc_row<T>(v);
+            acc[v] = (row < kb && li < nb) ? S.B[f.c0 + k0 + row + (int64_t)(cb + li) * S.ldb] : (T)0;
         }
-        // C1 -= L21' x_sep (x_sep final: ancestors are complete)
-        for (int rt = nn; rt < nj; rt += NB) {
-            const int nr = min(NB, nj - rt);
-            for (int c = g; c < nb; c += 4)
-                X[r + c * NB] = (r < nr) ? S.B[rows[rt + r] + (int64_t)(cb + c) * S.ldb] : (T)0;
-            for (int c = g; c < NB; c += 4)
-                Ps[r + c * LDP] = (r < nr && c < kb) ? Pj[(rt + r) + (int64_t)(k0 + c) * nj] : (T)0;
-            __syncthreads();
-            for (int q = 0; q < nr; ++q) {
-                const T lv = Ps[q + r * LDP];
+        for (int q = 0; q < nstep; ++q) {
+            if (q < nsep) {
+                const int rt = nn + q * NB, nr = min(NB, nj - rt);
+                __syncthreads();
 #pragma unroll
-                for (int i = 0; i < RB / 4; ++i) acc[i] -= lv * X[q + (g + 4 * i) * NB];
+                for (int c = g; c < RB; c += 4)
+                    X[r + c * LDX] = (r < nr && c < nb)
+                                         ? ldc(S.B + rows[rt + r] + (int64_t)(cb + c) * S.ldb) : (T)0;
+            } else {
+                const int ib = p - 1 - (q - nsep);
+                if (tid == 0) wait_geq(Y_.prog + J, p - ib, F.hang);
+                __syncthreads();
+                stage_rows(S.B, S.ldb, f.c0 + ib * NB, min(NB, nn - ib * NB), cb, nb, X);
             }
             __syncthreads();
+            frag_mma(la, X, acc);
+            if (q + 1 < nstep) frag(q + 1, la);
         }
-        // blocks below the diagonal block, finished in decreasing order
-        for (int ib = p - 1; ib > kblk; --ib) {
-            const int kbi = min(NB, nn - ib * NB);
-            if (tid == 0) {
-                wait_geq(Y_.prog + J, p - ib, F.hang);
-                acquire_agent();
-            }
-            __syncthreads();
-            for (int c = g; c < nb; c += 4)
-                X[r + c * NB] = (r < kbi) ? S.B[f.c0 + ib * NB + r + (int64_t)(cb + c) * S.ldb] : (T)0;
-            for (int c = g; c < NB; c += 4)
-                Ps[r + c * LDP] = (r < kbi && c < kb) ? Pj[(ib * NB + r) + (int64_t)(k0 + c) * nj] : (T)0;
-            __syncthreads();
-            for (int q = 0; q < kbi; ++q) {
-                const T lv = Ps[q + r * LDP];
+        const v4 x = times_w(wa, acc, Y);
 #pragma unroll
-                for (int i = 0; i < RB / 4; ++i) acc[i] -= lv * X[q + (g + 4 * i) * NB];
-            }
-            __syncthreads();
-        }
-        // X_k = W_k' acc
-#pragma unroll
-        for (int i = 0; i < RB / 4; ++i) Y[r + (g + 4 * i) * NB] = acc[i];
-        for (int c = g; c < NB; c += 4) Ps[r + c * LDP] = (r < kb && c < kb) ? W[k0 + r + (int64_t)c * f.nn] : (T)0;
-        __syncthreads();
-        if (r < kb) {
-#pragma unroll
-            for (int i = 0; i < RB / 4; ++i) {
-                const int c = g + 4 * i;
-                if (c < nb) {
-                    T s = (T)0;
-                    for (int q = 0; q < kb; ++q) s += Ps[q + r * LDP] * Y[q + c * NB];
-                    S.B[f.c0 + k0 + r + (int64_t)(cb + c) * S.ldb] = s;
-                }
-            }
+        for (int v = 0; v < 4; ++v) {
+            const int row = acc_row<T>(v);
+            if (row < kb && li < nb) stc(S.B + f.c0 + k0 + row + (int64_t)(cb + li) * S.ldb, x[v]);
         }
         __syncthreads();
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_fsolve(DevPlan P, DevFactor<T> F, DevSolve<T> S,
+__global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, DevSolve<T> S,
                                                 const PTask* tasks, int count, Sync Y_) {
     __shared__ T Ps[SMALL_NJ * LDS];
     __shared__ T X[LDS * RB];
-    __shared__ T Y[NB * RB];
+    __shared__ T Y[LDX * RB];
     __shared__ int s_t;
     const int tid = threadIdx.x;
     for (;;) {
@@ -396,16 +431,14 @@ __global__ __launch_bounds__(256) void k_fsolve(DevPlan P, DevFactor<T> F, DevSo
                 const int K = P.chd[q];
                 wait_geq(Y_.done + K, Y_.nt[K], F.hang);
             }
-            acquire_agent();
         }
         __syncthreads();
         if (tk.kind == PT_SMALL) small_fwd(P, F, S, f, J, Ps, X);
-        else if (tk.kind == PT_DIAG) diag_fwd(P, F, S, f, J, tk.a, tk.b, Y_, Ps, X, Y);
-        else rows_fwd(P, F, S, f, J, tk.a, tk.b, Y_, Ps, X, Y);
+        else if (tk.kind == PT_DIAG) diag_fwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y);
+        else rows_fwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y);
         drain_stores();
         __syncthreads();
         if (tid == 0) {
-            release_agent();
             if (tk.kind == PT_DIAG)
                 __hip_atomic_store(Y_.prog + J, tk.a / NB + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(Y_.done + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -417,11 +450,11 @@ __global__ __launch_bounds__(256) void k_fsolve(DevPlan P, DevFactor<T> F, DevSo
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bsolve(DevPlan P, DevFactor<T> F, DevSolve<T> S,
+__global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, DevSolve<T> S,
                                                 const PTask* tasks, int count, Sync Y_) {
     __shared__ T Ps[SMALL_NJ * LDS];
     __shared__ T X[LDS * RB];
-    __shared__ T Y[NB * RB];
+    __shared__ T Y[LDX * RB];
     __shared__ int s_t;
     const int tid = threadIdx.x;
     for (;;) {
@@ -435,15 +468,13 @@ __global__ __launch_bounds__(256) void k_bsolve(DevPlan P, DevFactor<T> F, DevSo
         const Front f = P.fronts[J];
         if (tid == 0) {
             if (f.parent >= 0) wait_geq(Y_.done + f.parent, Y_.nt[f.parent], F.hang);
-            acquire_agent();
         }
         __syncthreads();
         if (tk.kind == PT_SMALL) small_bwd(P, F, S, f, Ps, X);
-        else diag_bwd(P, F, S, f, J, tk.a, tk.b, Y_, Ps, X, Y);
+        else diag_bwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y);
         drain_stores();
         __syncthreads();
         if (tid == 0) {
-            release_agent();
             if (tk.kind == PT_DIAG) {
                 const int p = (f.nn + NB - 1) / NB;
                 __hip_atomic_store(Y_.prog + J, p - tk.a / NB, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
