@@ -39,6 +39,9 @@ SIGNATURES = {
     "mf_dag_profile_read": ([vp, f64p, ci], ci),
     "mf_dag_trace_read": ([vp, vp, i64], ci),
     "mf_plan_dag": ([vp, vp, i64], ci),
+    "mf_solve_trace_read": ([vp, vp, i64], ci),
+    "mf_plan_solve_tasks": ([vp, vp, i64], ci),
+    "mf_plan_fronts": ([vp, i64p, i64], ci),
 }
 
 # Names of the C-ABI entry points declared in include/mfhip.h.

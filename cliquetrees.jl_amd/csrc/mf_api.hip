@@ -100,6 +100,7 @@ struct mf_factor {
     size_t ir_ws_bytes = 0;
     unsigned long long* prof_buf = nullptr;
     unsigned long long* trace_buf = nullptr;   // per DAG ticket: start, end
+    unsigned long long* strace_buf = nullptr;  // per solve ticket (forward, then backward)
     int64_t* tgt_panel = nullptr;
     int64_t* tgt_ref = nullptr;
     int64_t nnz = -1;
@@ -250,7 +251,8 @@ void enqueue_solve(mf_factor* f, T* B, int64_t ldb, int nrhs) {
     double bytes = (double)sizeof(T) * (2.0 * (double)(H.ndz + H.nlz) + 4.0 * H.n * nrhs);
     timed(f, MF_KC_SOLVE, bytes, 1, [&] {
         launch_solve<T>(P, F, S, P.nJ, f->flags, f->prog, pl->d_ntf, pl->d_ntb, pl->d_ftask,
-                        (int)H.ftask.size(), pl->d_btask, (int)H.btask.size(), (int)H.n, f->stream);
+                        (int)H.ftask.size(), pl->d_btask, (int)H.btask.size(), (int)H.n,
+                        f->prof ? f->strace_buf : nullptr, f->stream);
     });
 }
 
@@ -879,6 +881,11 @@ int mf_dag_profile(mf_factor* f, int enable) {
         f->allocs.push_back(p);
         f->trace_buf = (unsigned long long*)p;
         launch_fill<unsigned long long>(f->trace_buf, 2 * (nt + 1), 0ull, f->stream);
+        const size_t ns = f->plan->H.ftask.size() + f->plan->H.btask.size();
+        if (hipMalloc(&p, 16 * (ns + 1)) != hipSuccess) return fail(MF_ENOMEM, "hipMalloc failed (trace)");
+        f->allocs.push_back(p);
+        f->strace_buf = (unsigned long long*)p;
+        launch_fill<unsigned long long>(f->strace_buf, 2 * (ns + 1), 0ull, f->stream);
     }
     f->prof = enable ? f->prof_buf : nullptr;
     return MF_OK;
@@ -891,6 +898,42 @@ int mf_dag_trace_read(mf_factor* f, unsigned long long* out, int64_t n) {
     if (!f->trace_buf) return fail(MF_ESTATE, "profiling is off");
     HIPCHK(hipMemcpyAsync(out, f->trace_buf, 16 * nt, hipMemcpyDeviceToHost, f->stream));
     HIPCHK(hipStreamSynchronize(f->stream));
+    return MF_OK;
+}
+
+int mf_solve_trace_read(mf_factor* f, unsigned long long* out, int64_t n) {
+    if (!f || !out) return fail(MF_EARG, "null argument");
+    const int64_t ns = (int64_t)(f->plan->H.ftask.size() + f->plan->H.btask.size());
+    if (n < 2 * ns) return fail(MF_EARG, "trace buffer too small (2 x solve tasks)");
+    if (!f->strace_buf) return fail(MF_ESTATE, "profiling is off");
+    HIPCHK(hipMemcpyAsync(out, f->strace_buf, 16 * ns, hipMemcpyDeviceToHost, f->stream));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    return MF_OK;
+}
+
+int mf_plan_solve_tasks(const mf_plan* pl, int32_t* tasks, int64_t n) {
+    if (!pl || !tasks) return fail(MF_EARG, "null argument");
+    const HostPlan& H = pl->H;
+    const size_t ns = H.ftask.size() + H.btask.size();
+    if (n < 4 * (int64_t)ns) return fail(MF_EARG, "buffer too small (4 x solve tasks)");
+    size_t t = 0;
+    for (const std::vector<PTask>* tl : {&H.ftask, &H.btask})
+        for (const PTask& k : *tl) {
+            const int32_t v[4] = {k.kind, k.front, k.a, k.b};
+            std::copy(v, v + 4, tasks + 4 * t++);
+        }
+    return MF_OK;
+}
+
+int mf_plan_fronts(const mf_plan* pl, int64_t* out, int64_t n) {
+    if (!pl || !out) return fail(MF_EARG, "null argument");
+    const HostPlan& H = pl->H;
+    if (n < 5 * (int64_t)H.fronts.size()) return fail(MF_EARG, "buffer too small (5 x fronts)");
+    for (size_t J = 0; J < H.fronts.size(); ++J) {
+        const Front& F = H.fronts[J];
+        const int64_t v[5] = {F.c0, F.nn, F.na, F.parent, (int64_t)pl->in_small[J]};
+        std::copy(v, v + 5, out + 5 * J);
+    }
     return MF_OK;
 }
 

@@ -340,34 +340,66 @@ __device__ void potrf_tile(const DevFactor<T>& F, const Front& f, int k0, bool i
 }
 
 constexpr int PB = 16;                          // block size inside the diagonal tile
-constexpr int PTILE = NB * LDS + 3 * PB * PB + PB + NB;  // LDS elements of potrf_blocked
+constexpr int PTILE = NB * LDS + 3 * PB * PB + 2 * NB;  // LDS elements of potrf_blocked
+
+// Value of v held by lane l of the wavefront (l wave-uniform).
+__device__ __forceinline__ double bcast(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+// 16 x 16 block product on the matrix cores: acc += sum over the K = 16 * kt
+// columns of  opA(i, k) * opB(k, j), with opA(i, k) = X[(ra + i) + (ca + k) * LDS]
+// (times sc[ca + k] when sc is given) and opB(k, j) = Y[(rb + k) + (cb + j) * ldy].
+template <typename T>
+__device__ __forceinline__ void blk_mma(const T* X, int ra, int ca, const T* Y, int rb, int cb, int ldy,
+                                        int kt, const T* sc, typename Mfma<T>::v4& acc) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    for (int s = 0; s < 4 * kt; ++s) {
+        const int k = 4 * s + lk;
+        T av = X[(ra + li) + (ca + k) * LDS];
+        if (sc) av *= sc[ca + k];
+        acc = Mfma<T>::op(av, Y[(rb + k) + (cb + li) * ldy], acc);
+    }
+}
 
 // Diagonal tile, blocked: the 64 x 64 tile lives in LDS and is factored in
 // four 16-column block steps (potrf2!'s right-looking recursion cut at 16):
 // (1) one wavefront factors the 16 x 16 diagonal block in registers with the
-// pivot rule, (2) one lane per row solves the rows below against it, (3) all
-// threads apply the rank-16 update to the trailing lower triangle.  Then
-// W = L^-1 in place: the four diagonal blocks are inverted one per
-// wavefront, and the block rows below follow in three stages of
-// W_ij = -W_ii (sum_k L_ik W_kj).  About 20 barriers instead of one per
-// column.  L goes to the panel and W (ld = nn, zeros above the diagonal) to
-// the W pool.  LDS: PTILE elements.
+// pivot rule (columns broadcast with readlane, no LDS round trip) and keeps
+// the reciprocals of the diagonal, (2) one lane per row solves the rows below
+// against it, (3) the rank-16 update of the trailing lower triangle runs on
+// the matrix cores, one 16 x 16 block per wavefront.  Then W = L^-1 in place:
+// the four diagonal blocks are inverted one per wavefront, and the block rows
+// below follow in three stages of W_ij = -W_ii (sum_k L_ik W_kj), again on the
+// matrix cores.  L goes to the panel and W (ld = nn, zeros above the
+// diagonal) to the W pool.  LDS: PTILE elements.
 template <typename T>
 __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front& f, int k0, bool invert_only,
                               T* lds, const int tid) {
+    typedef typename Mfma<T>::v4 v4;
     T* A = lds;                       // 64 x LDS, column-major
-    T* Tm = lds + NB * LDS;           // 3 blocks of 16 x 16 (stage products)
-    T* colv = Tm + 3 * PB * PB;       // one column of the diagonal block
-    T* pv = colv + PB;                // pivots of the tile (LDLt scaling)
+    T* Tm = lds + NB * LDS;           // 3 blocks of 16 x 16 (stage products, ld 16)
+    T* pv = Tm + 3 * PB * PB;         // pivots of the tile (LDLt scaling)
+    T* rd = pv + NB;                  // reciprocals of L's diagonal (Cholesky)
     const int nj = f.nn + f.na, kb = min(NB, f.nn - k0), gc0 = f.c0 + k0;
     T* blk = F.panel + f.poff + k0 + (int64_t)k0 * nj;
-    const int r = tid & 63, cg = tid >> 6, lane = tid & 63;
+    const int r = tid & 63, cg = tid >> 6, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
     const bool chol = F.diag == 0;
     for (int c = cg; c < NB; c += 4) {
         const T* colp = blk + (int64_t)c * nj;
         A[r + c * LDS] = (r < kb && c < kb && c <= r) ? colp[r] : (r == c ? (T)1 : (T)0);
     }
-    if (tid < NB) pv[tid] = (T)1;
+    if (tid < NB) {
+        pv[tid] = (T)1;
+        rd[tid] = (T)1;
+    }
     __syncthreads();
     if (!invert_only) {
         for (int b = 0; b < NB / PB; ++b) {
@@ -381,24 +413,48 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
 #pragma unroll
                 for (int k = 0; k < PB; ++k) {
                     if (c0 + k >= kb) continue;  // uniform
-                    if (lane < PB) colv[lane] = x[k];
+                    // the lane id is re-read per step so that the compiler does not
+                    // keep sixteen steps' worth of lane masks live in scalar registers
+                    int ln = lane;
+                    asm volatile("" : "+v"(ln));
+                    T colv[PB];                  // column k as it stands (lane j's x[k])
+#pragma unroll
+                    for (int j = k; j < PB; ++j) colv[j] = bcast(x[k], j);
                     const T akk = colv[k];
                     double piv = regularize(F, sign_of(F, gc0 + c0 + k), (double)akk);
                     const bool bad = (piv == 0.0);
                     if (bad) piv = 1.0;  // keeps the rest finite after a failure
-                    if (lane == 0) {
+                    // 1/sqrt(piv) and 1/piv from the hardware estimates plus two
+                    // Newton steps each (full double precision, a short
+                    // dependency chain on the pivot-to-pivot path)
+                    T ip, sq = (T)1, isq = (T)1;
+                    if (chol) {
+                        double y = __builtin_amdgcn_rsq(piv);
+                        const double h = 0.5 * piv;
+                        y = y * __builtin_fma(-h * y, y, 1.5);
+                        y = y * __builtin_fma(-h * y, y, 1.5);
+                        isq = (T)y;
+                        sq = (T)(piv * y);
+                        ip = (T)(y * y);
+                    } else {
+                        double y = __builtin_amdgcn_rcp(piv);
+                        y = __builtin_fma(__builtin_fma(-piv, y, 1.0), y, y);
+                        y = __builtin_fma(__builtin_fma(-piv, y, 1.0), y, y);
+                        ip = (T)y;
+                    }
+                    const T ark = x[k], ar = ark * ip;
+                    if (ln == 0) {
                         if (bad) atomicMin(F.info, (unsigned long long)(gc0 + c0 + k + 1));
                         else if (!chol) F.d[gc0 + c0 + k] = (T)piv;
                         pv[c0 + k] = (T)piv;
+                        rd[c0 + k] = isq;
                     }
-                    const T ip = (T)(1.0 / piv), ark = x[k], ar = ark * ip;
 #pragma unroll
                     for (int j = k + 1; j < PB; ++j) x[j] -= ar * colv[j];
-                    if (lane >= k) {
+                    if (ln >= k) {
                         if (chol) {
-                            const T sq = (T)sqrt(piv);
-                            x[k] = (lane == k) ? sq : ark * ((T)1 / sq);
-                        } else if (lane > k) {
+                            x[k] = (ln == k) ? sq : ark * isq;
+                        } else if (ln > k) {
                             x[k] = ar;  // LDLt keeps the unregularized pivot on the diagonal
                         }
                     }
@@ -418,7 +474,7 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
                 for (int j = 0; j < PB; ++j) y[j] = A[i + (c0 + j) * LDS];
 #pragma unroll
                 for (int k = 0; k < PB; ++k) {
-                    if (chol) y[k] *= (T)1 / A[(c0 + k) + (c0 + k) * LDS];
+                    if (chol) y[k] *= rd[c0 + k];
 #pragma unroll
                     for (int j = k + 1; j < PB; ++j) y[j] -= y[k] * A[(c0 + j) + (c0 + k) * LDS];
                 }
@@ -426,17 +482,27 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
                 for (int k = 0; k < PB; ++k) A[i + (c0 + k) * LDS] = chol ? y[k] : y[k] / pv[c0 + k];
             }
             __syncthreads();
-            // (3) trailing update of the lower triangle: A_ij -= sum_k L_ik s_k L_jk
-            const int nr = NB - rb;
-            for (int e = tid; e < nr * nr; e += 256) {
-                const int i = rb + e % nr, j = rb + e / nr;
-                if (j > i) continue;
-                T acc = (T)0;
+            // (3) trailing update of the lower triangle, A_ij -= sum_k L_ik s_k L_jk,
+            // one 16 x 16 block (ti >= tj > b) per wavefront on the matrix cores
+            {
+                int q = 0;
+                for (int ti = b + 1; ti < NB / PB; ++ti)
+                    for (int tj = b + 1; tj <= ti; ++tj, ++q) {
+                        if ((q & 3) != wave) continue;
+                        v4 acc = v4{0, 0, 0, 0};
 #pragma unroll
-                for (int k = 0; k < PB; ++k)
-                    acc += A[i + (c0 + k) * LDS] * (chol ? A[j + (c0 + k) * LDS]
-                                                        : A[j + (c0 + k) * LDS] * pv[c0 + k]);
-                A[i + j * LDS] -= acc;
+                        for (int s4 = 0; s4 < PB / 4; ++s4) {
+                            const int k = 4 * s4 + lk;
+                            T av = A[(PB * ti + li) + (c0 + k) * LDS];
+                            if (!chol) av *= pv[c0 + k];
+                            acc = Mfma<T>::op(av, A[(PB * tj + li) + (c0 + k) * LDS], acc);
+                        }
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int row = PB * ti + Mfma<T>::row(lk, v), col = PB * tj + li;
+                            if (row >= col) A[row + col * LDS] -= acc[v];
+                        }
+                    }
             }
             __syncthreads();
         }
@@ -445,7 +511,10 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
             if (r < kb && c <= r) colp[r] = A[r + c * LDS];
         }
         __syncthreads();  // L is read out before the inverse overwrites it
+    } else if (tid < NB) {
+        rd[tid] = chol ? (T)1 / A[tid + tid * LDS] : (T)1;
     }
+    if (invert_only) __syncthreads();
     // W = L^-1 in place.  Diagonal blocks: wavefront w inverts block w; lane
     // c < 16 keeps column c of W_ww (forward substitution, unit for LDLt).
     {
@@ -456,7 +525,7 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
             for (int i = 0; i < PB; ++i) w[i] = (i == lane) ? (T)1 : (T)0;
 #pragma unroll
             for (int k = 0; k < PB; ++k) {
-                if (chol) w[k] *= (T)1 / A[(c0 + k) + (c0 + k) * LDS];
+                if (chol) w[k] *= rd[c0 + k];
 #pragma unroll
                 for (int i = k + 1; i < PB; ++i) w[i] -= A[(c0 + i) + (c0 + k) * LDS] * w[k];
             }
@@ -469,23 +538,23 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
     }
     __syncthreads();
     for (int bi = 1; bi < NB / PB; ++bi) {
-        // T_ij = sum_{k=j}^{i-1} L_ik W_kj for j < i (W_kj rows done, L_ik intact)
-        for (int e = tid; e < bi * PB * PB; e += 256) {
-            const int bj = e / (PB * PB), rr = e % PB, cc = (e / PB) % PB;
-            const int i = bi * PB + rr, j = bj * PB + cc;
-            T acc = (T)0;
-            for (int m = bj * PB; m < bi * PB; ++m) acc += A[i + m * LDS] * A[m + j * LDS];
-            Tm[bj * PB * PB + rr + cc * PB] = acc;
+        // T_ij = sum_{k=j}^{i-1} L_ik W_kj for j < i (W_kj rows done, L_ik intact):
+        // block bj by wavefront bj
+        if (wave < bi) {
+            const int bj = wave;
+            v4 acc = v4{0, 0, 0, 0};
+            blk_mma<T>(A, PB * bi, PB * bj, A, PB * bj, PB * bj, LDS, bi - bj, nullptr, acc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Tm[bj * PB * PB + Mfma<T>::row(lk, v) + li * PB] = acc[v];
         }
         __syncthreads();
         // W_ij = -W_ii T_ij
-        for (int e = tid; e < bi * PB * PB; e += 256) {
-            const int bj = e / (PB * PB), rr = e % PB, cc = (e / PB) % PB;
-            const int i = bi * PB + rr, j = bj * PB + cc;
-            T acc = (T)0;
+        if (wave < bi) {
+            const int bj = wave;
+            v4 acc = v4{0, 0, 0, 0};
+            blk_mma<T>(A, PB * bi, PB * bi, Tm + bj * PB * PB, 0, 0, PB, 1, nullptr, acc);
 #pragma unroll
-            for (int m = 0; m < PB; ++m) acc += A[i + (bi * PB + m) * LDS] * Tm[bj * PB * PB + m + cc * PB];
-            A[i + j * LDS] = -acc;
+            for (int v = 0; v < 4; ++v) A[(PB * bi + Mfma<T>::row(lk, v)) + (PB * bj + li) * LDS] = -acc[v];
         }
         __syncthreads();
     }

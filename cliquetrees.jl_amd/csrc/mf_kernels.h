@@ -101,10 +101,12 @@ struct DevSolve {
 };
 // Forward (children first) and backward (parents first) persistent solves,
 // the D^-1 scaling in between for LDLt.
+// trace: nullptr, or 2 (nf + nb) slots for per-ticket start / end ticks.
 template <typename T>
 void launch_solve(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int nJ,
                   int32_t* done, int32_t* prog, const int32_t* ntf, const int32_t* ntb,
-                  const PTask* ftask, int nf, const PTask* btask, int nb, int n, hipStream_t s);
+                  const PTask* ftask, int nf, const PTask* btask, int nb, int n,
+                  unsigned long long* trace, hipStream_t s);
 template <typename TB>
 void launch_permute(const double* src, int64_t lds, TB* dst, int64_t ldd, const int64_t* perm,
                     int n, int nrhs, bool inverse, hipStream_t s);

@@ -79,6 +79,7 @@ struct Sync {
     int32_t* prog;    // per front: finished diagonal blocks (in chain order)
     const int32_t* nt;
     uint32_t* ticket;
+    unsigned long long* trace;  // nullptr or per ticket: start | workgroup << 48, end
 };
 
 // Children's update vectors scattered into Y (rows [r0, r1) of front f,
@@ -126,10 +127,14 @@ __device__ __forceinline__ void load_frag(const T* base, int64_t si, int64_t sj,
                                           T sgn, T (&a)[NB / 4]) {
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const int i = 16 * (threadIdx.x >> 6) + li;
+    // one running pointer (the column stride is not a compile-time constant)
+    const T* q = base + (int64_t)i * si + (int64_t)lk * sj;
+    const int64_t step = 4 * sj;
+    const bool row_ok = i < rows;
 #pragma unroll
     for (int s = 0; s < NB / 4; ++s) {
-        const int j = 4 * s + lk;
-        a[s] = (i < rows && j < cols) ? sgn * base[i * si + j * sj] : (T)0;
+        a[s] = (row_ok && 4 * s + lk < cols) ? sgn * *q : (T)0;
+        q += step;
     }
 }
 
@@ -423,6 +428,7 @@ __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, De
         const int t = s_t;
         __syncthreads();
         if (t >= count) break;
+        const unsigned long long t_begin = (Y_.trace && tid == 0) ? wall_clock64() : 0;
         const PTask tk = tasks[t];
         const int J = tk.front;
         const Front f = P.fronts[J];
@@ -442,6 +448,10 @@ __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, De
             if (tk.kind == PT_DIAG)
                 __hip_atomic_store(Y_.prog + J, tk.a / NB + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(Y_.done + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (Y_.trace) {
+                Y_.trace[2 * t] = t_begin | ((unsigned long long)blockIdx.x << 48);
+                Y_.trace[2 * t + 1] = wall_clock64();
+            }
         }
         // closes the thread-0 block before the back-edge: every thread of the
         // workgroup reaches the next ticket together
@@ -463,6 +473,7 @@ __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, De
         const int t = s_t;
         __syncthreads();
         if (t >= count) break;
+        const unsigned long long t_begin = (Y_.trace && tid == 0) ? wall_clock64() : 0;
         const PTask tk = tasks[t];
         const int J = tk.front;
         const Front f = P.fronts[J];
@@ -480,6 +491,10 @@ __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, De
                 __hip_atomic_store(Y_.prog + J, p - tk.a / NB, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             __hip_atomic_fetch_add(Y_.done + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (Y_.trace) {
+                Y_.trace[2 * t] = t_begin | ((unsigned long long)blockIdx.x << 48);
+                Y_.trace[2 * t + 1] = wall_clock64();
+            }
         }
         // closes the thread-0 block before the back-edge: every thread of the
         // workgroup reaches the next ticket together
@@ -523,9 +538,10 @@ inline unsigned blocks_for(int64_t tot) { return (unsigned)((tot + 255) / 256); 
 template <typename T>
 void launch_solve(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int nJ,
                   int32_t* done, int32_t* prog, const int32_t* ntf, const int32_t* ntb,
-                  const PTask* ftask, int nf, const PTask* btask, int nb, int n, hipStream_t s) {
+                  const PTask* ftask, int nf, const PTask* btask, int nb, int n,
+                  unsigned long long* trace, hipStream_t s) {
     if (nf <= 0) return;
-    Sync Y{done, prog, ntf, F.ticket};
+    Sync Y{done, prog, ntf, F.ticket, trace};
     const int grid_f = nf < 1024 ? nf : 1024;
     launch_fill<int32_t>(done, nJ, 0, s);
     launch_fill<int32_t>(prog, nJ, 0, s);
@@ -535,6 +551,7 @@ void launch_solve(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
         hipLaunchKernelGGL(k_dscale<T>, dim3(blocks_for((int64_t)n * S.nrhs)), dim3(256), 0, s,
                            (const T*)F.d, S.B, S.ldb, n, S.nrhs);
     Y.nt = ntb;
+    if (trace) Y.trace = trace + 2 * (size_t)nf;
     const int grid_b = nb < 1024 ? nb : 1024;
     launch_fill<int32_t>(done, nJ, 0, s);
     launch_fill<int32_t>(prog, nJ, 0, s);
@@ -566,7 +583,8 @@ void launch_permute_back(const void* src, int64_t lds, bool src_f32, double* dst
 #define MF_SINST(T)                                                                           \
     template void launch_solve<T>(const DevPlan&, const DevFactor<T>&, const DevSolve<T>&, int, \
                                   int32_t*, int32_t*, const int32_t*, const int32_t*,         \
-                                  const PTask*, int, const PTask*, int, int, hipStream_t);    \
+                                  const PTask*, int, const PTask*, int, int,                  \
+                                  unsigned long long*, hipStream_t);                          \
     template void launch_permute<T>(const double*, int64_t, T*, int64_t, const int64_t*, int,  \
                                     int, bool, hipStream_t);
 MF_SINST(double)
