@@ -210,6 +210,9 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
 // (M22 <- F22, cholesky.jl:542).  The child rows that land in window w are
 // [wstart[w], wstart[w+1]) (rel is increasing), so no search is needed; a
 // wavefront's LDS operations complete in order, so no barrier is needed.
+// Source s of the column keeps its addresses in lane s % 64 (loaded once per
+// column), every lane fetches its source's window bounds at once, and the
+// rows of one source are read four 64-row chunks at a time.
 template <typename T>
 __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& F, const EaTask& e, T* acc,
                           int lane) {
@@ -218,18 +221,58 @@ __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& 
     const bool to_u = pc >= f.nn;
     T* dst = to_u ? F.U + f.uoff + (int64_t)(pc - f.nn) * f.na - f.nn
                   : F.panel + f.poff + (int64_t)pc * nj;  // dst[p]: parent row p
+    const int ns = e.src1 - e.src0;
+    // per-source addresses (sources sb .. sb + 63 in lanes)
+    int64_t my_u = 0;
+    int my_rel = 0, my_ws = 0, my_tcol = 0;
+    auto load_meta = [&](int sb) {
+        if (sb + lane < ns) {
+            const int sidx = e.src0 + sb + lane;
+            const int K = P.ea_src[2 * sidx], tcol = P.ea_src[2 * sidx + 1];
+            const Front kf = P.fronts[K];
+            my_u = kf.uoff + (int64_t)tcol * kf.na;
+            my_rel = kf.reloff;
+            my_ws = P.wsoff[K];
+            my_tcol = tcol;
+        }
+    };
+    if (ns <= 64) load_meta(0);
     for (int w0 = (e.row0 / EA_WIN) * EA_WIN; w0 < e.row1; w0 += EA_WIN) {
         const int w = w0 / EA_WIN;
 #pragma unroll
         for (int j = 0; j < EA_WIN / 64; ++j) acc[lane + 64 * j] = (T)0;
-        for (int s = e.src0; s < e.src1; ++s) {
-            const int K = P.ea_src[2 * s], tcol = P.ea_src[2 * s + 1];
-            const Front kf = P.fronts[K];
-            const int32_t* ws = P.wstart + P.wsoff[K];
-            const int i0 = max(ws[w], tcol), i1 = ws[w + 1];
-            const int32_t* relK = P.rel + kf.reloff;
-            const T* UK = F.U + kf.uoff + (int64_t)tcol * kf.na;
-            for (int i = i0 + lane; i < i1; i += 64) acc[relK[i] - w0] += UK[i];
+        for (int sb = 0; sb < ns; sb += 64) {
+            if (ns > 64) load_meta(sb);
+            const int nsb = min(64, ns - sb);
+            int my_i0 = 0, my_i1 = 0;
+            if (lane < nsb) {
+                const int32_t* ws = P.wstart + my_ws;
+                my_i0 = max(ws[w], my_tcol);
+                my_i1 = ws[w + 1];
+            }
+            for (int q = 0; q < nsb; ++q) {
+                const int i0 = __builtin_amdgcn_readlane(my_i0, q), i1 = __builtin_amdgcn_readlane(my_i1, q);
+                if (i0 >= i1) continue;  // uniform
+                const int relo = __builtin_amdgcn_readlane(my_rel, q);
+                const unsigned ulo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(uint64_t)my_u, q);
+                const unsigned uhi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)my_u >> 32), q);
+                const int64_t uo = (int64_t)(((uint64_t)uhi << 32) | ulo);
+                const int32_t* relK = P.rel + relo;
+                const T* UK = F.U + uo;
+                for (int i = i0 + lane; i < i1; i += 256) {
+                    int rr[4];
+                    T vv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int ii = i + 64 * j;
+                        rr[j] = ii < i1 ? relK[ii] - w0 : -1;
+                        vv[j] = ii < i1 ? UK[ii] : (T)0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (rr[j] >= 0) acc[rr[j]] += vv[j];
+                }
+            }
         }
 #pragma unroll
         for (int j = 0; j < EA_WIN / 64; ++j) {
@@ -240,7 +283,6 @@ __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& 
             }
         }
     }
-    (void)nj;
 }
 
 // Diagonal tile of a big front: factors the kb x kb (kb <= 64) block at local
@@ -828,8 +870,14 @@ __device__ __forceinline__ bool dag_dep(const DevPlan& P, const DevFactor<T>& F,
     }
 }
 
-constexpr int WIN = 4;    // tickets a workgroup holds at once
-constexpr int WDEP = 16;  // lanes of wave 0 per held ticket (one dependency each)
+// Tickets a workgroup holds at once, and lanes of wave 0 per held ticket
+// (one dependency each).  Holding more than one ticket lets a workgroup run a
+// later ready ticket first, but every held ticket then waits for its holder:
+// measured on the 64^3 7-point Laplacian the launch took 36.4 ms with one
+// ticket, 39.2 ms with two, 41.5 ms with four and 44.3 ms with eight.
+constexpr int WIN = 1;
+constexpr int WDEP = 64;
+constexpr unsigned long long WMASK = WDEP == 64 ? ~0ull : (1ull << (WDEP & 63)) - 1;
 
 // Persistent tile-DAG factorization of the big region.  Tickets are handed
 // out in the planner's order (a list schedule, critical path first); a task
@@ -843,10 +891,10 @@ constexpr int WDEP = 16;  // lanes of wave 0 per held ticket (one dependency eac
 //   D.eacnt[J]   finished extend-add tasks of J (1 for a small front)
 //   D.cnt[toff_J + K*R_J + I]  panel columns applied to tile (I, K); K+1
 //                once the tile is final
-// Synchronization per task: lanes of wave 0 poll one dependency each with
-// relaxed loads (16 lanes per held ticket), then thread 0 issues one
-// agent-scope acquire; at the end every wave drains its stores, then thread
-// 0 publishes with an agent-scope release.
+// Synchronization per task: lanes of wave 0 poll the dependencies of the
+// held tickets with relaxed loads (WDEP lanes per ticket), then thread 0
+// issues one agent-scope acquire; at the end every wave drains its stores,
+// then thread 0 publishes with an agent-scope release.
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDag D) {
     constexpr int S1 = PTILE, S2 = 2 * KC * LDT, S3 = 2 * KW * LDW;
@@ -874,7 +922,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
         int tid = (int)threadIdx.x;
         asm volatile("" : "+v"(tid));
         if (tid < 64) {
-            const int slot = tid >> 4, q = tid & (WDEP - 1);
+            const int slot = tid / WDEP, q = tid & (WDEP - 1);
             int sel = -1;
             for (int spins = 0;; ++spins) {
                 const int t = s_win[slot];
@@ -906,7 +954,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                         const int ti = s_win[i];
                         if (ti < 0) continue;
                         any = true;
-                        if (((okm >> (WDEP * i)) & 0xffffull) == 0xffffull && ti < bt) {
+                        if (((okm >> (WDEP * i)) & WMASK) == WMASK && ti < bt) {
                             bt = ti;
                             best = i;
                         }
