@@ -521,17 +521,27 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 }
                 // trailing updates grouped by UPD_GROUP panel steps: tile
                 // (I, Jc) receives the columns [K0, K1) in one product once
-                // block column K1 - 1 is final; the group ending right before
-                // Jc (the look-ahead column) is issued first
-                const int32_t K1 = K + 1, K0 = (K / UPD_GROUP) * UPD_GROUP;
-                for (int pass = 0; pass < 2; ++pass)
+                // block column K1 - 1 is final.  The last step before Jc (the
+                // look-ahead column, which carries the next diagonal tile) is
+                // a product of its own, so the critical chain POTRF -> TRSM ->
+                // update -> POTRF runs a single-step update; the steps before
+                // it go in a partial group one step earlier.  Per step K:
+                //   pass 0: Jc = K + 1, columns [K, K + 1)
+                //   pass 1: Jc = K + 2, columns [K0, K + 1) (partial group)
+                //   pass 2: Jc >= K + 2, columns [K0, K + 1) (complete group)
+                const int32_t K0g = (K / UPD_GROUP) * UPD_GROUP;
+                const bool full = (K + 1) % UPD_GROUP == 0;
+                for (int pass = 0; pass < 3; ++pass)
                     for (int32_t J : L.big_list) {
                         const Front& F = P.fronts[J];
                         const DagFront& D = P.dagf[J];
                         if (K >= D.p) continue;
+                        if (pass == 1 && full) continue;   // a complete group: pass 2
+                        if (pass == 2 && !full) continue;  // group not complete yet
+                        const int32_t K1 = K + 1, K0 = pass == 0 ? K : K0g;
                         const int32_t nj = F.nn + F.na, k0 = K0 * NB, kk = (K1 - K0) * NB;
-                        int32_t jc0 = pass == 0 ? K + 1 : K + 2, jc1 = pass == 0 ? std::min(K + 2, D.p) : D.p;
-                        if (pass == 1 && K1 % UPD_GROUP != 0) continue;  // group not complete yet
+                        const int32_t jc0 = pass == 0 ? K + 1 : K + 2;
+                        const int32_t jc1 = pass == 2 ? D.p : std::min(jc0 + 1, D.p);
                         for (int32_t Jc = jc0; Jc < jc1; ++Jc)
                             for (int32_t I = Jc; I < D.R; ++I) {
                                 P.dag.push_back(DagTask{DT_UPD, J, (int32_t)P.gemm.size(), I, Jc, K0 | (K1 << 16)});
