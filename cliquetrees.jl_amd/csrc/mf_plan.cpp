@@ -578,62 +578,171 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     }
 
     // ---- persistent solve task lists ------------------------------------------
-    // Forward: children before parents; inside a big level the diagonal-block
-    // chains of all fronts are interleaved by block, then the update rows.
-    // Backward: the reverse topological order, chains by decreasing block.
+    // Each direction is one ticket list.  Forward tickets are ordered by a
+    // host list schedule (SIM_WORKERS workgroups, longest remaining path
+    // first), so that a parent front's tasks are handed out as soon as its
+    // children finish instead of after every task of the children's level:
+    // small front = 1 task; big front = diagonal blocks 0..p-1 (a chain), then
+    // update-row tasks (they need every block); every task of a front follows
+    // every task of its children.  Backward: small front = 1 task; big front
+    // = diagonal blocks p-1..0 (a chain) after every task of its parent.
     P.ftask.clear();
     P.btask.clear();
     P.ntf.assign(NJ, 0);
     P.ntb.assign(NJ, 0);
-    for (int32_t J : P.small_order) {
-        P.ftask.push_back(PTask{PT_SMALL, J, 0, 0});
-        P.ntf[J] = 1;
-    }
-    auto nblk = [&](int32_t J) { return (P.fronts[J].nn + NB - 1) / NB; };
-    for (int32_t l = 0; l < nlev; ++l) {
-        const Level& L = P.levels[l];
-        for (int32_t J : L.small_list) {
-            P.ftask.push_back(PTask{PT_SMALL, J, 0, 0});
-            P.ntf[J] = 1;
-        }
-        int32_t maxp = 0;
-        for (int32_t J : L.big_list) maxp = std::max(maxp, nblk(J));
-        for (int32_t k = 0; k < maxp; ++k)
-            for (int32_t J : L.big_list)
-                if (k < nblk(J)) {
-                    const Front& F = P.fronts[J];
-                    P.ftask.push_back(PTask{PT_DIAG, J, k * NB, std::min(NB, F.nn - k * NB)});
-                    P.ntf[J]++;
+    {
+        auto nblk = [&](int32_t J) { return (P.fronts[J].nn + NB - 1) / NB; };
+        auto nrow = [&](int32_t J) { return (P.fronts[J].na + NB - 1) / NB; };
+        // estimated task times (us) of the cost model
+        const double c_small = 7.5, c_diag = 12.0;
+        auto c_rows = [&](int32_t J) { return 3.0 + 1.5 * nblk(J); };
+        std::vector<std::vector<int32_t>> kids(NJ);
+        for (int32_t J = 0; J < NJ; ++J)
+            if (P.fronts[J].parent >= 0) kids[P.fronts[J].parent].push_back(J);
+        // list schedule of tasks given as (front, step); step -1 = small front
+        struct SimTask {
+            int32_t J, step;
+            double dur, bl;
+        };
+        auto schedule = [&](std::vector<SimTask>& tasks, std::vector<int32_t>& indeg,
+                            const std::function<void(int32_t, std::vector<int32_t>&)>& succ) {
+            std::vector<int32_t> order;
+            order.reserve(tasks.size());
+            typedef std::pair<double, int32_t> Item;
+            std::priority_queue<Item> ready;                                        // by bl
+            std::priority_queue<Item, std::vector<Item>, std::greater<Item>> run;  // by end
+            for (int32_t t = 0; t < (int32_t)tasks.size(); ++t)
+                if (indeg[t] == 0) ready.push({tasks[t].bl, t});
+            double now = 0;
+            int busy = 0;
+            std::vector<int32_t> out;
+            while (!ready.empty() || !run.empty()) {
+                while (busy < SIM_WORKERS && !ready.empty()) {
+                    const int32_t t = ready.top().second;
+                    ready.pop();
+                    order.push_back(t);
+                    run.push({now + tasks[t].dur, t});
+                    ++busy;
                 }
-        for (int32_t J : L.big_list) {
-            const Front& F = P.fronts[J];
-            for (int32_t r = F.nn; r < F.nn + F.na; r += NB) {
-                P.ftask.push_back(PTask{PT_ROWS, J, r, std::min(F.nn + F.na, r + NB)});
-                P.ntf[J]++;
+                if (run.empty()) break;
+                now = run.top().first;
+                const int32_t t = run.top().second;
+                run.pop();
+                --busy;
+                out.clear();
+                succ(t, out);
+                for (int32_t u : out)
+                    if (--indeg[u] == 0) ready.push({tasks[u].bl, u});
+            }
+            return order;
+        };
+        // forward
+        {
+            std::vector<int32_t> first(NJ), count(NJ);
+            std::vector<SimTask> tasks;
+            for (int32_t J = 0; J < NJ; ++J) {
+                first[J] = (int32_t)tasks.size();
+                if (P.is_small[J]) {
+                    tasks.push_back({J, -1, c_small, 0});
+                } else {
+                    for (int32_t k = 0; k < nblk(J); ++k) tasks.push_back({J, k, c_diag, 0});
+                    for (int32_t r = 0; r < nrow(J); ++r) tasks.push_back({J, nblk(J) + r, c_rows(J), 0});
+                }
+                count[J] = (int32_t)tasks.size() - first[J];
+            }
+            // entry = first task (small front or block 0); bottom levels root first
+            std::vector<double> entry_bl(NJ, 0);
+            for (int32_t J = NJ - 1; J >= 0; --J) {
+                const int32_t par = P.fronts[J].parent;
+                const double pb = par >= 0 ? entry_bl[par] : 0.0;
+                if (P.is_small[J]) {
+                    tasks[first[J]].bl = c_small + pb;
+                } else {
+                    const int32_t pk = nblk(J), nr = nrow(J);
+                    double rows_bl = 0;
+                    for (int32_t r = 0; r < nr; ++r) {
+                        SimTask& x = tasks[first[J] + pk + r];
+                        x.bl = x.dur + pb;
+                        rows_bl = std::max(rows_bl, x.bl);
+                    }
+                    double nxt = std::max(pb, rows_bl);
+                    for (int32_t k = pk - 1; k >= 0; --k) {
+                        SimTask& x = tasks[first[J] + k];
+                        x.bl = x.dur + nxt;
+                        nxt = std::max(pb, x.bl);
+                    }
+                }
+                entry_bl[J] = tasks[first[J]].bl;
+            }
+            std::vector<int32_t> indeg(tasks.size(), 0);
+            std::vector<int32_t> left(NJ, 0);  // unfinished tasks of the children
+            for (int32_t J = 0; J < NJ; ++J) {
+                for (int32_t K : kids[J]) left[J] += count[K];
+                if (P.is_small[J]) {
+                    indeg[first[J]] = left[J] > 0 ? 1 : 0;
+                } else {
+                    const int32_t pk = nblk(J);
+                    indeg[first[J]] = left[J] > 0 ? 1 : 0;
+                    for (int32_t k = 1; k < pk; ++k) indeg[first[J] + k] = 1;
+                    for (int32_t r = 0; r < nrow(J); ++r) indeg[first[J] + pk + r] = 1;
+                }
+            }
+            auto succ = [&](int32_t t, std::vector<int32_t>& out) {
+                const SimTask& x = tasks[t];
+                const int32_t J = x.J;
+                if (!P.is_small[J]) {
+                    const int32_t pk = nblk(J);
+                    if (x.step < pk - 1) out.push_back(first[J] + x.step + 1);
+                    if (x.step == pk - 1)
+                        for (int32_t r = 0; r < nrow(J); ++r) out.push_back(first[J] + pk + r);
+                }
+                const int32_t par = P.fronts[J].parent;
+                if (par >= 0 && --left[par] == 0) out.push_back(first[par]);
+            };
+            const std::vector<int32_t> order = schedule(tasks, indeg, succ);
+            if (order.size() != tasks.size()) return "solve schedule (forward) incomplete";
+            for (int32_t t : order) {
+                const SimTask& x = tasks[t];
+                const Front& F = P.fronts[x.J];
+                const int32_t pk = P.is_small[x.J] ? 0 : nblk(x.J);
+                if (x.step < 0) P.ftask.push_back(PTask{PT_SMALL, x.J, 0, 0});
+                else if (x.step < pk)
+                    P.ftask.push_back(PTask{PT_DIAG, x.J, x.step * NB, std::min(NB, F.nn - x.step * NB)});
+                else {
+                    const int32_t r0 = F.nn + (x.step - pk) * NB;
+                    P.ftask.push_back(PTask{PT_ROWS, x.J, r0, std::min(F.nn + F.na, r0 + NB)});
+                }
+                P.ntf[x.J]++;
             }
         }
-    }
-    for (int32_t l = nlev - 1; l >= 0; --l) {
-        const Level& L = P.levels[l];
-        for (int32_t J : L.small_list) {
-            P.btask.push_back(PTask{PT_SMALL, J, 0, 0});
-            P.ntb[J] = 1;
-        }
-        int32_t maxp = 0;
-        for (int32_t J : L.big_list) maxp = std::max(maxp, nblk(J));
-        for (int32_t t = 0; t < maxp; ++t)
-            for (int32_t J : L.big_list) {
-                int32_t p = nblk(J), k = p - 1 - t;
-                if (k >= 0) {
-                    const Front& F = P.fronts[J];
-                    P.btask.push_back(PTask{PT_DIAG, J, k * NB, std::min(NB, F.nn - k * NB)});
-                    P.ntb[J]++;
-                }
+        // backward: level order (levels from the root down; chains of a level
+        // interleaved by block, last block first; then the small region).  A
+        // list schedule like the forward one was measured slower here: the
+        // backward tasks of a level all become ready together when the level
+        // above finishes, and a level-ordered ticket list matches that.
+        int32_t nlev_b = (int32_t)P.levels.size();
+        for (int32_t l = nlev_b - 1; l >= 0; --l) {
+            const Level& L = P.levels[l];
+            for (int32_t J : L.small_list) {
+                P.btask.push_back(PTask{PT_SMALL, J, 0, 0});
+                P.ntb[J] = 1;
             }
-    }
-    for (auto it = P.small_order.rbegin(); it != P.small_order.rend(); ++it) {
-        P.btask.push_back(PTask{PT_SMALL, *it, 0, 0});
-        P.ntb[*it] = 1;
+            int32_t maxp = 0;
+            for (int32_t J : L.big_list) maxp = std::max(maxp, nblk(J));
+            for (int32_t t = 0; t < maxp; ++t)
+                for (int32_t J : L.big_list) {
+                    int32_t pk = nblk(J), k = pk - 1 - t;
+                    if (k >= 0) {
+                        const Front& F = P.fronts[J];
+                        P.btask.push_back(PTask{PT_DIAG, J, k * NB, std::min(NB, F.nn - k * NB)});
+                        P.ntb[J]++;
+                    }
+                }
+        }
+        for (auto it = P.small_order.rbegin(); it != P.small_order.rend(); ++it) {
+            P.btask.push_back(PTask{PT_SMALL, *it, 0, 0});
+            P.ntb[*it] = 1;
+        }
     }
 
     // ---- bounds validation of every task against the pools ----------------------
@@ -743,6 +852,24 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             for (int32_t J = 0; J < NJ; ++J)
                 if (P.fronts[J].parent >= 0 && last[P.fronts[J].parent] >= first[J])
                     return "backward solve order";
+            // inside a front: the diagonal chain in its direction, update rows
+            // after the last diagonal block
+            std::vector<int32_t> nextk(NJ, 0), seen_rows(NJ, 0);
+            for (const PTask& t : P.ftask) {
+                if (t.kind == PT_DIAG) {
+                    if (t.a / NB != nextk[t.front] || seen_rows[t.front]) return "forward chain order";
+                    nextk[t.front]++;
+                } else if (t.kind == PT_ROWS) {
+                    if (nextk[t.front] * NB < P.fronts[t.front].nn) return "forward rows before the chain";
+                    seen_rows[t.front] = 1;
+                }
+            }
+            for (int32_t J = 0; J < NJ; ++J) nextk[J] = (P.fronts[J].nn + NB - 1) / NB - 1;
+            for (const PTask& t : P.btask)
+                if (t.kind == PT_DIAG) {
+                    if (t.a / NB != nextk[t.front]) return "backward chain order";
+                    nextk[t.front]--;
+                }
         }
     }
 

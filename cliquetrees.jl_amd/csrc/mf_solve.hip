@@ -74,6 +74,24 @@ __device__ __forceinline__ void wait_geq(int32_t* p, int32_t target, int32_t* ha
     }
 }
 
+// Same, returning the value seen (>= target unless the wait gave up), so a
+// caller that runs behind the counter does not poll again for every step.
+__device__ __forceinline__ int wait_val(int32_t* p, int32_t target, int32_t* hang) {
+    int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while (v < target) {
+        if (++spins > (1 << 24)) {
+            atomicExch(hang, 1);
+            break;
+        }
+        if ((spins & 255) == 0 && __hip_atomic_load(hang, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            break;
+        __builtin_amdgcn_s_sleep(2);
+        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return v;
+}
+
 struct Sync {
     int32_t* done;    // per front: finished tasks
     int32_t* prog;    // per front: finished diagonal blocks (in chain order)
@@ -153,14 +171,35 @@ __device__ __forceinline__ int acc_row(int v) {
     return 16 * (threadIdx.x >> 6) + Mfma<T>::row((threadIdx.x & 63) >> 4, v);
 }
 
-// Xs(r, c) = B(row0 + r, cb + c) for r < rows, c < nb (device-coherent loads), else 0.
+// A 64-row block of right-hand sides in registers, thread (r, g) holding
+// B(row0 + r, cb + g + 4i) (device-coherent loads; 0 outside rows x nb), and
+// its copy into an LDS tile Xs (leading dimension LDX).  Fetching the next
+// block before the current product hides its latency.
 template <typename T>
-__device__ __forceinline__ void stage_rows(const T* B, int64_t ldb, int row0, int rows, int cb,
-                                           int nb, T* Xs) {
+__device__ __forceinline__ void fetch_rows(const T* B, int64_t ldb, int row0, int rows, int cb, int nb,
+                                           T (&x)[RB / 4]) {
     const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
 #pragma unroll
-    for (int c = g; c < RB; c += 4)
-        Xs[r + c * LDX] = (r < rows && c < nb) ? ldc(B + row0 + r + (int64_t)(cb + c) * ldb) : (T)0;
+    for (int i = 0; i < RB / 4; ++i) {
+        const int c = g + 4 * i;
+        x[i] = (r < rows && c < nb) ? ldc(B + row0 + r + (int64_t)(cb + c) * ldb) : (T)0;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void fetch_gather(const T* B, int64_t ldb, const int32_t* idx, int rows, int cb,
+                                             int nb, T (&x)[RB / 4]) {
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < RB / 4; ++i) {
+        const int c = g + 4 * i;
+        x[i] = (r < rows && c < nb) ? ldc(B + idx[r] + (int64_t)(cb + c) * ldb) : (T)0;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void put_rows(const T (&x)[RB / 4], T* Xs) {
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < RB / 4; ++i) Xs[r + (g + 4 * i) * LDX] = x[i];
 }
 
 // X_k = W acc for the block's rows < kb: acc goes through LDS (Ys) as the B
@@ -270,7 +309,7 @@ __device__ void small_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolv
 // counter releases them, then X_k = W_k acc.
 template <typename T>
 __device__ void diag_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
-                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* X, T* Y) {
+                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* X, T* Y, int* s_av) {
     typedef typename Mfma<T>::v4 v4;
     const int tid = threadIdx.x, li = tid & 15;
     const int nj = f.nn + f.na;
@@ -291,10 +330,22 @@ __device__ void diag_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
                          ? S.B[f.c0 + k0 + row + (int64_t)(cb + li) * S.ldb] + Y[row + li * LDX]
                          : (T)0;
         }
+        // thread 0 keeps the last progress it saw and polls only when it has
+        // caught up; a block already published is fetched one step ahead
+        int avail = 0;
+        bool have = false;
+        T xr[RB / 4];
         for (int ib = 0; ib < kblk; ++ib) {
-            if (tid == 0) wait_geq(Y_.prog + J, ib + 1, F.hang);
+            if (tid == 0) {
+                if (ib >= avail) avail = wait_val(Y_.prog + J, ib + 1, F.hang);
+                *s_av = avail;
+            }
             __syncthreads();
-            stage_rows(S.B, S.ldb, f.c0 + ib * NB, NB, cb, nb, X);
+            const int av = *s_av;
+            if (!have) fetch_rows(S.B, S.ldb, f.c0 + ib * NB, NB, cb, nb, xr);
+            put_rows(xr, X);
+            have = ib + 1 < kblk && ib + 1 < av;
+            if (have) fetch_rows(S.B, S.ldb, f.c0 + (ib + 1) * NB, NB, cb, nb, xr);
             __syncthreads();
             frag_mma(la, X, acc);
             if (ib + 1 < kblk) load_frag(Pj + k0 + (int64_t)(ib + 1) * NB * nj, 1, nj, kb, NB, (T)-1, la);
@@ -312,7 +363,7 @@ __device__ void diag_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
 // Forward update rows [r0, r1) (>= nn) of a big front: M2 = gather - L21 X.
 template <typename T>
 __device__ void rows_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
-                         const Front& f, int J, int r0, int r1, const Sync& Y_, T* X, T* Y) {
+                         const Front& f, int J, int r0, int r1, const Sync& Y_, T* X, T* Y, int* s_av) {
     typedef typename Mfma<T>::v4 v4;
     const int tid = threadIdx.x, li = tid & 15;
     const int nn = f.nn, na = f.na, nj = nn + na, m = r1 - r0;
@@ -330,11 +381,21 @@ __device__ void rows_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
             const int row = acc_row<T>(v);
             acc[v] = (row < m && li < nb) ? Y[row + li * LDX] : (T)0;
         }
+        int avail = 0;
+        bool have = false;
+        T xr[RB / 4];
         for (int ib = 0; ib < p; ++ib) {
             const int kbi = min(NB, nn - ib * NB);
-            if (tid == 0) wait_geq(Y_.prog + J, ib + 1, F.hang);
+            if (tid == 0) {
+                if (ib >= avail) avail = wait_val(Y_.prog + J, ib + 1, F.hang);
+                *s_av = avail;
+            }
             __syncthreads();
-            stage_rows(S.B, S.ldb, f.c0 + ib * NB, kbi, cb, nb, X);
+            const int av = *s_av;
+            if (!have) fetch_rows(S.B, S.ldb, f.c0 + ib * NB, kbi, cb, nb, xr);
+            put_rows(xr, X);
+            have = ib + 1 < p && ib + 1 < av;
+            if (have) fetch_rows(S.B, S.ldb, f.c0 + (ib + 1) * NB, min(NB, nn - (ib + 1) * NB), cb, nb, xr);
             __syncthreads();
             frag_mma(la, X, acc);
             if (ib + 1 < p)
@@ -355,9 +416,9 @@ __device__ void rows_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
 // progress counter releases them, last block first), then X_k = W_k' acc.
 template <typename T>
 __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
-                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* X, T* Y) {
+                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* X, T* Y, int* s_av) {
     typedef typename Mfma<T>::v4 v4;
-    const int tid = threadIdx.x, r = tid & 63, g = tid >> 6, li = tid & 15;
+    const int tid = threadIdx.x, li = tid & 15;
     const int nn = f.nn, nj = nn + f.na;
     const T* Pj = F.panel + f.poff;
     const T* W = F.W + f.woff;
@@ -386,20 +447,32 @@ __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
             const int row = acc_row<T>(v);
             acc[v] = (row < kb && li < nb) ? S.B[f.c0 + k0 + row + (int64_t)(cb + li) * S.ldb] : (T)0;
         }
-        for (int q = 0; q < nstep; ++q) {
+        // right-hand sides of step q: separator rows (final), then the
+        // blocks below the diagonal block as the progress counter releases them
+        auto fetch_q = [&](int q, T (&x)[RB / 4]) {
             if (q < nsep) {
-                const int rt = nn + q * NB, nr = min(NB, nj - rt);
-                __syncthreads();
-#pragma unroll
-                for (int c = g; c < RB; c += 4)
-                    X[r + c * LDX] = (r < nr && c < nb)
-                                         ? ldc(S.B + rows[rt + r] + (int64_t)(cb + c) * S.ldb) : (T)0;
+                const int rt = nn + q * NB;
+                fetch_gather(S.B, S.ldb, rows + rt, min(NB, nj - rt), cb, nb, x);
             } else {
                 const int ib = p - 1 - (q - nsep);
-                if (tid == 0) wait_geq(Y_.prog + J, p - ib, F.hang);
-                __syncthreads();
-                stage_rows(S.B, S.ldb, f.c0 + ib * NB, min(NB, nn - ib * NB), cb, nb, X);
+                fetch_rows(S.B, S.ldb, f.c0 + ib * NB, min(NB, nn - ib * NB), cb, nb, x);
             }
+        };
+        int avail = 0;
+        bool have = false;
+        T xr[RB / 4];
+        for (int q = 0; q < nstep; ++q) {
+            if (tid == 0) {
+                const int need = q < nsep ? 0 : p - (p - 1 - (q - nsep));
+                if (need > avail) avail = wait_val(Y_.prog + J, need, F.hang);
+                *s_av = avail;
+            }
+            __syncthreads();
+            const int av = *s_av;
+            if (!have) fetch_q(q, xr);
+            put_rows(xr, X);
+            have = q + 1 < nstep && (q + 1 < nsep || q + 2 - nsep <= av);
+            if (have) fetch_q(q + 1, xr);
             __syncthreads();
             frag_mma(la, X, acc);
             if (q + 1 < nstep) frag(q + 1, la);
@@ -420,7 +493,7 @@ __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, De
     __shared__ T Ps[SMALL_NJ * LDS];
     __shared__ T X[LDS * RB];
     __shared__ T Y[LDX * RB];
-    __shared__ int s_t;
+    __shared__ int s_t, s_av;
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) s_t = (int)atomicAdd(Y_.ticket, 1u);
@@ -440,8 +513,8 @@ __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, De
         }
         __syncthreads();
         if (tk.kind == PT_SMALL) small_fwd(P, F, S, f, J, Ps, X);
-        else if (tk.kind == PT_DIAG) diag_fwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y);
-        else rows_fwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y);
+        else if (tk.kind == PT_DIAG) diag_fwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y, &s_av);
+        else rows_fwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y, &s_av);
         drain_stores();
         __syncthreads();
         if (tid == 0) {
@@ -465,7 +538,7 @@ __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, De
     __shared__ T Ps[SMALL_NJ * LDS];
     __shared__ T X[LDS * RB];
     __shared__ T Y[LDX * RB];
-    __shared__ int s_t;
+    __shared__ int s_t, s_av;
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) s_t = (int)atomicAdd(Y_.ticket, 1u);
@@ -482,7 +555,7 @@ __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, De
         }
         __syncthreads();
         if (tk.kind == PT_SMALL) small_bwd(P, F, S, f, Ps, X);
-        else diag_bwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y);
+        else diag_bwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y, &s_av);
         drain_stores();
         __syncthreads();
         if (tid == 0) {
