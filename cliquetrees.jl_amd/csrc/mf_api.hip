@@ -116,6 +116,7 @@ struct mf_factor {
     int reg_kind = 0;
     double delta = 0, eps = 0;
     bool factored = false;
+    bool ref_stale = false;    // Dref/Lref/dref not yet copied out of the panels
     bool timing = false;
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> evpool;
@@ -236,9 +237,25 @@ void enqueue_factor(mf_factor* f) {
                  (int32_t)H.dag.size()};
         timed(f, MF_KC_DAG, H.engine_flops - pl->small_flops, 1, [&] { launch_dag<T>(P, F, D, s); });
     }
+}
+
+// The reference-layout Dval/Lval/d are written from the panels only when a
+// caller reads them (the solves and the refinement use the panels), so a
+// factor + solve step moves no bytes for them.
+template <typename T>
+void enqueue_copyout(mf_factor* f) {
+    const HostPlan& H = f->plan->H;
+    DevFactor<T> F = dev_factor<T>(f);
     timed(f, MF_KC_COPYOUT, 8.0 * 2.0 * (double)(H.ndz + H.nlz), H.nfr, [&] {
-        launch_copyout<T>(P, F, f->Dref, f->Lref, f->dref, (int)H.nfr, (int)H.n, s);
+        launch_copyout<T>(f->plan->D, F, f->Dref, f->Lref, f->dref, (int)H.nfr, (int)H.n, f->stream);
     });
+}
+
+void sync_ref(mf_factor* f) {
+    if (!f->ref_stale) return;
+    if (f->precision == MF_FP32) enqueue_copyout<float>(f);
+    else enqueue_copyout<double>(f);
+    f->ref_stale = false;
 }
 
 template <typename T>
@@ -454,6 +471,7 @@ int mf_factor_set_values(mf_factor* f, const double* Dval, const double* Lval, i
     HIPCHK(hipGetLastError());
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     f->factored = false;
+    f->ref_stale = false;
     return MF_OK;
 }
 
@@ -534,6 +552,7 @@ int mf_factor_refresh(mf_factor* f, const double* nzval, int device) {
     HIPCHK(hipGetLastError());
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     f->factored = false;
+    f->ref_stale = false;
     return MF_OK;
 }
 
@@ -559,6 +578,7 @@ int mf_factorize_async(mf_factor* f, int diag, const double* signs_dev, int reg_
     f->signs = keep;
     HIPCHK(hipGetLastError());
     f->factored = true;
+    f->ref_stale = true;
     return MF_OK;
 }
 
@@ -584,6 +604,8 @@ int mf_factorize(mf_factor* f, int diag, const double* signs, int reg_kind, doub
 int mf_factor_get(mf_factor* f, double* Dval, double* Lval, double* d, int device) {
     if (!f) return fail(MF_EARG, "null factor");
     const HostPlan& H = f->plan->H;
+    sync_ref(f);
+    HIPCHK(hipGetLastError());
     hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     if (Dval && H.ndz) HIPCHK(hipMemcpyAsync(Dval, f->Dref, 8 * H.ndz, k, f->stream));
     if (Lval && H.nlz) HIPCHK(hipMemcpyAsync(Lval, f->Lref, 8 * H.nlz, k, f->stream));
