@@ -227,7 +227,7 @@ void enqueue_factor(mf_factor* f) {
     launch_fill<int32_t>(f->flags, std::max<int>(P.nJ, 1), 0, s);
     launch_fill<uint32_t>(f->ticket, 1, 0u, s);
     timed(f, MF_KC_SMALL, pl->small_flops, H.n_small_region, [&] {
-        launch_small<T>(P, F, P.small_order, (int)H.n_small_region, true, s);
+        launch_small<T>(P, F, P.small_order, P.small_units, (int)H.small_units.size(), s);
     });
     if (!H.dag.empty()) {
         launch_fill<int32_t>(f->cnt, H.dag_counters, 0, s);
@@ -321,6 +321,7 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
         (rc = upload(H.chd, (int32_t**)&D.chd, pl->allocs)) ||
         (rc = upload(H.rel, (int32_t**)&D.rel, pl->allocs)) ||
         (rc = upload(H.small_order, (int32_t**)&D.small_order, pl->allocs)) ||
+        (rc = upload(H.small_units, (int32_t**)&D.small_units, pl->allocs)) ||
         (rc = upload(H.level_small, (int32_t**)&D.level_small, pl->allocs)) ||
         (rc = upload(H.level_big, (int32_t**)&D.level_big, pl->allocs)) ||
         (rc = upload(H.ea, (EaTask**)&D.ea, pl->allocs)) ||

@@ -85,6 +85,18 @@ __device__ __forceinline__ int ld_rel(const int32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Update matrices of small fronts are written and read with device-coherent
+// (sc1) accesses: a small front then publishes with its stores drained and a
+// flag, and its parent reads U without a cache-wide invalidate.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void stc(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 constexpr int LDS = SMALL_NJ + 1;  // leading dimension of LDS fronts and tiles
 constexpr int KC = 32;             // k-chunk of the GEMM tile
 constexpr int LDT = 64 + 4;        // leading dimension of the GEMM LDS stages
@@ -146,7 +158,7 @@ __device__ __forceinline__ void small_front(const DevPlan& P, const DevFactor<T>
         const T* UK = F.U + kf.uoff;
         if (r < ma) {
             const int rr = relK[r];
-            for (int c = cg; c <= r; c += 4) M[rr + relK[c] * LDS] += UK[r + c * ma];
+            for (int c = cg; c <= r; c += 4) M[rr + relK[c] * LDS] += ldc(UK + r + c * ma);
         }
         __syncthreads();
     }
@@ -159,46 +171,135 @@ __device__ __forceinline__ void small_front(const DevPlan& P, const DevFactor<T>
         for (int c = cg; c <= r && c < nn; c += 4) Pj[r + c * nj] = M[r + c * LDS];
     if (na > 0 && r < na) {
         T* Uj = F.U + f.uoff;
-        for (int c = cg; c <= r; c += 4) Uj[r + c * na] = M[(nn + r) + (nn + c) * LDS];
+        for (int c = cg; c <= r; c += 4) stc(Uj + r + c * na, M[(nn + r) + (nn + c) * LDS]);
     }
 }
 
+constexpr int LDT1 = TINY_NJ + 1;     // leading dimension of a wavefront's LDS front
+
+// One tiny front (nJ <= TINY_NJ) by one wavefront in its own LDS slab Mw:
+// the same extend-add, assembly, right-looking partial factorization and
+// write-out as small_front, lane r keeping row r.  A wavefront's LDS
+// operations complete in order, so no barrier is needed.
+template <typename T>
+__device__ __forceinline__ void wave_front(const DevPlan& P, const DevFactor<T>& F, int J, T* Mw,
+                                           const int lane) {
+    const Front f = P.fronts[J];
+    const int nn = f.nn, na = f.na, nj = nn + na, gc0 = f.c0;
+    const int r = lane;
+    if (r < nj)
+        for (int c = 0; c <= r; ++c) Mw[r + c * LDT1] = (T)0;
+    for (int q = f.chd0; q < f.chd1; ++q) {
+        const Front kf = P.fronts[P.chd[q]];
+        const int ma = kf.na;
+        const int32_t* relK = P.rel + kf.reloff;
+        const T* UK = F.U + kf.uoff;
+        if (r < ma) {
+            const int rr = relK[r];
+            for (int c = 0; c <= r; ++c) Mw[rr + relK[c] * LDT1] += ldc(UK + r + c * ma);
+        }
+    }
+    T* Pj = F.panel + f.poff;
+    if (r < nj)
+        for (int c = 0; c <= r && c < nn; ++c) Mw[r + c * LDT1] += Pj[r + c * nj];
+    for (int k = 0; k < nn; ++k) {
+        const T akk = Mw[k + k * LDT1];
+        double piv = regularize(F, sign_of(F, gc0 + k), (double)akk);
+        if (piv == 0.0) {
+            if (lane == 0) atomicMin(F.info, (unsigned long long)(gc0 + k + 1));
+            return;  // the front stays unfinished; info reports the column
+        }
+        T inv, spv;
+        if (F.diag == 0) {
+            const T rt = (T)sqrt(piv);
+            if (lane == 0) Mw[k + k * LDT1] = rt;
+            inv = (T)1 / rt;
+            spv = (T)1;
+        } else {
+            if (lane == 0) F.d[gc0 + k] = (T)piv;
+            inv = (T)1 / (T)piv;
+            spv = (T)piv;
+        }
+        if (r > k && r < nj) Mw[r + k * LDT1] *= inv;
+        if (r > k && r < nj) {
+            const T lr = Mw[r + k * LDT1] * spv;
+            for (int c = k + 1; c <= r; ++c) Mw[r + c * LDT1] -= lr * Mw[c + k * LDT1];
+        }
+    }
+    if (r < nj)
+        for (int c = 0; c <= r && c < nn; ++c) Pj[r + c * nj] = Mw[r + c * LDT1];
+    if (na > 0 && r < na) {
+        T* Uj = F.U + f.uoff;
+        for (int c = 0; c <= r; ++c) stc(Uj + r + c * na, Mw[(nn + r) + (nn + c) * LDT1]);
+    }
+}
+
+// Bounded wait of one lane until *flag >= target (see wait_ge).
+__device__ __forceinline__ void lane_wait_ge(int32_t* flag, int32_t target, int32_t* hang) {
+    int spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++spins > (1 << 24)) {
+            atomicExch(hang, 1);
+            break;
+        }
+        if ((spins & 255) == 0 && __hip_atomic_load(hang, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Persistent walk of the small region.  Tickets are units of the planner's
+// children-first order: one front of nJ > TINY_NJ (the whole workgroup), or
+// up to four consecutive tiny fronts (one per wavefront, each waiting for
+// its own children; a child in the same unit belongs to another wavefront,
+// which progresses on its own).  Update matrices go through device-coherent
+// accesses, so a front is published by draining its stores and setting its
+// flag; no cache-wide write-back or invalidate.
 template <typename T>
 __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const int32_t* list,
-                                               int count, int persistent) {
-    __shared__ T M[SMALL_NJ * LDS];
+                                               const int32_t* units, int count) {
+    constexpr int MS = SMALL_NJ * LDS > 4 * TINY_NJ * LDT1 ? SMALL_NJ * LDS : 4 * TINY_NJ * LDT1;
+    __shared__ T M[MS];
     __shared__ T s_piv, s_inv;
     __shared__ int s_fail, s_t;
-    const int tid = threadIdx.x;
-    int t = blockIdx.x;
     for (;;) {
-        if (persistent) {
-            if (tid == 0) s_t = (int)atomicAdd(F.ticket, 1u);
-            __syncthreads();
-            t = s_t;
-            __syncthreads();
-        }
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        if (tid == 0) s_t = (int)atomicAdd(F.ticket, 1u);
+        __syncthreads();
+        const int t = s_t;
+        __syncthreads();
         if (t >= count) break;
-        const int J = list[t];
-        if (tid == 0) {
-            s_fail = 0;
-            if (persistent) {
+        const int u = units[t], start = u >> 3, cnt = u & 7;
+        if (cnt == 0) {  // one front, the whole workgroup
+            const int J = list[start];
+            if (tid == 0) {
+                s_fail = 0;
                 const Front f = P.fronts[J];
                 for (int q = f.chd0; q < f.chd1; ++q) wait_ge(F.flags + P.chd[q], 1, F.hang);
-                acquire_agent();
+            }
+            __syncthreads();
+            small_front<T>(P, F, J, M, &s_piv, &s_inv, &s_fail, tid);
+            drain_stores();
+            __syncthreads();
+            if (tid == 0) __hip_atomic_store(F.flags + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const int wave = tid >> 6, lane = tid & 63;
+            if (wave < cnt) {
+                const int J = list[start + wave];
+                const Front f = P.fronts[J];
+                if (lane == 0)
+                    for (int q = f.chd0; q < f.chd1; ++q) lane_wait_ge(F.flags + P.chd[q], 1, F.hang);
+                // the children's data is read after the wait (device-coherent loads)
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                wave_front<T>(P, F, J, M + wave * TINY_NJ * LDT1, lane);
+                drain_stores();
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) __hip_atomic_store(F.flags + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        __syncthreads();
-        small_front<T>(P, F, J, M, &s_piv, &s_inv, &s_fail, tid);
-        drain_stores();
-        __syncthreads();
-        if (!persistent) break;
-        if (tid == 0) {
-            release_agent();
-            __hip_atomic_store(F.flags + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        // closes the thread-0 block before the back-edge: every thread of the
-        // workgroup reaches the next ticket together
+        // closes the ticket before the back-edge: every thread reaches the
+        // next ticket together
         __syncthreads();
     }
 }
@@ -1143,12 +1244,11 @@ void launch_refresh(const DevFactor<T>& F, const int64_t* tp, const int64_t* tr,
 }
 
 template <typename T>
-void launch_small(const DevPlan& P, const DevFactor<T>& F, const int32_t* list, int count,
-                  bool persistent, hipStream_t s) {
+void launch_small(const DevPlan& P, const DevFactor<T>& F, const int32_t* list, const int32_t* units,
+                  int count, hipStream_t s) {
     if (count <= 0) return;
-    int grid = persistent ? (count < 2048 ? count : 2048) : count;
-    hipLaunchKernelGGL(k_small<T>, dim3(grid), dim3(256), 0, s, P, F, list, count,
-                       persistent ? 1 : 0);
+    const int grid = count < 2048 ? count : 2048;
+    hipLaunchKernelGGL(k_small<T>, dim3(grid), dim3(256), 0, s, P, F, list, units, count);
 }
 
 template <typename T>
@@ -1184,8 +1284,8 @@ void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, doubl
     template void launch_refresh<T>(const DevFactor<T>&, const int64_t*, const int64_t*,        \
                                     const double*, int64_t, double*, int64_t, double*,          \
                                     hipStream_t);                                               \
-    template void launch_small<T>(const DevPlan&, const DevFactor<T>&, const int32_t*, int,     \
-                                  bool, hipStream_t);                                           \
+    template void launch_small<T>(const DevPlan&, const DevFactor<T>&, const int32_t*,          \
+                                  const int32_t*, int, hipStream_t);                            \
     template void launch_diag<T>(const DevPlan&, const DevFactor<T>&, int, int, bool,           \
                                  hipStream_t);                                                 \
     template void launch_copyout<T>(const DevPlan&, const DevFactor<T>&, double*, double*,      \

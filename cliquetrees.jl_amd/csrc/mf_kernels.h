@@ -18,6 +18,7 @@ struct DevPlan {
     const int32_t* chd;
     const int32_t* rel;
     const int32_t* small_order;
+    const int32_t* small_units;  // tickets of the small region: start << 3 | tiny count (0: one front)
     const int32_t* level_small;
     const int32_t* level_big;
     const EaTask* ea;
@@ -79,8 +80,8 @@ void launch_refresh(const DevFactor<T>& F, const int64_t* tgt_panel, const int64
                     const double* nzval, int64_t nnz, double* Dref, int64_t ndz, double* Lref,
                     hipStream_t s);
 template <typename T>
-void launch_small(const DevPlan& P, const DevFactor<T>& F, const int32_t* list, int count,
-                  bool persistent, hipStream_t s);
+void launch_small(const DevPlan& P, const DevFactor<T>& F, const int32_t* list, const int32_t* units,
+                  int count, hipStream_t s);
 // invert_only: the block already holds the factor; only W is produced.
 template <typename T>
 void launch_diag(const DevPlan& P, const DevFactor<T>& F, int t0, int t1, bool invert_only,

@@ -259,6 +259,25 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     std::stable_sort(P.small_order.begin(), P.small_order.end(),
                      [&](int32_t a, int32_t b) { return slev[a] < slev[b]; });
     P.n_small_region = (int64_t)P.small_order.size();
+    // tickets of the small region: up to four consecutive fronts of nJ <= TINY_NJ
+    // share a ticket (one wavefront each); larger small fronts take one
+    P.small_units.clear();
+    for (int32_t i = 0; i < (int32_t)P.small_order.size();) {
+        const Front& F = P.fronts[P.small_order[i]];
+        if (F.nn + F.na > TINY_NJ) {
+            P.small_units.push_back(i << 3);
+            ++i;
+            continue;
+        }
+        int32_t c = 1;
+        while (c < 4 && i + c < (int32_t)P.small_order.size()) {
+            const Front& G = P.fronts[P.small_order[i + c]];
+            if (G.nn + G.na > TINY_NJ) break;
+            ++c;
+        }
+        P.small_units.push_back((i << 3) | c);
+        i += c;
+    }
     P.levels.assign(nlev, Level());
     for (int32_t J = 0; J < NJ; ++J) {
         if (in_small_region[J]) continue;

@@ -40,6 +40,7 @@ struct HostPlan {
     int64_t panel_elems = 0, u_elems = 0, w_elems = 0, mv_elems1 = 0;
     // schedule
     std::vector<int32_t> small_order;  // small region, ticket order (children first)
+    std::vector<int32_t> small_units;  // small_order in tickets: start << 3 | n tiny fronts (0: one front)
     std::vector<int32_t> level_small, level_big;
     std::vector<Level> levels;
     std::vector<EaTask> ea;
