@@ -938,12 +938,12 @@ int mf_plan_solve_tasks(const mf_plan* pl, int32_t* tasks, int64_t n) {
     if (!pl || !tasks) return fail(MF_EARG, "null argument");
     const HostPlan& H = pl->H;
     const size_t ns = H.ftask.size() + H.btask.size();
-    if (n < 4 * (int64_t)ns) return fail(MF_EARG, "buffer too small (4 x solve tasks)");
+    if (n < 5 * (int64_t)ns) return fail(MF_EARG, "buffer too small (5 x solve tasks)");
     size_t t = 0;
     for (const std::vector<PTask>* tl : {&H.ftask, &H.btask})
         for (const PTask& k : *tl) {
-            const int32_t v[4] = {k.kind, k.front, k.a, k.b};
-            std::copy(v, v + 4, tasks + 4 * t++);
+            const int32_t v[5] = {k.kind, k.front, k.a, k.b, k.c};
+            std::copy(v, v + 5, tasks + 5 * t++);
         }
     return MF_OK;
 }

@@ -764,6 +764,31 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         }
     }
 
+    // consecutive tickets of tiny fronts share one ticket, one wavefront each
+    // (a front may wait for another front of the same ticket: that one runs
+    // on another wavefront of the workgroup, so it makes progress)
+    for (std::vector<PTask>* tl : {&P.ftask, &P.btask}) {
+        std::vector<PTask> out;
+        out.reserve(tl->size());
+        auto tiny = [&](const PTask& t) {
+            const Front& F = P.fronts[t.front];
+            return t.kind == PT_SMALL && F.nn + F.na <= TINY_NJ;
+        };
+        for (size_t i = 0; i < tl->size();) {
+            if (!tiny((*tl)[i])) {
+                out.push_back((*tl)[i]);
+                out.back().c = 0;
+                ++i;
+                continue;
+            }
+            int32_t fr[4] = {-1, -1, -1, -1};
+            int n = 0;
+            while (n < 4 && i < tl->size() && tiny((*tl)[i])) fr[n++] = (*tl)[i++].front;
+            out.push_back(PTask{PT_TINY, fr[0], fr[1], fr[2], fr[3]});
+        }
+        tl->swap(out);
+    }
+
     // ---- bounds validation of every task against the pools ----------------------
     {
         auto pool = [&](int8_t a) {
@@ -826,20 +851,24 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             }
         }
         for (const std::vector<PTask>* tl : {&P.ftask, &P.btask}) {
-            if ((int64_t)tl->size() != (int64_t)P.small_order.size() + [&] {
-                    int64_t c = 0;
-                    for (const Level& L : P.levels) c += (int64_t)L.small_list.size();
-                    return c;
-                }() + [&] {
-                    int64_t c = 0;
-                    for (const PTask& t : *tl) c += t.kind != PT_SMALL;
-                    return c;
-                }())
-                return "solve task count";
+            int64_t nsmall = (int64_t)P.small_order.size();
+            for (const Level& L : P.levels) nsmall += (int64_t)L.small_list.size();
+            int64_t seen = 0;
+            for (const PTask& t : *tl) {
+                if (t.kind == PT_SMALL) ++seen;
+                if (t.kind == PT_TINY)
+                    for (int32_t J : {t.front, t.a, t.b, t.c}) seen += J >= 0;
+            }
+            if (seen != nsmall) return "solve task count";
             for (const PTask& t : *tl) {
                 const Front& F = P.fronts[t.front];
                 int32_t nj = F.nn + F.na;
-                if (t.kind == PT_SMALL) {
+                if (t.kind == PT_TINY) {
+                    for (int32_t J : {t.front, t.a, t.b, t.c})
+                        if (J >= NJ || (J >= 0 && P.fronts[J].nn + P.fronts[J].na > TINY_NJ))
+                            return "tiny solve task";
+                    if (t.front < 0) return "tiny solve task";
+                } else if (t.kind == PT_SMALL) {
                     if (nj > SMALL_NJ) return "small solve task on a big front";
                 } else if (t.kind == PT_DIAG) {
                     if (F.woff < 0 || t.a < 0 || t.b <= 0 || t.b > NB || t.a + t.b > F.nn || t.a % NB)
@@ -853,21 +882,27 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         }
         // every dependency points to an earlier ticket
         {
-            std::vector<int32_t> first(NJ, -1), last(NJ, -1);
-            for (int32_t i = 0; i < (int32_t)P.ftask.size(); ++i) {
-                int32_t J = P.ftask[i].front;
-                if (first[J] < 0) first[J] = i;
-                last[J] = i;
-            }
+            // positions 4 i + slot (slot = wavefront inside a tiny ticket)
+            std::vector<int64_t> first(NJ, -1), last(NJ, -1);
+            auto mark = [&](const std::vector<PTask>& tl) {
+                std::fill(first.begin(), first.end(), -1);
+                for (int64_t i = 0; i < (int64_t)tl.size(); ++i) {
+                    const PTask& t = tl[i];
+                    const int32_t fr[4] = {t.front, t.kind == PT_TINY ? t.a : -1,
+                                           t.kind == PT_TINY ? t.b : -1, t.kind == PT_TINY ? t.c : -1};
+                    for (int slot = 0; slot < 4; ++slot) {
+                        const int32_t J = fr[slot];
+                        if (J < 0) continue;
+                        if (first[J] < 0) first[J] = 4 * i + slot;
+                        last[J] = 4 * i + slot;
+                    }
+                }
+            };
+            mark(P.ftask);
             for (int32_t J = 0; J < NJ; ++J)
                 for (int32_t q = P.fronts[J].chd0; q < P.fronts[J].chd1; ++q)
                     if (last[P.chd[q]] >= first[J]) return "forward solve order";
-            std::fill(first.begin(), first.end(), -1);
-            for (int32_t i = 0; i < (int32_t)P.btask.size(); ++i) {
-                int32_t J = P.btask[i].front;
-                if (first[J] < 0) first[J] = i;
-                last[J] = i;
-            }
+            mark(P.btask);
             for (int32_t J = 0; J < NJ; ++J)
                 if (P.fronts[J].parent >= 0 && last[P.fronts[J].parent] >= first[J])
                     return "backward solve order";

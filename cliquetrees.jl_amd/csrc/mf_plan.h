@@ -11,12 +11,15 @@
 namespace mf {
 
 // Task of the persistent triangular solves (one launch per direction).
-enum : int32_t { PT_SMALL = 0, PT_DIAG = 1, PT_ROWS = 2 };
+enum : int32_t { PT_SMALL = 0, PT_DIAG = 1, PT_ROWS = 2, PT_TINY = 3 };
 struct PTask {
     int32_t kind;    // PT_SMALL: a whole small front; PT_DIAG: diagonal block rows
-                     // [a, a+b) of a big front; PT_ROWS: update rows [a, b)
+                     // [a, a+b) of a big front; PT_ROWS: update rows [a, b);
+                     // PT_TINY: up to four fronts of nJ <= TINY_NJ (front, a, b,
+                     // c; -1 when absent), one wavefront each
     int32_t front;
     int32_t a, b;
+    int32_t c;
 };
 
 struct Level {

@@ -304,6 +304,89 @@ __device__ void small_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolv
     }
 }
 
+// Tiny fronts (nJ <= TINY_NJ), one wavefront each: the panel in a private
+// LDS slab Pw (leading dimension LDT1), the right-hand sides of the pass in
+// Xw (leading dimension TINY_NJ); lane r keeps row r.  A wavefront's LDS
+// operations complete in order, so no barrier is needed.
+constexpr int LDT1 = TINY_NJ + 1;
+
+template <typename T>
+__device__ void wave_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int J, T* Pw,
+                         T* Xw) {
+    const int r = threadIdx.x & 63;
+    const Front f = P.fronts[J];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const bool unit = F.diag != 0;
+    const T* Pj = F.panel + f.poff;
+    if (r < nj)
+        for (int c = 0; c < nn; ++c) Pw[r + c * LDT1] = Pj[r + c * nj];
+    T* MvJ = S.Mv + P.mvoff1[J] * S.nrhs;
+    for (int cb = 0; cb < S.nrhs; cb += RB) {
+        const int nb = min(RB, S.nrhs - cb);
+        if (r < TINY_NJ)
+            for (int c = 0; c < RB; ++c) Xw[r + c * TINY_NJ] = (T)0;
+        for (int q = f.chd0; q < f.chd1; ++q) {
+            const int K = P.chd[q];
+            const Front kf = P.fronts[K];
+            const int ma = kf.na;
+            if (r < ma) {
+                const int rr = P.rel[kf.reloff + r];
+                const T* MvK = S.Mv + P.mvoff1[K] * S.nrhs;
+                for (int c = 0; c < nb; ++c) Xw[rr + c * TINY_NJ] += ldc(MvK + r + (int64_t)(cb + c) * ma);
+            }
+        }
+        if (r < nn)
+            for (int c = 0; c < nb; ++c) Xw[r + c * TINY_NJ] += S.B[f.c0 + r + (int64_t)(cb + c) * S.ldb];
+        for (int k = 0; k < nn; ++k) {
+            if (!unit && r < nb) Xw[k + r * TINY_NJ] /= Pw[k + k * LDT1];
+            if (r > k && r < nj) {
+                const T p = Pw[r + k * LDT1];
+                for (int c = 0; c < nb; ++c) Xw[r + c * TINY_NJ] -= p * Xw[k + c * TINY_NJ];
+            }
+        }
+        if (r < nj) {
+            for (int c = 0; c < nb; ++c) {
+                if (r < nn) stc(S.B + f.c0 + r + (int64_t)(cb + c) * S.ldb, Xw[r + c * TINY_NJ]);
+                else stc(MvJ + (r - nn) + (int64_t)(cb + c) * na, Xw[r + c * TINY_NJ]);
+            }
+        }
+    }
+}
+
+template <typename T>
+__device__ void wave_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, int J, T* Pw,
+                         T* Xw) {
+    const int r = threadIdx.x & 63;
+    const Front f = P.fronts[J];
+    const int nn = f.nn, nj = nn + f.na;
+    const bool unit = F.diag != 0;
+    const T* Pj = F.panel + f.poff;
+    if (r < nj)
+        for (int c = 0; c < nn; ++c) Pw[r + c * LDT1] = Pj[r + c * nj];
+    const int32_t* rows = P.rows + f.rowoff;
+    for (int cb = 0; cb < S.nrhs; cb += RB) {
+        const int nb = min(RB, S.nrhs - cb);
+        if (r < nj)
+            for (int c = 0; c < nb; ++c) Xw[r + c * TINY_NJ] = ldc(S.B + rows[r] + (int64_t)(cb + c) * S.ldb);
+        if (r < nn) {
+            for (int c = 0; c < nb; ++c) {
+                T s = (T)0;
+                for (int q = nn; q < nj; ++q) s += Pw[q + r * LDT1] * Xw[q + c * TINY_NJ];
+                Xw[r + c * TINY_NJ] -= s;
+            }
+        }
+        for (int k = nn - 1; k >= 0; --k) {
+            if (!unit && r < nb) Xw[k + r * TINY_NJ] /= Pw[k + k * LDT1];
+            if (r < k) {
+                const T p = Pw[k + r * LDT1];
+                for (int c = 0; c < nb; ++c) Xw[r + c * TINY_NJ] -= p * Xw[k + c * TINY_NJ];
+            }
+        }
+        if (r < nn)
+            for (int c = 0; c < nb; ++c) stc(S.B + f.c0 + r + (int64_t)(cb + c) * S.ldb, Xw[r + c * TINY_NJ]);
+    }
+}
+
 // Forward diagonal block [k0, k0+kb) of a big front: acc = B_k + gathered
 // children - sum_{ib<k} L_{k,ib} X_ib, taking X_ib as the front's progress
 // counter releases them, then X_k = W_k acc.
@@ -490,9 +573,12 @@ __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, DevSolve<T> S,
                                                 const PTask* tasks, int count, Sync Y_) {
-    __shared__ T Ps[SMALL_NJ * LDS];
-    __shared__ T X[LDS * RB];
-    __shared__ T Y[LDX * RB];
+    constexpr int NPS = SMALL_NJ * LDS > 4 * TINY_NJ * LDT1 ? SMALL_NJ * LDS : 4 * TINY_NJ * LDT1;
+    static_assert(2 * LDX * RB >= 4 * TINY_NJ * RB, "tiny right-hand-side slabs");
+    __shared__ T Ps[NPS];
+    __shared__ T XY[2 * LDX * RB];  // X and Y of the big-front tasks, or four tiny slabs
+    T* const X = XY;
+    T* const Y = XY + LDX * RB;
     __shared__ int s_t, s_av;
     const int tid = threadIdx.x;
     for (;;) {
@@ -503,6 +589,30 @@ __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, De
         if (t >= count) break;
         const unsigned long long t_begin = (Y_.trace && tid == 0) ? wall_clock64() : 0;
         const PTask tk = tasks[t];
+        if (tk.kind == PT_TINY) {
+            const int w = tid >> 6, lane = tid & 63;
+            const int J = w == 0 ? tk.front : w == 1 ? tk.a : w == 2 ? tk.b : tk.c;
+            if (J >= 0) {
+                const Front f = P.fronts[J];
+                if (lane == 0)
+                    for (int q = f.chd0; q < f.chd1; ++q) {
+                        const int K = P.chd[q];
+                        wait_geq(Y_.done + K, Y_.nt[K], F.hang);
+                    }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                wave_fwd(P, F, S, J, Ps + w * TINY_NJ * LDT1, XY + w * TINY_NJ * RB);
+                drain_stores();
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) __hip_atomic_fetch_add(Y_.done + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (Y_.trace && tid == 0) {
+                Y_.trace[2 * t] = t_begin | ((unsigned long long)blockIdx.x << 48);
+                Y_.trace[2 * t + 1] = wall_clock64();
+            }
+            __syncthreads();
+            continue;
+        }
         const int J = tk.front;
         const Front f = P.fronts[J];
         if (tid == 0) {
@@ -535,9 +645,12 @@ __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, De
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, DevSolve<T> S,
                                                 const PTask* tasks, int count, Sync Y_) {
-    __shared__ T Ps[SMALL_NJ * LDS];
-    __shared__ T X[LDS * RB];
-    __shared__ T Y[LDX * RB];
+    constexpr int NPS = SMALL_NJ * LDS > 4 * TINY_NJ * LDT1 ? SMALL_NJ * LDS : 4 * TINY_NJ * LDT1;
+    static_assert(2 * LDX * RB >= 4 * TINY_NJ * RB, "tiny right-hand-side slabs");
+    __shared__ T Ps[NPS];
+    __shared__ T XY[2 * LDX * RB];  // X and Y of the big-front tasks, or four tiny slabs
+    T* const X = XY;
+    T* const Y = XY + LDX * RB;
     __shared__ int s_t, s_av;
     const int tid = threadIdx.x;
     for (;;) {
@@ -548,6 +661,26 @@ __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, De
         if (t >= count) break;
         const unsigned long long t_begin = (Y_.trace && tid == 0) ? wall_clock64() : 0;
         const PTask tk = tasks[t];
+        if (tk.kind == PT_TINY) {
+            const int w = tid >> 6, lane = tid & 63;
+            const int J = w == 0 ? tk.front : w == 1 ? tk.a : w == 2 ? tk.b : tk.c;
+            if (J >= 0) {
+                const Front f = P.fronts[J];
+                if (lane == 0 && f.parent >= 0) wait_geq(Y_.done + f.parent, Y_.nt[f.parent], F.hang);
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                wave_bwd(P, F, S, J, Ps + w * TINY_NJ * LDT1, XY + w * TINY_NJ * RB);
+                drain_stores();
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) __hip_atomic_fetch_add(Y_.done + J, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (Y_.trace && tid == 0) {
+                Y_.trace[2 * t] = t_begin | ((unsigned long long)blockIdx.x << 48);
+                Y_.trace[2 * t + 1] = wall_clock64();
+            }
+            __syncthreads();
+            continue;
+        }
         const int J = tk.front;
         const Front f = P.fronts[J];
         if (tid == 0) {

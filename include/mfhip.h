@@ -197,8 +197,9 @@ int mf_dag_trace_read(mf_factor* f, unsigned long long* out, int64_t n);
 int mf_plan_dag(const mf_plan* plan, int32_t* tasks, int64_t n);
 /* With the profile on, the last solve's per-ticket times (forward tickets,
  * then backward), as mf_dag_trace_read; n >= 2 x solve tasks.
- * mf_plan_solve_tasks: 4 int32 per ticket (kind 0 small front / 1 diagonal
- * block / 2 update rows, front, a, b), forward then backward.
+ * mf_plan_solve_tasks: 5 int32 per ticket (kind 0 small front / 1 diagonal
+ * block / 2 update rows / 3 up to four tiny fronts, front, a, b, c),
+ * forward then backward.
  * mf_plan_fronts: 5 int64 per internal front (first permuted column, nn, na,
  * parent or -1, 1 if in the small region). */
 int mf_solve_trace_read(mf_factor* f, unsigned long long* out, int64_t n);
