@@ -316,7 +316,29 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
         for (int64_t g = H.resptr[j]; g < H.resptr[j + 1]; ++g) pl->col2ref[g] = (int32_t)j;
     for (int32_t J : H.small_order) pl->in_small[J] = 1;
     std::vector<int32_t> ea_src = H.ea_src;
+    std::vector<EaDst> ea_dst(H.ea.size());
+    for (size_t e = 0; e < H.ea.size(); ++e) {
+        const Front& F = H.fronts[H.ea[e].front];
+        const int32_t pc = H.ea[e].col;
+        const bool to_u = pc >= F.nn;
+        ea_dst[e].off = to_u ? F.uoff + (int64_t)(pc - F.nn) * F.na - F.nn
+                             : F.poff + (int64_t)pc * (F.nn + F.na);
+        ea_dst[e].to_u = to_u ? 1 : 0;
+        ea_dst[e].pad = 0;
+    }
+    std::vector<EaSrc> ea_srcx(H.ea_src.size() / 2);
+    for (size_t q = 0; q < ea_srcx.size(); ++q) {
+        const int32_t K = H.ea_src[2 * q], tcol = H.ea_src[2 * q + 1];
+        const Front& Kf = H.fronts[K];
+        ea_srcx[q].u_off = Kf.uoff + (int64_t)tcol * Kf.na;
+        ea_srcx[q].rel_off = Kf.reloff;
+        ea_srcx[q].ws_off = H.wsoff[K];
+        ea_srcx[q].tcol = tcol;
+        ea_srcx[q].pad = 0;
+    }
     if ((rc = upload(H.fronts, (Front**)&D.fronts, pl->allocs)) ||
+        (rc = upload(ea_dst, (EaDst**)&D.ea_dst, pl->allocs)) ||
+        (rc = upload(ea_srcx, (EaSrc**)&D.ea_srcx, pl->allocs)) ||
         (rc = upload(H.rows, (int32_t**)&D.rows, pl->allocs)) ||
         (rc = upload(H.chd, (int32_t**)&D.chd, pl->allocs)) ||
         (rc = upload(H.rel, (int32_t**)&D.rel, pl->allocs)) ||

@@ -311,37 +311,39 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
 // (M22 <- F22, cholesky.jl:542).  The child rows that land in window w are
 // [wstart[w], wstart[w+1]) (rel is increasing), so no search is needed; a
 // wavefront's LDS operations complete in order, so no barrier is needed.
-// Source s of the column keeps its addresses in lane s % 64 (loaded once per
-// column), every lane fetches its source's window bounds at once, and the
-// rows of one source are read four 64-row chunks at a time.
+// The column's destination and its sources' addresses come precomputed
+// (EaDst, EaSrc; source s in lane s % 64), the panel values of the window
+// are fetched together with the window bounds, and the next source's rows
+// are fetched while the current source's are added.
 template <typename T>
-__device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& F, const EaTask& e, T* acc,
+__device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& F, int ei, T* acc,
                           int lane) {
-    const Front f = P.fronts[e.front];
-    const int nj = f.nn + f.na, pc = e.col;
-    const bool to_u = pc >= f.nn;
-    T* dst = to_u ? F.U + f.uoff + (int64_t)(pc - f.nn) * f.na - f.nn
-                  : F.panel + f.poff + (int64_t)pc * nj;  // dst[p]: parent row p
+    const EaTask e = P.ea[ei];
+    const EaDst ed = P.ea_dst[ei];
+    T* dst = (ed.to_u ? F.U : F.panel) + ed.off;  // dst[p]: parent row p
+    const bool to_u = ed.to_u != 0;
     const int ns = e.src1 - e.src0;
-    // per-source addresses (sources sb .. sb + 63 in lanes)
     int64_t my_u = 0;
     int my_rel = 0, my_ws = 0, my_tcol = 0;
     auto load_meta = [&](int sb) {
         if (sb + lane < ns) {
-            const int sidx = e.src0 + sb + lane;
-            const int K = P.ea_src[2 * sidx], tcol = P.ea_src[2 * sidx + 1];
-            const Front kf = P.fronts[K];
-            my_u = kf.uoff + (int64_t)tcol * kf.na;
-            my_rel = kf.reloff;
-            my_ws = P.wsoff[K];
-            my_tcol = tcol;
+            const EaSrc x = P.ea_srcx[e.src0 + sb + lane];
+            my_u = x.u_off;
+            my_rel = x.rel_off;
+            my_ws = x.ws_off;
+            my_tcol = x.tcol;
         }
     };
     if (ns <= 64) load_meta(0);
     for (int w0 = (e.row0 / EA_WIN) * EA_WIN; w0 < e.row1; w0 += EA_WIN) {
         const int w = w0 / EA_WIN;
+        T pre[EA_WIN / 64];
 #pragma unroll
-        for (int j = 0; j < EA_WIN / 64; ++j) acc[lane + 64 * j] = (T)0;
+        for (int j = 0; j < EA_WIN / 64; ++j) {
+            const int p = w0 + lane + 64 * j;
+            pre[j] = (!to_u && p >= e.row0 && p < e.row1) ? dst[p] : (T)0;
+            acc[lane + 64 * j] = (T)0;
+        }
         for (int sb = 0; sb < ns; sb += 64) {
             if (ns > 64) load_meta(sb);
             const int nsb = min(64, ns - sb);
@@ -351,37 +353,61 @@ __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& 
                 my_i0 = max(ws[w], my_tcol);
                 my_i1 = ws[w + 1];
             }
-            for (int q = 0; q < nsb; ++q) {
-                const int i0 = __builtin_amdgcn_readlane(my_i0, q), i1 = __builtin_amdgcn_readlane(my_i1, q);
-                if (i0 >= i1) continue;  // uniform
+            // rows i0 + lane + 64 j (j < 4) of source q
+            int rr[4];
+            T vv[4];
+            auto fetch = [&](int q, int i) {
+                const int i1 = __builtin_amdgcn_readlane(my_i1, q);
                 const int relo = __builtin_amdgcn_readlane(my_rel, q);
                 const unsigned ulo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(uint64_t)my_u, q);
                 const unsigned uhi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)my_u >> 32), q);
                 const int64_t uo = (int64_t)(((uint64_t)uhi << 32) | ulo);
-                const int32_t* relK = P.rel + relo;
-                const T* UK = F.U + uo;
-                for (int i = i0 + lane; i < i1; i += 256) {
-                    int rr[4];
-                    T vv[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int ii = i + lane + 64 * j;
+                    rr[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
+                    vv[j] = ii < i1 ? F.U[uo + ii] : (T)0;
+                }
+            };
+            if (nsb > 0) fetch(0, __builtin_amdgcn_readlane(my_i0, 0));
+            for (int q = 0; q < nsb; ++q) {
+                const int i0 = __builtin_amdgcn_readlane(my_i0, q), i1 = __builtin_amdgcn_readlane(my_i1, q);
+                int cr[4];
+                T cv[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    cr[j] = rr[j];
+                    cv[j] = vv[j];
+                }
+                if (q + 1 < nsb) fetch(q + 1, __builtin_amdgcn_readlane(my_i0, q + 1));
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (cr[j] >= 0) acc[cr[j]] += cv[j];
+                // rows beyond the first 256 of this source in the window
+                for (int i = i0 + 256; i < i1; i += 256) {
+                    const int relo = __builtin_amdgcn_readlane(my_rel, q);
+                    const unsigned ulo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(uint64_t)my_u, q);
+                    const unsigned uhi =
+                        (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)my_u >> 32), q);
+                    const int64_t uo = (int64_t)(((uint64_t)uhi << 32) | ulo);
+                    int r2[4];
+                    T v2[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const int ii = i + 64 * j;
-                        rr[j] = ii < i1 ? relK[ii] - w0 : -1;
-                        vv[j] = ii < i1 ? UK[ii] : (T)0;
+                        const int ii = i + lane + 64 * j;
+                        r2[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
+                        v2[j] = ii < i1 ? F.U[uo + ii] : (T)0;
                     }
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        if (rr[j] >= 0) acc[rr[j]] += vv[j];
+                        if (r2[j] >= 0) acc[r2[j]] += v2[j];
                 }
             }
         }
 #pragma unroll
         for (int j = 0; j < EA_WIN / 64; ++j) {
             const int p = w0 + lane + 64 * j;
-            if (p >= e.row0 && p < e.row1) {
-                if (to_u) dst[p] = acc[lane + 64 * j];
-                else dst[p] += acc[lane + 64 * j];
-            }
+            if (p >= e.row0 && p < e.row1) dst[p] = to_u ? acc[lane + 64 * j] : pre[j] + acc[lane + 64 * j];
         }
     }
 }
@@ -1096,7 +1122,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
             case DT_EA: {
                 const int wave = tid >> 6, lane = tid & 63;
                 for (int e = d.a + wave; e < d.b; e += 4)
-                    ea_column<T>(P, F, P.ea[e], smem + wave * EA_WIN, lane);
+                    ea_column<T>(P, F, e, smem + wave * EA_WIN, lane);
                 break;
             }
             case DT_SMALL:

@@ -55,6 +55,22 @@ struct EaTask {
     int32_t src1;
 };
 
+// Device form of an extend-add column: destination (panel or U element of
+// parent row 0, so dst[p] is parent row p) and, per source, the child's U
+// column, rel map and window table, so the kernel reads them in one step.
+struct EaDst {
+    int64_t off;
+    int32_t to_u;
+    int32_t pad;
+};
+struct EaSrc {
+    int64_t u_off;    // U element (0, tcol) of the child
+    int32_t rel_off;  // child's rel map in rel[]
+    int32_t ws_off;   // child's window table in wstart[]
+    int32_t tcol;     // child column (its first row)
+    int32_t pad;
+};
+
 enum : int8_t { ARR_PANEL = 0, ARR_U = 1, ARR_W = 2 };
 enum : int8_t {
     G_SUB = 0,      // C -= A diag(s) B'            (trailing / SYRK updates)

@@ -23,6 +23,8 @@ struct DevPlan {
     const int32_t* level_big;
     const EaTask* ea;
     const int32_t* ea_src;
+    const EaDst* ea_dst;        // per extend-add column
+    const EaSrc* ea_srcx;       // per extend-add source (parallel to ea_src pairs)
     const int32_t* wstart;       // extend-add window tables
     const int32_t* wsoff;
     const GemmTask* gemm;
