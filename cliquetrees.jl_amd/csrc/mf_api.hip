@@ -327,14 +327,20 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
         ea_dst[e].pad = 0;
     }
     std::vector<EaSrc> ea_srcx(H.ea_src.size() / 2);
-    for (size_t q = 0; q < ea_srcx.size(); ++q) {
-        const int32_t K = H.ea_src[2 * q], tcol = H.ea_src[2 * q + 1];
-        const Front& Kf = H.fronts[K];
-        ea_srcx[q].u_off = Kf.uoff + (int64_t)tcol * Kf.na;
-        ea_srcx[q].rel_off = Kf.reloff;
-        ea_srcx[q].ws_off = H.wsoff[K];
-        ea_srcx[q].tcol = tcol;
-        ea_srcx[q].pad = 0;
+    for (const EaTask& e : H.ea) {
+        const int32_t w = e.row0 / EA_WIN;
+        for (int32_t q = e.src0; q < e.src1; ++q) {
+            const int32_t K = H.ea_src[2 * q], tcol = H.ea_src[2 * q + 1];
+            const Front& Kf = H.fronts[K];
+            EaSrc& x = ea_srcx[q];
+            x.u_off = Kf.uoff + (int64_t)tcol * Kf.na;
+            x.rel_off = Kf.reloff;
+            x.ws_off = H.wsoff[K];
+            x.tcol = tcol;
+            x.i0 = std::max(H.wstart[H.wsoff[K] + w], tcol);
+            x.i1 = H.wstart[H.wsoff[K] + w + 1];
+            x.pad = 0;
+        }
     }
     if ((rc = upload(H.fronts, (Front**)&D.fronts, pl->allocs)) ||
         (rc = upload(ea_dst, (EaDst**)&D.ea_dst, pl->allocs)) ||

@@ -324,7 +324,7 @@ __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& 
     const bool to_u = ed.to_u != 0;
     const int ns = e.src1 - e.src0;
     int64_t my_u = 0;
-    int my_rel = 0, my_ws = 0, my_tcol = 0;
+    int my_rel = 0, my_ws = 0, my_tcol = 0, my_f0 = 0, my_f1 = 0;
     auto load_meta = [&](int sb) {
         if (sb + lane < ns) {
             const EaSrc x = P.ea_srcx[e.src0 + sb + lane];
@@ -332,8 +332,11 @@ __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& 
             my_rel = x.rel_off;
             my_ws = x.ws_off;
             my_tcol = x.tcol;
+            my_f0 = x.i0;
+            my_f1 = x.i1;
         }
     };
+    const int wfirst = e.row0 / EA_WIN;
     if (ns <= 64) load_meta(0);
     for (int w0 = (e.row0 / EA_WIN) * EA_WIN; w0 < e.row1; w0 += EA_WIN) {
         const int w = w0 / EA_WIN;
@@ -347,8 +350,8 @@ __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& 
         for (int sb = 0; sb < ns; sb += 64) {
             if (ns > 64) load_meta(sb);
             const int nsb = min(64, ns - sb);
-            int my_i0 = 0, my_i1 = 0;
-            if (lane < nsb) {
+            int my_i0 = my_f0, my_i1 = my_f1;  // the first window's bounds come with the record
+            if (lane < nsb && w != wfirst) {
                 const int32_t* ws = P.wstart + my_ws;
                 my_i0 = max(ws[w], my_tcol);
                 my_i1 = ws[w + 1];
@@ -366,7 +369,7 @@ __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& 
                 for (int j = 0; j < 4; ++j) {
                     const int ii = i + lane + 64 * j;
                     rr[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
-                    vv[j] = ii < i1 ? F.U[uo + ii] : (T)0;
+                    vv[j] = ii < i1 ? ldc(F.U + uo + ii) : (T)0;
                 }
             };
             if (nsb > 0) fetch(0, __builtin_amdgcn_readlane(my_i0, 0));
@@ -396,7 +399,7 @@ __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& 
                     for (int j = 0; j < 4; ++j) {
                         const int ii = i + lane + 64 * j;
                         r2[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
-                        v2[j] = ii < i1 ? F.U[uo + ii] : (T)0;
+                        v2[j] = ii < i1 ? ldc(F.U + uo + ii) : (T)0;
                     }
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
@@ -1104,7 +1107,13 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                 s_sel = sel;
                 s_fail = 0;
                 const unsigned long long a0 = D.prof ? wall_clock64() : 0;
-                if (sel >= 0) acquire_agent();
+                // extend-add and small-front tasks read the only data other
+                // workgroups of this launch wrote for them (children's update
+                // matrices) with device-coherent loads: no cache invalidate
+                if (sel >= 0) {
+                    const int kd = D.task[s_win[sel]].kind;
+                    if (kd != DT_EA && kd != DT_SMALL) acquire_agent();
+                }
                 if (D.prof) atomicAdd(D.prof + 3 * DT_KINDS + 1, wall_clock64() - a0);
             }
         }

@@ -68,6 +68,7 @@ struct EaSrc {
     int32_t rel_off;  // child's rel map in rel[]
     int32_t ws_off;   // child's window table in wstart[]
     int32_t tcol;     // child column (its first row)
+    int32_t i0, i1;   // child rows landing in the column's first window
     int32_t pad;
 };
 
