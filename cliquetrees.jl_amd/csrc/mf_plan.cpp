@@ -1112,11 +1112,11 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 case DT_EA: {
                     double rows = 0;
                     for (int32_t e = d.a; e < d.b; ++e) rows += P.ea[e].row1 - P.ea[e].row0;
-                    c += rows * 0.0066;
+                    c += rows * 0.0053;
                     break;
                 }
                 case DT_SMALL: c += 20.0; break;
-                case DT_POTRF: c += 48.0; break;
+                case DT_POTRF: c += 26.0; break;
                 default: {
                     const GemmTask& g = P.gemm[d.a];
                     c += 2.0 * g.m * g.n * (double)g.k / (g.wide ? 1.3e5 : 7.5e4);
