@@ -120,7 +120,7 @@ __device__ __forceinline__ void factor_lds(T* M, int nn, int nj, int gc0, const 
                 *s_piv = (T)1;
                 *s_inv = (T)1 / r;
             } else {
-                F.d[gc0 + k] = (T)piv;
+                stc(F.d + gc0 + k, (T)piv);
                 *s_piv = (T)piv;
                 *s_inv = (T)1 / (T)piv;
             }
@@ -168,7 +168,7 @@ __device__ __forceinline__ void small_front(const DevPlan& P, const DevFactor<T>
     __syncthreads();
     factor_lds<T>(M, nn, nj, f.c0, F, s_piv, s_inv, s_fail, tid);
     if (r < nj)
-        for (int c = cg; c <= r && c < nn; c += 4) Pj[r + c * nj] = M[r + c * LDS];
+        for (int c = cg; c <= r && c < nn; c += 4) stc(Pj + r + c * nj, M[r + c * LDS]);
     if (na > 0 && r < na) {
         T* Uj = F.U + f.uoff;
         for (int c = cg; c <= r; c += 4) stc(Uj + r + c * na, M[(nn + r) + (nn + c) * LDS]);
@@ -216,7 +216,7 @@ __device__ __forceinline__ void wave_front(const DevPlan& P, const DevFactor<T>&
             inv = (T)1 / rt;
             spv = (T)1;
         } else {
-            if (lane == 0) F.d[gc0 + k] = (T)piv;
+            if (lane == 0) stc(F.d + gc0 + k, (T)piv);
             inv = (T)1 / (T)piv;
             spv = (T)piv;
         }
@@ -227,7 +227,7 @@ __device__ __forceinline__ void wave_front(const DevPlan& P, const DevFactor<T>&
         }
     }
     if (r < nj)
-        for (int c = 0; c <= r && c < nn; ++c) Pj[r + c * nj] = Mw[r + c * LDT1];
+        for (int c = 0; c <= r && c < nn; ++c) stc(Pj + r + c * nj, Mw[r + c * LDT1]);
     if (na > 0 && r < na) {
         T* Uj = F.U + f.uoff;
         for (int c = 0; c <= r; ++c) stc(Uj + r + c * na, Mw[(nn + r) + (nn + c) * LDT1]);
@@ -410,7 +410,7 @@ __device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& 
 #pragma unroll
         for (int j = 0; j < EA_WIN / 64; ++j) {
             const int p = w0 + lane + 64 * j;
-            if (p >= e.row0 && p < e.row1) dst[p] = to_u ? acc[lane + 64 * j] : pre[j] + acc[lane + 64 * j];
+            if (p >= e.row0 && p < e.row1) stc(dst + p, to_u ? acc[lane + 64 * j] : pre[j] + acc[lane + 64 * j]);
         }
     }
 }
@@ -449,7 +449,7 @@ __device__ void potrf_tile(const DevFactor<T>& F, const Front& f, int k0, bool i
             if (bad) piv = 1.0;  // keeps the rest finite after a failure
             if (tid == 0) {
                 if (bad) atomicMin(F.info, (unsigned long long)(gc0 + k + 1));
-                else if (F.diag != 0) F.d[gc0 + k] = (T)piv;
+                else if (F.diag != 0) stc(F.d + gc0 + k, (T)piv);
             }
             const T ip = (T)(1.0 / piv);
             const T ark = col[r];
@@ -617,7 +617,7 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
                     const T ark = x[k], ar = ark * ip;
                     if (ln == 0) {
                         if (bad) atomicMin(F.info, (unsigned long long)(gc0 + c0 + k + 1));
-                        else if (!chol) F.d[gc0 + c0 + k] = (T)piv;
+                        else if (!chol) stc(F.d + gc0 + c0 + k, (T)piv);
                         pv[c0 + k] = (T)piv;
                         rd[c0 + k] = isq;
                     }
@@ -680,7 +680,7 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
         }
         for (int c = cg; c < kb; c += 4) {
             T* colp = blk + (int64_t)c * nj;
-            if (r < kb && c <= r) colp[r] = A[r + c * LDS];
+            if (r < kb && c <= r) stc(colp + r, A[r + c * LDS]);
         }
         __syncthreads();  // L is read out before the inverse overwrites it
     } else if (tid < NB) {
@@ -733,7 +733,7 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
     T* W = F.W + f.woff + k0;
     for (int c = cg; c < kb; c += 4) {
         T* colp = W + (int64_t)c * f.nn;
-        if (r < kb) colp[r] = (c <= r) ? A[r + c * LDS] : (T)0;
+        if (r < kb) stc(colp + r, (c <= r) ? A[r + c * LDS] : (T)0);
     }
 }
 
@@ -837,11 +837,11 @@ __device__ __forceinline__ void gemm_tile(const DevFactor<T>& F, const GemmTask&
                 if (row < g.m && col < g.n && (!g.lower || row >= col)) {
                     T* cp = C + row + (int64_t)col * g.ldc;
                     if (sub) {
-                        *cp = cv[i][j][v] - acc[i][j][v];
+                        stc(cp, cv[i][j][v] - acc[i][j][v]);
                     } else {
                         T x = acc[i][j][v];
                         if (scale) x /= F.d[g.s_col + col];
-                        *cp = x;
+                        stc(cp, x);
                     }
                 }
             }
@@ -938,7 +938,7 @@ __device__ __forceinline__ void gemm_tile128(const DevFactor<T>& F, const GemmTa
             for (int v = 0; v < 4; ++v) {
                 const int row = wr + 16 * i + Mfma<T>::row(lk, v), col = wc + 16 * j + li;
                 if (row < g.m && col < g.n && (!g.lower || row >= col))
-                    C[row + (int64_t)col * g.ldc] = cv[j][v] - acc[i][j][v];
+                    stc(C + row + (int64_t)col * g.ldc, cv[j][v] - acc[i][j][v]);
             }
     }
 }
@@ -1162,7 +1162,9 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                 t_mark = now;
             }
             const unsigned long long r0 = D.prof ? wall_clock64() : 0;
-            release_agent();
+            // every task output is written through (device-coherent stores), and
+            // every wave drained its stores before the barrier: no L2 write-back
+            __builtin_amdgcn_s_waitcnt(0);
             constexpr int R = __ATOMIC_RELAXED, S = __HIP_MEMORY_SCOPE_AGENT;
             switch (d.kind) {
                 case DT_EA: __hip_atomic_fetch_add(D.eacnt + J, 1, R, S); break;
