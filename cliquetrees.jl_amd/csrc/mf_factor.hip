@@ -45,30 +45,17 @@ __device__ __forceinline__ double regularize(const DevFactor<T>& F, double s, do
     return Djj;
 }
 
-// Waits until this wave's global stores (and loads) have completed, i.e. reached
-// L2.  Before the closing barrier of a task every wave drains, so that thread
-// 0's agent-scope release (an L2 write-back) covers the whole workgroup's data.
+// Waits until this wave's global stores (and loads) have completed.  Task
+// outputs are device-coherent stores, so once every wave has drained before
+// the closing barrier, the counter update that thread 0 makes afterwards
+// follows all of the task's data.
 __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0); }
-// Agent-scope acquire (L1/L2 invalidate) and release (L2 write-back), each
-// followed by an explicit wait: the compiler omits the wait after the
-// write-back when its own counters are already zero (after drain_stores), and
-// a flag stored before the write-back completes can be seen by a workgroup on
-// another XCD ahead of the data.  The wait after the invalidate makes it
-// complete before the barrier releases the workgroup's other waves.
-__device__ __forceinline__ void acquire_agent() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __builtin_amdgcn_s_waitcnt(0);
-}
-__device__ __forceinline__ void release_agent() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_s_waitcnt(0);
-}
 
 // Bounded wait until *flag >= target; gives up (and records it) after ~1 s.
 __device__ __forceinline__ void wait_ge(int32_t* flag, int32_t target, int32_t* hang) {
     int spins = 0;
-    // poll with relaxed loads (an acquire per poll would invalidate the caches
-    // every other workgroup is using); the caller issues one acquire fence
+    // poll with relaxed loads; the data the flag guards is read with
+    // device-coherent loads afterwards
     while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         if (++spins > (1 << 24)) {
             atomicExch(hang, 1);
@@ -566,7 +553,7 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
     const bool chol = F.diag == 0;
     for (int c = cg; c < NB; c += 4) {
         const T* colp = blk + (int64_t)c * nj;
-        A[r + c * LDS] = (r < kb && c < kb && c <= r) ? colp[r] : (r == c ? (T)1 : (T)0);
+        A[r + c * LDS] = (r < kb && c < kb && c <= r) ? ldc(colp + r) : (r == c ? (T)1 : (T)0);
     }
     if (tid < NB) {
         pv[tid] = (T)1;
@@ -777,7 +764,7 @@ __device__ __forceinline__ void gemm_tile(const DevFactor<T>& F, const GemmTask&
                 for (int v = 0; v < 4; ++v) {
                     const int row = wr + i * 16 + Mfma<T>::row(lk, v), col = wc + j * 16 + li;
                     cv[i][j][v] = (row < g.m && col < g.n && (!g.lower || row >= col))
-                                      ? C[row + (int64_t)col * g.ldc] : (T)0;
+                                      ? ldc(C + row + (int64_t)col * g.ldc) : (T)0;
                 }
     }
     v4 acc[2][2];
@@ -794,10 +781,10 @@ __device__ __forceinline__ void gemm_tile(const DevFactor<T>& F, const GemmTask&
             T av = (T)0, bv = (T)0;
             if (gk < g.k) {
                 if (r < g.m) {
-                    av = A[r + (int64_t)gk * g.lda];
-                    if (s) av *= s[gk];
+                    av = ldc(A + r + (int64_t)gk * g.lda);
+                    if (s) av *= ldc(s + gk);
                 }
-                if (r < g.n) bv = B[r + (int64_t)gk * g.ldb];
+                if (r < g.n) bv = ldc(B + r + (int64_t)gk * g.ldb);
             }
             ra[i] = av;
             rb[i] = bv;
@@ -840,7 +827,7 @@ __device__ __forceinline__ void gemm_tile(const DevFactor<T>& F, const GemmTask&
                         stc(cp, cv[i][j][v] - acc[i][j][v]);
                     } else {
                         T x = acc[i][j][v];
-                        if (scale) x /= F.d[g.s_col + col];
+                        if (scale) x /= ldc(F.d + g.s_col + col);
                         stc(cp, x);
                     }
                 }
@@ -885,10 +872,10 @@ __device__ __forceinline__ void gemm_tile128(const DevFactor<T>& F, const GemmTa
             T av = (T)0, bv = (T)0;
             if (gk < g.k) {
                 if (r < g.m) {
-                    av = A[r + (int64_t)gk * g.lda];
-                    if (s) av *= s[gk];
+                    av = ldc(A + r + (int64_t)gk * g.lda);
+                    if (s) av *= ldc(s + gk);
                 }
-                if (r < g.n) bv = B[r + (int64_t)gk * g.ldb];
+                if (r < g.n) bv = ldc(B + r + (int64_t)gk * g.ldb);
             }
             ra[i] = av;
             rb[i] = bv;
@@ -930,7 +917,7 @@ __device__ __forceinline__ void gemm_tile128(const DevFactor<T>& F, const GemmTa
             for (int v = 0; v < 4; ++v) {
                 const int row = wr + 16 * i + Mfma<T>::row(lk, v), col = wc + 16 * j + li;
                 cv[j][v] = (row < g.m && col < g.n && (!g.lower || row >= col))
-                               ? C[row + (int64_t)col * g.ldc] : (T)0;
+                               ? ldc(C + row + (int64_t)col * g.ldc) : (T)0;
             }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -1022,9 +1009,13 @@ constexpr unsigned long long WMASK = WDEP == 64 ? ~0ull : (1ull << (WDEP & 63)) 
 //   D.cnt[toff_J + K*R_J + I]  panel columns applied to tile (I, K); K+1
 //                once the tile is final
 // Synchronization per task: lanes of wave 0 poll the dependencies of the
-// held tickets with relaxed loads (WDEP lanes per ticket), then thread 0
-// issues one agent-scope acquire; at the end every wave drains its stores,
-// then thread 0 publishes with an agent-scope release.
+// held tickets with relaxed loads (WDEP lanes per ticket).  Every value a
+// task reads that another task of this launch wrote is loaded with a
+// device-coherent (sc1) load, and every task output is stored with a
+// device-coherent store, so no L2 write-back or invalidate is needed: at the
+// end every wave drains its stores, the workgroup meets at a barrier, and
+// thread 0 publishes with relaxed counter updates.  (Whole-L2 write-backs per
+// task cost 3 ms of the 35 ms launch on the 64^3 Laplacian.)
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDag D) {
     constexpr int S1 = PTILE, S2 = 2 * KC * LDT, S3 = 2 * KW * LDW;
@@ -1107,13 +1098,8 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                 s_sel = sel;
                 s_fail = 0;
                 const unsigned long long a0 = D.prof ? wall_clock64() : 0;
-                // extend-add and small-front tasks read the only data other
-                // workgroups of this launch wrote for them (children's update
-                // matrices) with device-coherent loads: no cache invalidate
-                if (sel >= 0) {
-                    const int kd = D.task[s_win[sel]].kind;
-                    if (kd != DT_EA && kd != DT_SMALL) acquire_agent();
-                }
+                // no acquire: every read of data this launch wrote is a
+                // device-coherent load (ldc), so no cache needs an invalidate
                 if (D.prof) atomicAdd(D.prof + 3 * DT_KINDS + 1, wall_clock64() - a0);
             }
         }
