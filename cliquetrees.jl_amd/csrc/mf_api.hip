@@ -89,7 +89,7 @@ struct mf_factor {
     uint32_t* ticket = nullptr;
     int32_t* hang = nullptr;
     int32_t* cnt = nullptr;     // tile-DAG counters
-    int32_t* eacnt = nullptr;
+    int32_t* eacnt = nullptr;   // 2 (fronts + 1): all extend-add tasks, then panel ones
     unsigned long long* prof = nullptr;  // task-kind profile of the tile-DAG (NULL: off)
     // FP64 matrix of the refinement residual (full symmetric CSC, original order)
     int64_t A_n = -1, A_nnz = 0;
@@ -231,9 +231,11 @@ void enqueue_factor(mf_factor* f) {
     });
     if (!H.dag.empty()) {
         launch_fill<int32_t>(f->cnt, H.dag_counters, 0, s);
-        launch_fill<int32_t>(f->eacnt, std::max<int>(P.nJ, 1), 0, s);
+        const size_t nfr1 = H.fronts.size() + 1;
+        launch_fill<int32_t>(f->eacnt, 2 * nfr1, 0, s);
         launch_fill<uint32_t>(f->ticket, 1, 0u, s);
-        DevDag D{pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->prof, f->prof ? f->trace_buf : nullptr,
+        DevDag D{pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->eacnt + nfr1,
+                 f->prof, f->prof ? f->trace_buf : nullptr,
                  (int32_t)H.dag.size()};
         timed(f, MF_KC_DAG, H.engine_flops - pl->small_flops, 1, [&] { launch_dag<T>(P, F, D, s); });
     }
@@ -436,7 +438,7 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
               alloc((void**)&f->prog, 4 * (H.fronts.size() + 1)) &&
               alloc((void**)&f->ticket, 64) && alloc((void**)&f->hang, 64) &&
               alloc((void**)&f->cnt, 4 * (size_t)(H.dag_counters + 1)) &&
-              alloc((void**)&f->eacnt, 4 * (H.fronts.size() + 1)) &&
+              alloc((void**)&f->eacnt, 8 * (H.fronts.size() + 1)) &&
               alloc((void**)&f->perm_dev, 8 * (size_t)(H.n + 1));
     if (!ok) {
         delete f;

@@ -301,103 +301,118 @@ __global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const 
 // The column's destination and its sources' addresses come precomputed
 // (EaDst, EaSrc; source s in lane s % 64), the panel values of the window
 // are fetched together with the window bounds, and the next source's rows
-// are fetched while the current source's are added.
+// are fetched while the current source's are added.  A wavefront takes the
+// columns e_first, e_first + 4, ... < e_end; the records of the next column
+// are fetched while the current one is processed, so a column costs about
+// one memory round trip (its sources' rows) instead of three (records,
+// source records, rows).
 template <typename T>
-__device__ __forceinline__ void ea_column(const DevPlan& P, const DevFactor<T>& F, int ei, T* acc,
-                          int lane) {
-    const EaTask e = P.ea[ei];
-    const EaDst ed = P.ea_dst[ei];
-    T* dst = (ed.to_u ? F.U : F.panel) + ed.off;  // dst[p]: parent row p
-    const bool to_u = ed.to_u != 0;
-    const int ns = e.src1 - e.src0;
-    int64_t my_u = 0;
-    int my_rel = 0, my_ws = 0, my_tcol = 0, my_f0 = 0, my_f1 = 0;
-    auto load_meta = [&](int sb) {
-        if (sb + lane < ns) {
-            const EaSrc x = P.ea_srcx[e.src0 + sb + lane];
-            my_u = x.u_off;
-            my_rel = x.rel_off;
-            my_ws = x.ws_off;
-            my_tcol = x.tcol;
-            my_f0 = x.i0;
-            my_f1 = x.i1;
+__device__ __forceinline__ void ea_columns(const DevPlan& P, const DevFactor<T>& F, int e_first, int e_end,
+                                           T* acc, int lane) {
+    EaTask e_nx = P.ea[e_first];
+    EaDst d_nx = P.ea_dst[e_first];
+    EaSrc s_nx{};
+    if (e_nx.src1 - e_nx.src0 <= 64 && lane < e_nx.src1 - e_nx.src0) s_nx = P.ea_srcx[e_nx.src0 + lane];
+    for (int ei = e_first; ei < e_end; ei += 4) {
+        const EaTask e = e_nx;
+        const EaDst ed = d_nx;
+        int64_t my_u = s_nx.u_off;
+        int my_rel = s_nx.rel_off, my_ws = s_nx.ws_off, my_tcol = s_nx.tcol, my_f0 = s_nx.i0, my_f1 = s_nx.i1;
+        if (ei + 4 < e_end) {
+            e_nx = P.ea[ei + 4];
+            d_nx = P.ea_dst[ei + 4];
+            const int nsn = e_nx.src1 - e_nx.src0;
+            if (nsn <= 64 && lane < nsn) s_nx = P.ea_srcx[e_nx.src0 + lane];
         }
-    };
-    const int wfirst = e.row0 / EA_WIN;
-    if (ns <= 64) load_meta(0);
-    for (int w0 = (e.row0 / EA_WIN) * EA_WIN; w0 < e.row1; w0 += EA_WIN) {
-        const int w = w0 / EA_WIN;
-        T pre[EA_WIN / 64];
-#pragma unroll
-        for (int j = 0; j < EA_WIN / 64; ++j) {
-            const int p = w0 + lane + 64 * j;
-            pre[j] = (!to_u && p >= e.row0 && p < e.row1) ? dst[p] : (T)0;
-            acc[lane + 64 * j] = (T)0;
-        }
-        for (int sb = 0; sb < ns; sb += 64) {
-            if (ns > 64) load_meta(sb);
-            const int nsb = min(64, ns - sb);
-            int my_i0 = my_f0, my_i1 = my_f1;  // the first window's bounds come with the record
-            if (lane < nsb && w != wfirst) {
-                const int32_t* ws = P.wstart + my_ws;
-                my_i0 = max(ws[w], my_tcol);
-                my_i1 = ws[w + 1];
+        T* dst = (ed.to_u ? F.U : F.panel) + ed.off;  // dst[p]: parent row p
+        const bool to_u = ed.to_u != 0;
+        const int ns = e.src1 - e.src0;
+        auto load_meta = [&](int sb) {
+            if (sb + lane < ns) {
+                const EaSrc x = P.ea_srcx[e.src0 + sb + lane];
+                my_u = x.u_off;
+                my_rel = x.rel_off;
+                my_ws = x.ws_off;
+                my_tcol = x.tcol;
+                my_f0 = x.i0;
+                my_f1 = x.i1;
             }
-            // rows i0 + lane + 64 j (j < 4) of source q
-            int rr[4];
-            T vv[4];
-            auto fetch = [&](int q, int i) {
-                const int i1 = __builtin_amdgcn_readlane(my_i1, q);
-                const int relo = __builtin_amdgcn_readlane(my_rel, q);
-                const unsigned ulo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(uint64_t)my_u, q);
-                const unsigned uhi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)my_u >> 32), q);
-                const int64_t uo = (int64_t)(((uint64_t)uhi << 32) | ulo);
+        };
+        const int wfirst = e.row0 / EA_WIN;
+        for (int w0 = (e.row0 / EA_WIN) * EA_WIN; w0 < e.row1; w0 += EA_WIN) {
+            const int w = w0 / EA_WIN;
+            T pre[EA_WIN / 64];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int ii = i + lane + 64 * j;
-                    rr[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
-                    vv[j] = ii < i1 ? ldc(F.U + uo + ii) : (T)0;
+            for (int j = 0; j < EA_WIN / 64; ++j) {
+                const int p = w0 + lane + 64 * j;
+                pre[j] = (!to_u && p >= e.row0 && p < e.row1) ? dst[p] : (T)0;
+                acc[lane + 64 * j] = (T)0;
+            }
+            for (int sb = 0; sb < ns; sb += 64) {
+                if (ns > 64) load_meta(sb);
+                const int nsb = min(64, ns - sb);
+                int my_i0 = my_f0, my_i1 = my_f1;  // the first window's bounds come with the record
+                if (lane < nsb && w != wfirst) {
+                    const int32_t* ws = P.wstart + my_ws;
+                    my_i0 = max(ws[w], my_tcol);
+                    my_i1 = ws[w + 1];
                 }
-            };
-            if (nsb > 0) fetch(0, __builtin_amdgcn_readlane(my_i0, 0));
-            for (int q = 0; q < nsb; ++q) {
-                const int i0 = __builtin_amdgcn_readlane(my_i0, q), i1 = __builtin_amdgcn_readlane(my_i1, q);
-                int cr[4];
-                T cv[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    cr[j] = rr[j];
-                    cv[j] = vv[j];
-                }
-                if (q + 1 < nsb) fetch(q + 1, __builtin_amdgcn_readlane(my_i0, q + 1));
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (cr[j] >= 0) acc[cr[j]] += cv[j];
-                // rows beyond the first 256 of this source in the window
-                for (int i = i0 + 256; i < i1; i += 256) {
+                // rows i0 + lane + 64 j (j < 4) of source q
+                int rr[4];
+                T vv[4];
+                auto fetch = [&](int q, int i) {
+                    const int i1 = __builtin_amdgcn_readlane(my_i1, q);
                     const int relo = __builtin_amdgcn_readlane(my_rel, q);
                     const unsigned ulo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(uint64_t)my_u, q);
-                    const unsigned uhi =
-                        (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)my_u >> 32), q);
+                    const unsigned uhi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)my_u >> 32), q);
                     const int64_t uo = (int64_t)(((uint64_t)uhi << 32) | ulo);
-                    int r2[4];
-                    T v2[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int ii = i + lane + 64 * j;
-                        r2[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
-                        v2[j] = ii < i1 ? ldc(F.U + uo + ii) : (T)0;
+                        rr[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
+                        vv[j] = ii < i1 ? ldc(F.U + uo + ii) : (T)0;
                     }
+                };
+                if (nsb > 0) fetch(0, __builtin_amdgcn_readlane(my_i0, 0));
+                for (int q = 0; q < nsb; ++q) {
+                    const int i0 = __builtin_amdgcn_readlane(my_i0, q), i1 = __builtin_amdgcn_readlane(my_i1, q);
+                    int cr[4];
+                    T cv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        cr[j] = rr[j];
+                        cv[j] = vv[j];
+                    }
+                    if (q + 1 < nsb) fetch(q + 1, __builtin_amdgcn_readlane(my_i0, q + 1));
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        if (r2[j] >= 0) acc[r2[j]] += v2[j];
+                        if (cr[j] >= 0) acc[cr[j]] += cv[j];
+                    // rows beyond the first 256 of this source in the window
+                    for (int i = i0 + 256; i < i1; i += 256) {
+                        const int relo = __builtin_amdgcn_readlane(my_rel, q);
+                        const unsigned ulo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(uint64_t)my_u, q);
+                        const unsigned uhi =
+                            (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)my_u >> 32), q);
+                        const int64_t uo = (int64_t)(((uint64_t)uhi << 32) | ulo);
+                        int r2[4];
+                        T v2[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int ii = i + lane + 64 * j;
+                            r2[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
+                            v2[j] = ii < i1 ? ldc(F.U + uo + ii) : (T)0;
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (r2[j] >= 0) acc[r2[j]] += v2[j];
+                    }
                 }
             }
-        }
 #pragma unroll
-        for (int j = 0; j < EA_WIN / 64; ++j) {
-            const int p = w0 + lane + 64 * j;
-            if (p >= e.row0 && p < e.row1) stc(dst + p, to_u ? acc[lane + 64 * j] : pre[j] + acc[lane + 64 * j]);
+            for (int j = 0; j < EA_WIN / 64; ++j) {
+                const int p = w0 + lane + 64 * j;
+                if (p >= e.row0 && p < e.row1) stc(dst + p, to_u ? acc[lane + 64 * j] : pre[j] + acc[lane + 64 * j]);
+            }
         }
     }
 }
@@ -959,8 +974,8 @@ __device__ __forceinline__ bool dag_dep(const DevPlan& P, const DevFactor<T>& F,
         case DT_POTRF:
             *nw = 1;
             if (q >= 1) return false;
-            *w = d.a == 0 ? D.eacnt + J : cnt + d.a * g.R + d.a;
-            *target = d.a == 0 ? g.n_ea : d.a;
+            *w = d.a == 0 ? D.eapcnt + J : cnt + d.a * g.R + d.a;
+            *target = d.a == 0 ? g.n_eap : d.a;
             return true;
         case DT_TRSM:
             *nw = d.c > 0 ? 2 : 1;
@@ -976,9 +991,15 @@ __device__ __forceinline__ bool dag_dep(const DevPlan& P, const DevFactor<T>& F,
             *target = q == 2 ? K0 : K1;
             return true;
         }
-        default: {  // DT_SYRK: the tile's row blocks of the panel are final
-            *nw = 2 + (d.d & 1) + ((d.d >> 1) & 1);
+        default: {  // DT_SYRK: the tile's row blocks of the panel are final, and
+                    // every extend-add of the front (the update-matrix columns) is done
+            *nw = 3 + (d.d & 1) + ((d.d >> 1) & 1);
             if (q >= *nw) return false;
+            if (q == *nw - 1) {
+                *w = D.eacnt + J;
+                *target = g.n_ea;
+                return true;
+            }
             const int blk = q == 0 ? d.b : q == 1 ? d.c : (q == 2 && (d.d & 1)) ? d.b + 1 : d.c + 1;
             *w = cnt + (g.p - 1) * g.R + blk;
             *target = g.p;
@@ -1006,6 +1027,9 @@ constexpr unsigned long long WMASK = WDEP == 64 ? ~0ull : (1ull << (WDEP & 63)) 
 // resident workgroups.  Counters:
 //   F.flags[J]   finished update-matrix tiles of front J (1 for a small front)
 //   D.eacnt[J]   finished extend-add tasks of J (1 for a small front)
+//   D.eapcnt[J]  finished extend-add tasks of J into panel columns (the
+//                first diagonal tile waits only for these; the update-matrix
+//                tiles wait for all of them)
 //   D.cnt[toff_J + K*R_J + I]  panel columns applied to tile (I, K); K+1
 //                once the tile is final
 // Synchronization per task: lanes of wave 0 poll the dependencies of the
@@ -1116,8 +1140,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
         switch (d.kind) {
             case DT_EA: {
                 const int wave = tid >> 6, lane = tid & 63;
-                for (int e = d.a + wave; e < d.b; e += 4)
-                    ea_column<T>(P, F, e, smem + wave * EA_WIN, lane);
+                if (d.a + wave < d.b) ea_columns<T>(P, F, d.a + wave, d.b, smem + wave * EA_WIN, lane);
                 break;
             }
             case DT_SMALL:
@@ -1153,7 +1176,10 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
             __builtin_amdgcn_s_waitcnt(0);
             constexpr int R = __ATOMIC_RELAXED, S = __HIP_MEMORY_SCOPE_AGENT;
             switch (d.kind) {
-                case DT_EA: __hip_atomic_fetch_add(D.eacnt + J, 1, R, S); break;
+                case DT_EA:
+                    if (d.c) __hip_atomic_fetch_add(D.eapcnt + J, 1, R, S);
+                    __hip_atomic_fetch_add(D.eacnt + J, 1, R, S);
+                    break;
                 case DT_SMALL:
                     __hip_atomic_store(D.eacnt + J, 1, R, S);
                     __hip_atomic_store(F.flags + J, 1, R, S);

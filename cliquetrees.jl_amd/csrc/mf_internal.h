@@ -106,7 +106,8 @@ struct DagTask {
     int32_t front;
     int32_t a;   // DT_EA: first EaTask; DT_POTRF: panel block K; else: GemmTask index
     int32_t b;   // DT_EA: end EaTask; DT_TRSM/UPD/SYRK: row block I
-    int32_t c;   // DT_TRSM: panel block K; DT_UPD/SYRK: column block Jc
+    int32_t c;   // DT_EA: 1 for panel columns, 0 for update-matrix columns;
+                 // DT_TRSM: panel block K; DT_UPD/SYRK: column block Jc
     int32_t d;   // DT_UPD: panel blocks [K0, K1) as K0 | K1 << 16; DT_SYRK: bit 0 /
                  // bit 1: the tile also covers row block I+1 / column block Jc+1
 };
@@ -116,9 +117,9 @@ struct DagFront {
     int32_t toff;    // offset of the front's tile counters (K * R + I)
     int32_t R;       // row blocks (panel blocks p, then update-row blocks)
     int32_t p;       // panel (column) blocks
-    int32_t n_ea;    // extend-add tasks that must finish before the panel
+    int32_t n_ea;    // extend-add tasks (all; 1 for a small front)
     int32_t n_done;  // tasks whose completion makes U final (SYRK tiles, or 1 for small)
-    int32_t pad;
+    int32_t n_eap;   // extend-add tasks into panel columns: the panel waits only for these
 };
 
 // Copy-out of one reference front from its internal panel.

@@ -66,6 +66,7 @@ struct DevDag {
     const int32_t* uwait;
     int32_t* cnt;      // tile counters
     int32_t* eacnt;    // finished extend-add tasks per front
+    int32_t* eapcnt;   // finished extend-add tasks into panel columns per front
     unsigned long long* prof;  // NULL, or per task kind: wait, body (wall-clock ticks), count
     unsigned long long* trace; // NULL, or per ticket: start | workgroup << 48, end (wall clock)
     int32_t ntask;

@@ -509,16 +509,22 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 D.n_done = 0;  // counted with the update-matrix tiles below
             }
             // extend-add groups: consecutive columns of one front, about
-            // EA_ROWS parent rows (each wave of the task takes every fourth column)
+            // EA_ROWS parent rows (each wave of the task takes every fourth
+            // column).  A group holds panel columns or update-matrix columns,
+            // never both: the panel's first diagonal tile waits only for the
+            // panel groups, the update-matrix tiles for all of them.
             for (int32_t e = L.ea0; e < L.ea1;) {
                 int32_t J = P.ea[e].front, e1 = e + 1;
+                const bool pan = P.ea[e].col < P.fronts[J].nn;
                 int64_t rows = P.ea[e].row1 - P.ea[e].row0;
-                while (e1 < L.ea1 && e1 - e < 32 && P.ea[e1].front == J && rows < EA_ROWS) {
+                while (e1 < L.ea1 && e1 - e < 32 && P.ea[e1].front == J && rows < EA_ROWS &&
+                       (P.ea[e1].col < P.fronts[J].nn) == pan) {
                     rows += P.ea[e1].row1 - P.ea[e1].row0;
                     ++e1;
                 }
-                P.dag.push_back(DagTask{DT_EA, J, e, e1, 0, 0});
+                P.dag.push_back(DagTask{DT_EA, J, e, e1, pan ? 1 : 0, 0});
                 P.dagf[J].n_ea++;
+                if (pan) P.dagf[J].n_eap++;
                 e = e1;
             }
             int32_t maxp = 0;
@@ -758,9 +764,18 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     }
                 }
         }
-        for (auto it = P.small_order.rbegin(); it != P.small_order.rend(); ++it) {
-            P.btask.push_back(PTask{PT_SMALL, *it, 0, 0});
-            P.ntb[*it] = 1;
+        // the small region by depth (distance from the root): a front's
+        // children follow right after its own depth, however low their
+        // subtrees are (height order would put a short subtree's fronts at
+        // the end of the list and make them wait for unrelated tickets)
+        std::vector<int32_t> depth(NJ, 0);
+        for (int32_t J = NJ - 1; J >= 0; --J)
+            if (P.fronts[J].parent >= 0) depth[J] = depth[P.fronts[J].parent] + 1;
+        std::vector<int32_t> border(P.small_order.rbegin(), P.small_order.rend());
+        std::stable_sort(border.begin(), border.end(), [&](int32_t a, int32_t b) { return depth[a] < depth[b]; });
+        for (int32_t J : border) {
+            P.btask.push_back(PTask{PT_SMALL, J, 0, 0});
+            P.ntb[J] = 1;
         }
     }
 
@@ -965,7 +980,8 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 case DT_POTRF:
                     if (d.a == 0) {
                         for (int32_t e = P.gtask0[J]; e < P.gtask1[J]; ++e)
-                            if (!edge(e, t)) return "dataflow: potrf edge";
+                            if (P.dag[e].kind != DT_EA || P.dag[e].c)
+                                if (!edge(e, t)) return "dataflow: potrf edge";
                     } else if (!edge(lastw[tile(J, d.a, d.a)], t)) {
                         return "dataflow: potrf without its update";
                     }
@@ -989,6 +1005,9 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     const int32_t rows[4] = {d.b, d.c, (d.d & 1) ? d.b + 1 : d.b, (d.d & 2) ? d.c + 1 : d.c};
                     for (int32_t I : rows)
                         if (!edge(fin[tile(J, I, p - 1)], t)) return "dataflow: syrk without its rows";
+                    for (int32_t e = P.gtask0[J]; e < P.gtask1[J]; ++e)
+                        if (P.dag[e].kind == DT_EA && !P.dag[e].c && !edge(e, t))
+                            return "dataflow: syrk without its extend-add";
                     break;
                 }
                 default: break;
@@ -1191,7 +1210,8 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     // ---- DAG order validation: every wait targets an earlier ticket ------------
     {
         const int32_t NT = (int32_t)P.dag.size();
-        std::vector<int32_t> last_ea(NJ, -1), first_panel(NJ, INT32_MAX), last_done(NJ, -1);
+        std::vector<int32_t> last_ea(NJ, -1), last_pea(NJ, -1), first_panel(NJ, INT32_MAX), last_done(NJ, -1);
+        std::vector<int32_t> n_ea(NJ, 0), n_eap(NJ, 0);
         std::vector<int32_t> setter(P.dag_counters + 1, -1);  // ticket that finalizes (I, K) at value K+1
         std::map<int64_t, int32_t> upd_at;                      // (tile, K) -> ticket of UPD
         auto cnt_id = [&](int32_t J, int32_t I, int32_t K) {
@@ -1201,7 +1221,14 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             const DagTask& d = P.dag[t];
             const DagFront& D = P.dagf[d.front];
             switch (d.kind) {
-                case DT_EA: last_ea[d.front] = t; break;
+                case DT_EA:
+                    last_ea[d.front] = t;
+                    n_ea[d.front]++;
+                    if (d.c) {
+                        last_pea[d.front] = t;
+                        n_eap[d.front]++;
+                    }
+                    break;
                 case DT_SMALL: last_done[d.front] = t; break;
                 case DT_POTRF: setter[cnt_id(d.front, d.a, d.a)] = t; first_panel[d.front] = std::min(first_panel[d.front], t); break;
                 case DT_TRSM: setter[cnt_id(d.front, d.b, d.c)] = t; break;
@@ -1211,6 +1238,9 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             }
             if (d.kind >= DT_POTRF && (d.front < 0 || D.p <= 0)) return "dag task on a front without panel";
         }
+        for (int32_t J = 0; J < NJ; ++J)
+            if (n_ea[J] > 0 && (n_ea[J] != P.dagf[J].n_ea || n_eap[J] != P.dagf[J].n_eap))
+                return "dag: extend-add counts";
         std::vector<int32_t> last_consume(NJ, -1);
         for (int32_t t = 0; t < NT; ++t)
             if (P.dag[t].kind == DT_EA || P.dag[t].kind == DT_SMALL) last_consume[P.dag[t].front] = t;
@@ -1230,7 +1260,8 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 }
                 continue;
             }
-            if (last_ea[d.front] >= t) return "dag: panel task before extend-add";
+            if (last_pea[d.front] >= t) return "dag: panel task before extend-add";
+            if (d.kind == DT_SYRK && last_ea[d.front] >= t) return "dag: update-matrix tile before extend-add";
             auto need = [&](int32_t I, int32_t K) { return setter[cnt_id(d.front, I, K)]; };
             if (d.kind == DT_POTRF && d.a > 0) {
                 auto it = upd_at.find(cnt_id(d.front, d.a, d.a) * 4096 + d.a - 1);

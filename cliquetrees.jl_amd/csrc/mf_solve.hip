@@ -494,71 +494,142 @@ __device__ void rows_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
     }
 }
 
+// Separator part of a backward diagonal block, split over the four
+// wavefronts: wave w takes the separator tiles q = w, w + 4, ... (64 rows
+// each, as eight chunks of 8) and accumulates part[i] = -sum L21(rows,
+// block)' x_sep(rows) for all four 16-row slices i of the block, with no
+// barrier between chunks.  The front's separator row indices sit in LDS
+// (sidx, local row - nn), so the only global loads in flight are the
+// operands: those of chunk u + 1 load while chunk u is multiplied.  Lane
+// (li, lk) of chunk u holds rows nn + 64q + 8c + 4s + lk (c = u % 8, s < 2).
+constexpr int SEP_S = 2;  // 4-row MFMA steps per chunk
+
+template <typename T>
+__device__ __forceinline__ void sep_operands(const T* Pj, int64_t nj, int nn, int k0, int kb, const T* B,
+                                             int64_t ldb, int cb, int nb, int wv, int u, const int32_t* sidx,
+                                             T (&a)[4 * SEP_S], T (&b)[SEP_S]) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int base = 64 * (wv + 4 * (u >> 3)) + 8 * (u & 7) + lk;  // separator row (local - nn)
+    const int na = (int)nj - nn;
+#pragma unroll
+    for (int s = 0; s < SEP_S; ++s)
+        b[s] = (base + 4 * s < na && li < nb) ? ldc(B + sidx[base + 4 * s] + (int64_t)(cb + li) * ldb) : (T)0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = 16 * i + li;
+        const T* col = Pj + (int64_t)(k0 + m) * nj + nn + base;
+#pragma unroll
+        for (int s = 0; s < SEP_S; ++s) a[SEP_S * i + s] = (m < kb && base + 4 * s < na) ? -col[4 * s] : (T)0;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void sep_partial(const T* Pj, int nj, int nn, int nsep, int k0, int kb,
+                                            const int32_t* sidx, const T* B, int64_t ldb, int cb, int nb,
+                                            typename Mfma<T>::v4 (&part)[4]) {
+    const int wv = threadIdx.x >> 6;
+    const int nu = nsep > wv ? 8 * ((nsep - wv + 3) / 4) : 0;
+    if (nu == 0) return;
+    T a0[4 * SEP_S], b0[SEP_S], a1[4 * SEP_S], b1[SEP_S];
+    auto mma = [&](const T (&a)[4 * SEP_S], const T (&b)[SEP_S]) {
+#pragma unroll
+        for (int s = 0; s < SEP_S; ++s)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) part[i] = Mfma<T>::op(a[SEP_S * i + s], b[s], part[i]);
+    };
+    sep_operands(Pj, nj, nn, k0, kb, B, ldb, cb, nb, wv, 0, sidx, a0, b0);
+    // nu is a multiple of 8: chunks alternate between the two operand sets
+    for (int u = 0; u < nu; u += 2) {
+        sep_operands(Pj, nj, nn, k0, kb, B, ldb, cb, nb, wv, u + 1, sidx, a1, b1);
+        mma(a0, b0);
+        if (u + 2 < nu) sep_operands(Pj, nj, nn, k0, kb, B, ldb, cb, nb, wv, u + 2, sidx, a0, b0);
+        mma(a1, b1);
+    }
+}
+
+// LDS elements of a solve workgroup's panel buffer (a small front, four tiny
+// fronts, or the separator indices and partial sums of diag_bwd)
+constexpr int NPS = SMALL_NJ * LDS > 4 * TINY_NJ * LDT1 ? SMALL_NJ * LDS : 4 * TINY_NJ * LDT1;
+
 // Backward diagonal block [k0, k0+kb): acc = C_k - L21' x_sep - sum_{ib>k}
 // L_{ib,k}' X_ib (x_sep final: the ancestors are complete; X_ib as the
 // progress counter releases them, last block first), then X_k = W_k' acc.
+// The separator part is split over the wavefronts (sep_partial) and summed
+// in wave order through Red (4 x 64 x 16 elements of LDS), so the result
+// does not depend on timing.
 template <typename T>
 __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
-                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* X, T* Y, int* s_av) {
+                         const Front& f, int J, int k0, int kb, const Sync& Y_, T* X, T* Y, T* Red,
+                         int* s_av) {
     typedef typename Mfma<T>::v4 v4;
-    const int tid = threadIdx.x, li = tid & 15;
+    const int tid = threadIdx.x, li = tid & 15, lk = (tid & 63) >> 4, wv = tid >> 6;
     const int nn = f.nn, nj = nn + f.na;
     const T* Pj = F.panel + f.poff;
     const T* W = F.W + f.woff;
     const int32_t* rows = P.rows + f.rowoff;
     const int p = (nn + NB - 1) / NB, kblk = k0 / NB;
-    const int nsep = (nj - nn + NB - 1) / NB, nstep = nsep + (p - 1 - kblk);
-    // A fragment of step q: separator tile q (L21' rows nn + 64q...), then the
-    // blocks below the diagonal block, last first (L_{ib,k}')
-    auto frag = [&](int q, T (&a)[NB / 4]) {
-        if (q < nsep) {
-            const int rt = nn + q * NB;
-            load_frag(Pj + rt + (int64_t)k0 * nj, nj, 1, kb, min(NB, nj - rt), (T)-1, a);
-        } else {
-            const int ib = p - 1 - (q - nsep);
-            load_frag(Pj + ib * NB + (int64_t)k0 * nj, nj, 1, kb, min(NB, nn - ib * NB), (T)-1, a);
-        }
+    const int nsep = (nj - nn + NB - 1) / NB, nch = p - 1 - kblk;
+    // A fragment of chain step t: L_{ib,k}' for ib = p - 1 - t
+    auto cfrag = [&](int t, T (&a)[NB / 4]) {
+        const int ib = p - 1 - t;
+        load_frag(Pj + ib * NB + (int64_t)k0 * nj, nj, 1, kb, min(NB, nn - ib * NB), (T)-1, a);
     };
     for (int cb = 0; cb < S.nrhs; cb += RB) {
         const int nb = min(RB, S.nrhs - cb);
-        T wa[NB / 4], la[NB / 4];
-        load_frag(W + k0, f.nn, 1, kb, kb, (T)1, wa);
-        if (nstep > 0) frag(0, la);
         v4 acc;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = acc_row<T>(v);
             acc[v] = (row < kb && li < nb) ? S.B[f.c0 + k0 + row + (int64_t)(cb + li) * S.ldb] : (T)0;
         }
-        // right-hand sides of step q: separator rows (final), then the
-        // blocks below the diagonal block as the progress counter releases them
-        auto fetch_q = [&](int q, T (&x)[RB / 4]) {
-            if (q < nsep) {
-                const int rt = nn + q * NB;
-                fetch_gather(S.B, S.ldb, rows + rt, min(NB, nj - rt), cb, nb, x);
-            } else {
-                const int ib = p - 1 - (q - nsep);
-                fetch_rows(S.B, S.ldb, f.c0 + ib * NB, min(NB, nn - ib * NB), cb, nb, x);
+        if (nsep > 0) {
+            v4 part[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) part[i] = v4{0, 0, 0, 0};
+            // separator row indices in LDS (Red's space until the reduction);
+            // fronts with more separator rows than fit read them from rows[]
+            const int32_t* sidx = rows + nn;
+            if (nj - nn <= NPS * (int)sizeof(T) / 4) {
+                int32_t* si = reinterpret_cast<int32_t*>(Red);
+                for (int e = tid; e < nj - nn; e += blockDim.x) si[e] = rows[nn + e];
+                __syncthreads();
+                sidx = si;
             }
-        };
+            sep_partial(Pj, nj, nn, nsep, k0, kb, sidx, S.B, S.ldb, cb, nb, part);
+            __syncthreads();  // every wave is done with the indices in Red
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Red[wv * 1024 + 16 * i + Mfma<T>::row(lk, v) + li * 64] = part[i][v];
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int e = acc_row<T>(v) + li * 64;
+                acc[v] += ((Red[e] + Red[1024 + e]) + Red[2048 + e]) + Red[3072 + e];
+            }
+        }
+        // W and the first chain operand after the separator part (fewer live
+        // registers during it)
+        T wa[NB / 4], la[NB / 4];
+        load_frag(W + k0, f.nn, 1, kb, kb, (T)1, wa);
+        if (nch > 0) cfrag(0, la);
         int avail = 0;
         bool have = false;
         T xr[RB / 4];
-        for (int q = 0; q < nstep; ++q) {
+        for (int t = 0; t < nch; ++t) {
             if (tid == 0) {
-                const int need = q < nsep ? 0 : p - (p - 1 - (q - nsep));
-                if (need > avail) avail = wait_val(Y_.prog + J, need, F.hang);
+                if (t + 1 > avail) avail = wait_val(Y_.prog + J, t + 1, F.hang);
                 *s_av = avail;
             }
             __syncthreads();
             const int av = *s_av;
-            if (!have) fetch_q(q, xr);
+            const int ib = p - 1 - t;
+            if (!have) fetch_rows(S.B, S.ldb, f.c0 + ib * NB, min(NB, nn - ib * NB), cb, nb, xr);
             put_rows(xr, X);
-            have = q + 1 < nstep && (q + 1 < nsep || q + 2 - nsep <= av);
-            if (have) fetch_q(q + 1, xr);
+            have = t + 1 < nch && t + 2 <= av;
+            if (have) fetch_rows(S.B, S.ldb, f.c0 + (ib - 1) * NB, min(NB, nn - (ib - 1) * NB), cb, nb, xr);
             __syncthreads();
             frag_mma(la, X, acc);
-            if (q + 1 < nstep) frag(q + 1, la);
+            if (t + 1 < nch) cfrag(t + 1, la);
         }
         const v4 x = times_w(wa, acc, Y);
 #pragma unroll
@@ -573,7 +644,6 @@ __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, DevSolve<T> S,
                                                 const PTask* tasks, int count, Sync Y_) {
-    constexpr int NPS = SMALL_NJ * LDS > 4 * TINY_NJ * LDT1 ? SMALL_NJ * LDS : 4 * TINY_NJ * LDT1;
     static_assert(2 * LDX * RB >= 4 * TINY_NJ * RB, "tiny right-hand-side slabs");
     __shared__ T Ps[NPS];
     __shared__ T XY[2 * LDX * RB];  // X and Y of the big-front tasks, or four tiny slabs
@@ -645,8 +715,8 @@ __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, De
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, DevSolve<T> S,
                                                 const PTask* tasks, int count, Sync Y_) {
-    constexpr int NPS = SMALL_NJ * LDS > 4 * TINY_NJ * LDT1 ? SMALL_NJ * LDS : 4 * TINY_NJ * LDT1;
     static_assert(2 * LDX * RB >= 4 * TINY_NJ * RB, "tiny right-hand-side slabs");
+    static_assert(NPS >= 4 * NB * RB, "separator partial sums of diag_bwd in Ps");
     __shared__ T Ps[NPS];
     __shared__ T XY[2 * LDX * RB];  // X and Y of the big-front tasks, or four tiny slabs
     T* const X = XY;
@@ -688,7 +758,7 @@ __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, De
         }
         __syncthreads();
         if (tk.kind == PT_SMALL) small_bwd(P, F, S, f, Ps, X);
-        else diag_bwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y, &s_av);
+        else diag_bwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y, Ps, &s_av);
         drain_stores();
         __syncthreads();
         if (tid == 0) {
