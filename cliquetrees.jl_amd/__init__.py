@@ -9,6 +9,7 @@ from .factorization import (  # noqa: F401
     ChordalCholesky, ChordalLDLt, ChordalFactorization, DynamicRegularization,
     PosDefException, SingularException, MFError, cholesky_, ldlt_, ldiv_, solve, issuccess, refine,
     chol_impl, div_impl, device_plan, DevicePlan, DeviceFactor, checkinfo,
+    save_factor, load_factor, FACTOR_FORMAT,
     MF_FP64, MF_FP32, MF_CHOLESKY, MF_LDLT, STAT_NAMES, KC_NAMES,
 )
 from . import problems  # noqa: F401

@@ -506,6 +506,34 @@ int mf_factor_set_values(mf_factor* f, const double* Dval, const double* Lval, i
     return MF_OK;
 }
 
+int mf_factor_load(mf_factor* f, const double* Dval, const double* Lval, const double* d, int diag,
+                   int device) {
+    if (!f || (diag != MF_CHOLESKY && diag != MF_LDLT)) return fail(MF_EARG, "bad argument");
+    if (f->precision != MF_FP64) return fail(MF_EARG, "mf_factor_load needs an FP64 factor object");
+    const HostPlan& H = f->plan->H;
+    if (diag == MF_LDLT && !d && H.n) return fail(MF_EARG, "LDLt factor without d");
+    int rc = mf_factor_set_values(f, Dval, Lval, device);
+    if (rc) return rc;
+    hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    f->diag = diag;
+    if (H.n) {
+        if (d) {
+            HIPCHK(hipMemcpyAsync(f->dref, d, 8 * H.n, k, f->stream));
+            if (diag == MF_LDLT) HIPCHK(hipMemcpyAsync(f->d, d, 8 * H.n, k, f->stream));
+        } else {
+            launch_fill<double>(f->dref, H.n, 1.0, f->stream);
+        }
+    }
+    launch_fill<unsigned long long>(f->info, 1, ~0ull, f->stream);
+    // W = inverses of the 64 x 64 diagonal blocks, as the factorization leaves them
+    launch_diag<double>(f->plan->D, dev_factor<double>(f), 0, (int)H.diag.size(), true, f->stream);
+    HIPCHK(hipGetLastError());
+    if (!device) HIPCHK(hipStreamSynchronize(f->stream));
+    f->factored = true;
+    f->ref_stale = false;
+    return MF_OK;
+}
+
 int mf_factor_set_pattern(mf_factor* f, int64_t nnz, const int64_t* flat) {
     if (!f || nnz < 0 || (nnz > 0 && !flat)) return fail(MF_EARG, "bad argument");
     const HostPlan& H = f->plan->H;

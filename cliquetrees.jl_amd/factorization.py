@@ -212,6 +212,34 @@ class ChordalFactorization:
     def stats(self):
         return self._plan.stats()
 
+    @classmethod
+    def from_factor(cls, P: Permutation, S: ChordalSymbolic, Dval, Lval, d=None, info=0):
+        """A factorization given by its values, as the reference constructs a
+        ChordalFactorization from stored Dval/Lval/d (chordal_factorization.jl:1-21):
+        ready for ldiv_/solve without refactoring (mf_factor_load)."""
+        F = cls.__new__(cls)
+        F.P, F.S = P, S
+        F.perm, F.invp = P.perm, P.invp
+        F.n = S.n
+        F.info = int(info)
+        F._plan = device_plan(S)
+        F._f = DeviceFactor(F._plan, MF_FP64)
+        F._pattern = None
+        F._matrix_key = None
+        F._A = None
+        Dval = np.ascontiguousarray(Dval, dtype=np.float64)
+        Lval = np.ascontiguousarray(Lval, dtype=np.float64)
+        if Dval.shape != (S.ndz,) or Lval.shape != (S.nlz,):
+            raise ValueError("Dval/Lval do not match the symbolic factorization")
+        dp = None
+        if d is not None:
+            d = np.ascontiguousarray(d, dtype=np.float64)
+            if d.shape != (S.n,):
+                raise ValueError("d has the wrong length")
+            dp = _ptr(d)
+        _check(_native.hip().mf_factor_load(F._f.handle, _ptr(Dval), _ptr(Lval), dp, cls.DIAG, 0))
+        return F
+
 
 class ChordalCholesky(ChordalFactorization):
     DIAG = MF_CHOLESKY
@@ -363,3 +391,48 @@ def div_impl(S: ChordalSymbolic, Dval, Lval, d, C, diag=MF_CHOLESKY):
                                      Lval.ctypes.data_as(_native.f64p), dp,
                                      C.ctypes.data_as(_native.f64p), C.shape[1], S.n, diag))
     return C[:, 0] if vec else C
+
+
+# ---- factor files ---------------------------------------------------------------
+# One .npz of flat arrays (no pickled objects): the ChordalSymbolic arrays
+# (chordal_symbolic.jl:15-28), the permutation, and the factor values in the
+# reference layout (Dval, Lval, d; chordal_factorization.jl:1-21).  Loading
+# uploads the plan and the values and inverts the diagonal blocks; the
+# factor can be solved with at once.
+FACTOR_FORMAT = "cliquetrees-amd-factor-1"
+_SYM_ARRAYS = ["resptr", "sepptr", "septgt", "reltgt", "chdptr", "chdtgt", "pnt", "idx", "Dptr", "Lptr"]
+_SYM_SCALARS = ["n", "nfr", "nMptr", "nMval", "nNval", "nFval"]
+
+
+def save_factor(F: ChordalFactorization, path):
+    """Write the factor F (after cholesky_/ldlt_ or from_factor) to path (.npz)."""
+    if F.info != 0:
+        raise ValueError("the factorization failed; nothing to save")
+    S = F.S
+    out = {"format": np.array(FACTOR_FORMAT), "diag": np.int64(F.DIAG), "info": np.int64(F.info),
+           "perm": np.asarray(F.perm, dtype=np.int64), "invp": np.asarray(F.invp, dtype=np.int64),
+           "Dval": F.Dval, "Lval": F.Lval, "d": F.d}
+    for k in _SYM_ARRAYS:
+        out[k] = np.asarray(getattr(S, k), dtype=np.int64)
+    for k in _SYM_SCALARS:
+        out[k] = np.int64(getattr(S, k))
+    np.savez(path, **out)
+
+
+def load_factor(path) -> ChordalFactorization:
+    """Read a factor written by save_factor: a ChordalCholesky or ChordalLDLt on
+    the device, ready for ldiv_/solve."""
+    with np.load(path, allow_pickle=False) as z:
+        if str(z["format"]) != FACTOR_FORMAT:
+            raise ValueError(f"not a factor file of format {FACTOR_FORMAT}")
+        S = ChordalSymbolic(**{k: np.array(z[k], dtype=np.int64) for k in _SYM_ARRAYS},
+                            **{k: int(z[k]) for k in _SYM_SCALARS})
+        perm = np.array(z["perm"], dtype=np.int64)
+        invp = np.array(z["invp"], dtype=np.int64)
+        P = Permutation.from_perm(perm)
+        if not np.array_equal(P.invp, invp):
+            raise ValueError("inconsistent permutation in the factor file")
+        diag = int(z["diag"])
+        cls = ChordalLDLt if diag == MF_LDLT else ChordalCholesky
+        return cls.from_factor(P, S, z["Dval"], z["Lval"], z["d"], info=int(z["info"]))
+

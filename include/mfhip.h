@@ -84,6 +84,19 @@ int mf_factor_sync(mf_factor* f);
 int mf_factor_set_values(mf_factor* f, const double* Dval, const double* Lval, int device);
 
 /*
+ * A factor given by its values: the reference's ChordalFactorization built
+ * from stored Dval/Lval/d (chordal_factorization.jl:1-21), as a saved factor
+ * is restored.  Dval/Lval hold the factor in the reference layout (unit L
+ * with the pivots in d for LDLt, diag = MF_LDLT); d may be NULL for
+ * Cholesky (then d reads back as ones).  The diagonal blocks are inverted on
+ * the device and the factor is ready for mf_solve / mf_solve_natural, as
+ * after mf_factorize.  FP64 factor objects only.  device != 0: device
+ * pointers, nothing waits on the host.
+ */
+int mf_factor_load(mf_factor* f, const double* Dval, const double* Lval, const double* d, int diag,
+                   int device);
+
+/*
  * Register flatindices(F, A) (chordal_factorization.jl:322-328) for a
  * pattern: flat[k] is the 1-based slot of nonzero k in [Dval; Lval] or 0.
  * mf_factor_refresh then does fill!(F, 0) followed by setflatindex! of every

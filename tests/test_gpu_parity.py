@@ -441,3 +441,47 @@ def test_repeat_factorization_bitwise(ct):
     torch.cuda.synchronize()
     bad = [i for i, Y in enumerate(Ys) if not torch.equal(Y, Xs[0])]
     assert not bad, bad
+
+
+@pytest.mark.parametrize("name", ["lap2d_30_nd", "lap3d_16"])
+def test_factor_file_round_trip(ct, oracle, cases, name, tmp_path):
+    """save_factor / load_factor: the restored factor holds the same values bit
+    for bit and solves like the factor it was saved from (its diagonal-block
+    inverses are recomputed, so to rounding) and like the oracle."""
+    A, order, snd = cases[name]
+    print(f"case {name}", flush=True)
+    P, S = ct.symbolic(A, alg=order, snd=snd)
+    F = ct.ChordalCholesky(A, P, S)
+    ct.cholesky_(F)
+    path = tmp_path / "factor.npz"
+    ct.save_factor(F, path)
+    G = ct.load_factor(path)
+    assert isinstance(G, ct.ChordalCholesky) and G.info == 0
+    assert np.array_equal(G.Dval, F.Dval) and np.array_equal(G.Lval, F.Lval)
+    assert np.array_equal(G.perm, F.perm)
+    B = np.random.default_rng(7).standard_normal((A.shape[0], 16))
+    X, Y = ct.solve(F, B), ct.solve(G, B)
+    assert rel(Y, X) <= 1e-12
+    Dv, Lv = oracle.assemble(S, P, A)
+    D0, L0, d0, _ = oracle.chol_impl(S, Dv, Lv, diag=0)
+    assert rel(Y, oracle.solve(S, P, D0, L0, d0, B, diag=0)) <= 1e-11
+    assert np.linalg.norm(A @ Y - B) / np.linalg.norm(B) <= 1e-12
+
+
+def test_factor_file_round_trip_ldlt(ct, oracle, tmp_path):
+    K, nh, m = ct.problems.kkt_matrix((6, 6, 5), seed=5)
+    P, S = ct.symbolic(K, alg="mmd") if _has_mmd(ct) else ct.symbolic(K, alg=_kkt_order(nh, m))
+    F = ct.ChordalLDLt(K, P, S)
+    ct.ldlt_(F)
+    path = tmp_path / "factor_ldlt.npz"
+    ct.save_factor(F, path)
+    G = ct.load_factor(path)
+    assert isinstance(G, ct.ChordalLDLt)
+    assert np.array_equal(G.d, F.d) and np.array_equal(G.Lval, F.Lval)
+    b = np.random.default_rng(3).standard_normal(K.shape[0])
+    x, y = ct.solve(F, b), ct.solve(G, b)
+    assert rel(y, x) <= 1e-10
+    # a file of another format is refused
+    np.savez(tmp_path / "bad.npz", format=np.array("something-else"))
+    with pytest.raises(ValueError):
+        ct.load_factor(tmp_path / "bad.npz")
