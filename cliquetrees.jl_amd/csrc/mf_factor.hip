@@ -357,20 +357,28 @@ __device__ __forceinline__ void ea_columns(const DevPlan& P, const DevFactor<T>&
                     my_i0 = max(ws[w], my_tcol);
                     my_i1 = ws[w + 1];
                 }
-                // rows i0 + lane + 64 j (j < 4) of source q
+                // rows i0 + lane + 64 j (j < 4) of source q.  The loads are
+                // unconditional (rows past the source's end read its row 0,
+                // always in bounds, and are flagged in ok), so the eight loads
+                // of a fetch are in flight together; a load under a per-lane
+                // branch would be waited for inside that branch.
                 int rr[4];
                 T vv[4];
+                unsigned ok = 0;
                 auto fetch = [&](int q, int i) {
                     const int i1 = __builtin_amdgcn_readlane(my_i1, q);
                     const int relo = __builtin_amdgcn_readlane(my_rel, q);
                     const unsigned ulo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(uint64_t)my_u, q);
                     const unsigned uhi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)my_u >> 32), q);
                     const int64_t uo = (int64_t)(((uint64_t)uhi << 32) | ulo);
+                    ok = 0;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int ii = i + lane + 64 * j;
-                        rr[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
-                        vv[j] = ii < i1 ? ldc(F.U + uo + ii) : (T)0;
+                        const int ic = ii < i1 ? ii : 0;
+                        ok |= (ii < i1 ? 1u : 0u) << j;
+                        rr[j] = P.rel[relo + ic];
+                        vv[j] = ldc(F.U + uo + ic);
                     }
                 };
                 if (nsb > 0) fetch(0, __builtin_amdgcn_readlane(my_i0, 0));
@@ -378,6 +386,7 @@ __device__ __forceinline__ void ea_columns(const DevPlan& P, const DevFactor<T>&
                     const int i0 = __builtin_amdgcn_readlane(my_i0, q), i1 = __builtin_amdgcn_readlane(my_i1, q);
                     int cr[4];
                     T cv[4];
+                    const unsigned cok = ok;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         cr[j] = rr[j];
@@ -386,7 +395,7 @@ __device__ __forceinline__ void ea_columns(const DevPlan& P, const DevFactor<T>&
                     if (q + 1 < nsb) fetch(q + 1, __builtin_amdgcn_readlane(my_i0, q + 1));
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        if (cr[j] >= 0) acc[cr[j]] += cv[j];
+                        if ((cok >> j) & 1u) acc[cr[j] - w0] += cv[j];
                     // rows beyond the first 256 of this source in the window
                     for (int i = i0 + 256; i < i1; i += 256) {
                         const int relo = __builtin_amdgcn_readlane(my_rel, q);
@@ -399,12 +408,13 @@ __device__ __forceinline__ void ea_columns(const DevPlan& P, const DevFactor<T>&
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             const int ii = i + lane + 64 * j;
-                            r2[j] = ii < i1 ? P.rel[relo + ii] - w0 : -1;
-                            v2[j] = ii < i1 ? ldc(F.U + uo + ii) : (T)0;
+                            const int ic = ii < i1 ? ii : 0;
+                            r2[j] = P.rel[relo + ic];
+                            v2[j] = ldc(F.U + uo + ic);
                         }
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            if (r2[j] >= 0) acc[r2[j]] += v2[j];
+                            if (i + lane + 64 * j < i1) acc[r2[j] - w0] += v2[j];
                     }
                 }
             }
