@@ -144,14 +144,44 @@ __device__ __forceinline__ void small_front(const DevPlan& P, const DevFactor<T>
         const int32_t* relK = P.rel + kf.reloff;
         const T* UK = F.U + kf.uoff;
         if (r < ma) {
+            // all loads first (fixed count, clamped in-bounds addresses), then
+            // the adds: loads under a per-lane loop bound would each be waited
+            // for inside the loop
             const int rr = relK[r];
-            for (int c = cg; c <= r; c += 4) M[rr + relK[c] * LDS] += ldc(UK + r + c * ma);
+            for (int i0 = 0; i0 < SMALL_NJ / 4; i0 += 8) {
+                if (cg + 4 * i0 > r) break;
+                int cc[8];
+                T v[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int c = cg + 4 * (i0 + i), cs = c <= r ? c : 0;
+                    cc[i] = relK[cs];
+                    v[i] = ldc(UK + r + cs * ma);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (cg + 4 * (i0 + i) <= r) M[rr + cc[i] * LDS] += v[i];
+            }
         }
         __syncthreads();
     }
     T* Pj = F.panel + f.poff;
-    if (r < nj)
-        for (int c = cg; c <= r && c < nn; c += 4) M[r + c * LDS] += Pj[r + c * nj];
+    if (r < nj) {
+        for (int i0 = 0; i0 < SMALL_NJ / 4; i0 += 8) {
+            if (cg + 4 * i0 > r || cg + 4 * i0 >= nn) break;
+            T v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int c = cg + 4 * (i0 + i);
+                v[i] = (c <= r && c < nn) ? Pj[r + c * nj] : (T)0;
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int c = cg + 4 * (i0 + i);
+                if (c <= r && c < nn) M[r + c * LDS] += v[i];
+            }
+        }
+    }
     __syncthreads();
     factor_lds<T>(M, nn, nj, f.c0, F, s_piv, s_inv, s_fail, tid);
     if (r < nj)
@@ -182,13 +212,42 @@ __device__ __forceinline__ void wave_front(const DevPlan& P, const DevFactor<T>&
         const int32_t* relK = P.rel + kf.reloff;
         const T* UK = F.U + kf.uoff;
         if (r < ma) {
+            // loads first, eight at a time (fixed count, clamped in-bounds
+            // addresses), then the adds
             const int rr = relK[r];
-            for (int c = 0; c <= r; ++c) Mw[rr + relK[c] * LDT1] += ldc(UK + r + c * ma);
+#pragma unroll
+            for (int c0 = 0; c0 < TINY_NJ; c0 += 8) {
+                if (c0 > r) break;
+                int cc[8];
+                T v[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int c = c0 + i, cs = c <= r ? c : 0;
+                    cc[i] = relK[cs];
+                    v[i] = ldc(UK + r + cs * ma);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (c0 + i <= r) Mw[rr + cc[i] * LDT1] += v[i];
+            }
         }
     }
     T* Pj = F.panel + f.poff;
-    if (r < nj)
-        for (int c = 0; c <= r && c < nn; ++c) Mw[r + c * LDT1] += Pj[r + c * nj];
+    if (r < nj) {
+#pragma unroll
+        for (int c0 = 0; c0 < TINY_NJ; c0 += 8) {
+            if (c0 > r || c0 >= nn) break;
+            T v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int c = c0 + i;
+                v[i] = (c <= r && c < nn) ? Pj[r + c * nj] : (T)0;
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (c0 + i <= r && c0 + i < nn) Mw[r + (c0 + i) * LDT1] += v[i];
+        }
+    }
     for (int k = 0; k < nn; ++k) {
         const T akk = Mw[k + k * LDT1];
         double piv = regularize(F, sign_of(F, gc0 + k), (double)akk);
@@ -242,7 +301,7 @@ __device__ __forceinline__ void lane_wait_ge(int32_t* flag, int32_t target, int3
 // accesses, so a front is published by draining its stores and setting its
 // flag; no cache-wide write-back or invalidate.
 template <typename T>
-__global__ __launch_bounds__(256) void k_small(DevPlan P, DevFactor<T> F, const int32_t* list,
+__global__ __launch_bounds__(256, 4) void k_small(DevPlan P, DevFactor<T> F, const int32_t* list,
                                                const int32_t* units, int count) {
     constexpr int MS = SMALL_NJ * LDS > 4 * TINY_NJ * LDT1 ? SMALL_NJ * LDS : 4 * TINY_NJ * LDT1;
     __shared__ T M[MS];
