@@ -358,6 +358,8 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
         (rc = upload(ea_src, (int32_t**)&D.ea_src, pl->allocs)) ||
         (rc = upload(H.wstart, (int32_t**)&D.wstart, pl->allocs)) ||
         (rc = upload(H.wsoff, (int32_t**)&D.wsoff, pl->allocs)) ||
+        (rc = upload(H.sgstart, (int32_t**)&D.sgstart, pl->allocs)) ||
+        (rc = upload(H.sgoff, (int32_t**)&D.sgoff, pl->allocs)) ||
         (rc = upload(H.gemm, (GemmTask**)&D.gemm, pl->allocs)) ||
         (rc = upload(H.diag, (DiagTask**)&D.diag, pl->allocs)) ||
         (rc = upload(H.ftask, &pl->d_ftask, pl->allocs)) ||

@@ -27,6 +27,8 @@ struct DevPlan {
     const EaSrc* ea_srcx;       // per extend-add source (parallel to ea_src pairs)
     const int32_t* wstart;       // extend-add window tables
     const int32_t* wsoff;
+    const int32_t* sgstart;      // solve gather tables: sgstart[sgoff[K] + b] = first row of
+    const int32_t* sgoff;        // child K landing in or after row block b of its parent
     const GemmTask* gemm;
     const DiagTask* diag;
     const OutFront* out;

@@ -602,6 +602,46 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         P.dag_counters = toff + 1;
     }
 
+    // ---- solve gather tables ----------------------------------------------------
+    // A solve task of a big front J covers one row block b of J (diagonal
+    // blocks [64b, 64b + 64) for b < p, then update rows [nn + 64(b-p), ...)).
+    // For each child K of J, sgstart[sgoff[K] + b] is the first row i of K
+    // with rel_K[i] >= the first row of block b (rel is increasing), b = 0..R,
+    // so the rows of K landing in block b are [tab[b], tab[b+1]).
+    P.sgoff.assign(NJ, -1);
+    P.sgstart.clear();
+    for (int32_t J = 0; J < NJ; ++J) {
+        if (P.is_small[J]) continue;
+        const Front& F = P.fronts[J];
+        const int32_t p = (F.nn + NB - 1) / NB, R = p + (F.na + NB - 1) / NB;
+        auto brow = [&](int32_t b) { return b < p ? std::min(b * NB, F.nn) : std::min(F.nn + (b - p) * NB, F.nn + F.na); };
+        for (int32_t q = F.chd0; q < F.chd1; ++q) {
+            const int32_t K = P.chd[q];
+            const Front& Kf = P.fronts[K];
+            P.sgoff[K] = (int32_t)P.sgstart.size();
+            int32_t i = 0;
+            for (int32_t b = 0; b <= R; ++b) {
+                const int32_t r0 = b < R ? brow(b) : F.nn + F.na;
+                while (i < Kf.na && P.rel[Kf.reloff + i] < r0) ++i;
+                P.sgstart.push_back(i);
+            }
+        }
+    }
+
+    for (int32_t J = 0; J < NJ; ++J) {
+        if (P.is_small[J]) continue;
+        const Front& F = P.fronts[J];
+        const int32_t R = (F.nn + NB - 1) / NB + (F.na + NB - 1) / NB;
+        for (int32_t q = F.chd0; q < F.chd1; ++q) {
+            const int32_t K = P.chd[q], o = P.sgoff[K];
+            if (o < 0 || (size_t)o + R + 1 > P.sgstart.size()) return "solve gather table";
+            for (int32_t b = 0; b <= R; ++b)
+                if (P.sgstart[o + b] < 0 || P.sgstart[o + b] > P.fronts[K].na ||
+                    (b > 0 && P.sgstart[o + b] < P.sgstart[o + b - 1]))
+                    return "solve gather table range";
+        }
+    }
+
     // ---- persistent solve task lists ------------------------------------------
     // Each direction is one ticket list.  Forward tickets are ordered by a
     // host list schedule (SIM_WORKERS workgroups, longest remaining path
@@ -889,7 +929,9 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     if (F.woff < 0 || t.a < 0 || t.b <= 0 || t.b > NB || t.a + t.b > F.nn || t.a % NB)
                         return "diag solve task";
                 } else if (t.kind == PT_ROWS) {
-                    if (t.a < F.nn || t.b > nj || t.b <= t.a || t.b - t.a > NB) return "rows solve task";
+                    if (t.a < F.nn || t.b > nj || t.b <= t.a || t.b - t.a > NB || (t.a - F.nn) % NB ||
+                        (t.b != nj && t.b - t.a != NB))
+                        return "rows solve task";
                 } else {
                     return "solve task kind";
                 }

@@ -49,6 +49,7 @@ struct HostPlan {
     std::vector<EaTask> ea;
     std::vector<int32_t> ea_src;       // pairs (child, child column)
     std::vector<int32_t> wstart, wsoff;  // extend-add window tables per child (-1: none)
+    std::vector<int32_t> sgstart, sgoff;  // solve gather tables per child of a big front (-1: none)
     std::vector<GemmTask> gemm;
     std::vector<DiagTask> diag;
     std::vector<PTask> ftask, btask;   // forward / backward solve tasks, ticket order

@@ -100,12 +100,15 @@ struct Sync {
     unsigned long long* trace;  // nullptr or per ticket: start | workgroup << 48, end
 };
 
-// Children's update vectors scattered into Y (rows [r0, r1) of front f,
-// column c of the pass at Y[(row - r0) + c * LDX]).
+// Children's update vectors scattered into Y (rows [r0, r1) of front f, one
+// row block: [64b, 64b + 64) or [nn + 64m, ...); column c of the pass at
+// Y[(row - r0) + c * LDX]).
 template <typename T>
 __device__ void gather_children(const DevPlan& P, const DevSolve<T>& S, const Front& f, int r0,
                                 int r1, int cb, int nb, T* Y) {
     const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
+    const int p = (f.nn + NB - 1) / NB, blk = r0 < f.nn ? r0 / NB : p + (r0 - f.nn) / NB;
+    (void)r1;
     for (int e = tid; e < LDX * RB; e += blockDim.x) Y[e] = (T)0;
     __syncthreads();
     for (int q = f.chd0; q < f.chd1; ++q) {
@@ -113,17 +116,9 @@ __device__ void gather_children(const DevPlan& P, const DevSolve<T>& S, const Fr
         const Front kf = P.fronts[K];
         const int ma = kf.na;
         const int32_t* relK = P.rel + kf.reloff;
-        int lo = 0, hi = ma;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (relK[mid] < r0) lo = mid + 1; else hi = mid;
-        }
-        const int a = lo;
-        hi = ma;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (relK[mid] < r1) lo = mid + 1; else hi = mid;
-        }
+        // rows of K landing in [r0, r1) (one row block of f): precomputed bounds
+        const int32_t* tb = P.sgstart + P.sgoff[K];
+        const int a = tb[blk], lo = tb[blk + 1];
         const T* MvK = S.Mv + P.mvoff1[K] * S.nrhs;
         for (int i = a + r; i < lo; i += 64) {
             const int yr = relK[i] - r0;
