@@ -144,11 +144,17 @@ __device__ __forceinline__ void load_frag(const T* base, int64_t si, int64_t sj,
     const T* q = base + (int64_t)i * si + (int64_t)lk * sj;
     const int64_t step = 4 * sj;
     const bool row_ok = i < rows;
+    // every load unconditional (base, element (0, 0), stands in outside the
+    // tile) and the predicate and sign applied afterwards: a load whose value
+    // is used inside a per-lane branch is waited for inside that branch
+    T v[NB / 4];
 #pragma unroll
     for (int s = 0; s < NB / 4; ++s) {
-        a[s] = (row_ok && 4 * s + lk < cols) ? sgn * *q : (T)0;
+        v[s] = (row_ok && 4 * s + lk < cols) ? *q : *base;
         q += step;
     }
+#pragma unroll
+    for (int s = 0; s < NB / 4; ++s) a[s] = (row_ok && 4 * s + lk < cols) ? sgn * v[s] : (T)0;
 }
 
 // acc += A (registers) * Xs (64 x 16, LDS, leading dimension LDX)
@@ -509,13 +515,21 @@ __device__ __forceinline__ void sep_operands(const T* Pj, int64_t nj, int nn, in
 #pragma unroll
     for (int s = 0; s < SEP_S; ++s)
         b[s] = (base + 4 * s < na && li < nb) ? ldc(B + sidx[base + 4 * s] + (int64_t)(cb + li) * ldb) : (T)0;
+    // loads unconditional (the block's first separator entry stands in), the
+    // predicate and sign afterwards, so that none is waited for on its own
+    const T* anchor = Pj + (int64_t)k0 * nj + nn;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int m = 16 * i + li;
         const T* col = Pj + (int64_t)(k0 + m) * nj + nn + base;
 #pragma unroll
-        for (int s = 0; s < SEP_S; ++s) a[SEP_S * i + s] = (m < kb && base + 4 * s < na) ? -col[4 * s] : (T)0;
+        for (int s = 0; s < SEP_S; ++s) a[SEP_S * i + s] = (m < kb && base + 4 * s < na) ? col[4 * s] : *anchor;
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int s = 0; s < SEP_S; ++s)
+            a[SEP_S * i + s] = (16 * i + li < kb && base + 4 * s < na) ? -a[SEP_S * i + s] : (T)0;
 }
 template <typename T>
 __device__ __forceinline__ void sep_partial(const T* Pj, int nj, int nn, int nsep, int k0, int kb,
