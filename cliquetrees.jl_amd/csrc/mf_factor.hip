@@ -635,9 +635,20 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
     const int r = tid & 63, cg = tid >> 6, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const bool chol = F.diag == 0;
-    for (int c = cg; c < NB; c += 4) {
-        const T* colp = blk + (int64_t)c * nj;
-        A[r + c * LDS] = (r < kb && c < kb && c <= r) ? ldc(colp + r) : (r == c ? (T)1 : (T)0);
+    {
+        // all sixteen loads in flight (outside the tile: the tile's first
+        // element stands in), then the LDS writes
+        T tv[NB / 4];
+#pragma unroll
+        for (int i = 0; i < NB / 4; ++i) {
+            const int c = cg + 4 * i;
+            tv[i] = ldc(blk + ((r < kb && c < kb && c <= r) ? r + (int64_t)c * nj : 0));
+        }
+#pragma unroll
+        for (int i = 0; i < NB / 4; ++i) {
+            const int c = cg + 4 * i;
+            A[r + c * LDS] = (r < kb && c < kb && c <= r) ? tv[i] : (r == c ? (T)1 : (T)0);
+        }
     }
     if (tid < NB) {
         pv[tid] = (T)1;
@@ -856,29 +867,28 @@ __device__ __forceinline__ void gemm_tile(const DevFactor<T>& F, const GemmTask&
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = v4{0, 0, 0, 0};
-    T ra[PER], rb[PER];
+    T ra[PER], rb[PER], rs[PER];
     const int r = tid & 63, kq = tid >> 6;  // element (r, kq + 4i) of each chunk
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int gk = k0 + kq + 4 * i;
-            T av = (T)0, bv = (T)0;
+            T av = (T)0, bv = (T)0, sv = (T)1;
             if (gk < g.k) {
-                if (r < g.m) {
-                    av = ldc(A + r + (int64_t)gk * g.lda);
-                    if (s) av *= ldc(s + gk);
-                }
+                if (r < g.m) av = ldc(A + r + (int64_t)gk * g.lda);
+                if (s) sv = ldc(s + gk);  // applied when staged: no wait here
                 if (r < g.n) bv = ldc(B + r + (int64_t)gk * g.ldb);
             }
             ra[i] = av;
             rb[i] = bv;
+            rs[i] = sv;
         }
     };
     fetch(0);
     for (int k0 = 0; k0 < g.k; k0 += KC) {
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            As[(kq + 4 * i) * LDT + r] = ra[i];
+            As[(kq + 4 * i) * LDT + r] = s ? ra[i] * rs[i] : ra[i];
             Bs[(kq + 4 * i) * LDT + r] = rb[i];
         }
         __syncthreads();
@@ -897,6 +907,12 @@ __device__ __forceinline__ void gemm_tile(const DevFactor<T>& F, const GemmTask&
         __syncthreads();
     }
     const bool scale = (g.mode == G_TRSM) && F.diag != 0;
+    // the LDLt pivots of this thread's two columns, loaded before the stores
+    // (unconditionally, column clamped) rather than one wait per element
+    T dcol[2] = {(T)1, (T)1};
+    if (scale)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) dcol[j] = ldc(F.d + g.s_col + min(wc + j * 16 + li, g.n - 1));
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
@@ -911,7 +927,7 @@ __device__ __forceinline__ void gemm_tile(const DevFactor<T>& F, const GemmTask&
                         stc(cp, cv[i][j][v] - acc[i][j][v]);
                     } else {
                         T x = acc[i][j][v];
-                        if (scale) x /= ldc(F.d + g.s_col + col);
+                        if (scale) x /= dcol[j];
                         stc(cp, x);
                     }
                 }
@@ -947,29 +963,28 @@ __device__ __forceinline__ void gemm_tile128(const DevFactor<T>& F, const GemmTa
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = v4{0, 0, 0, 0};
-    T ra[PER], rb[PER];
+    T ra[PER], rb[PER], rs[PER];
     const int r = tid & 127, kq = tid >> 7;  // element (r, kq + 2i) of each chunk
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int gk = k0 + kq + 2 * i;
-            T av = (T)0, bv = (T)0;
+            T av = (T)0, bv = (T)0, sv = (T)1;
             if (gk < g.k) {
-                if (r < g.m) {
-                    av = ldc(A + r + (int64_t)gk * g.lda);
-                    if (s) av *= ldc(s + gk);
-                }
+                if (r < g.m) av = ldc(A + r + (int64_t)gk * g.lda);
+                if (s) sv = ldc(s + gk);  // applied when staged: no wait here
                 if (r < g.n) bv = ldc(B + r + (int64_t)gk * g.ldb);
             }
             ra[i] = av;
             rb[i] = bv;
+            rs[i] = sv;
         }
     };
     fetch(0);
     for (int k0 = 0; k0 < g.k; k0 += KW) {
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            As[(kq + 2 * i) * LDW + r] = ra[i];
+            As[(kq + 2 * i) * LDW + r] = s ? ra[i] * rs[i] : ra[i];
             Bs[(kq + 2 * i) * LDW + r] = rb[i];
         }
         __syncthreads();
