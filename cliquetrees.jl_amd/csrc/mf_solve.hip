@@ -219,6 +219,54 @@ __device__ __forceinline__ typename Mfma<T>::v4 times_w(const T (&wa)[NB / 4], t
     return x;
 }
 
+// Loads of the small-front paths: every load of a loop issued first, from a
+// clamped (always valid) address, then the LDS updates.  A load whose value
+// is stored or added inside a per-lane branch or loop is otherwise waited
+// for at once, one memory round trip per element.
+
+// Ps[r + c * ldp] = Pj[r + c * nj] for c = c0, c0 + dc, ... < nn (rows r < nj)
+template <typename T, int ldp, int dc>
+__device__ __forceinline__ void load_panel(const T* Pj, int nj, int nn, int r, int c0, T* Ps) {
+    for (int cb = c0; cb < nn; cb += 8 * dc) {
+        T v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int c = cb + dc * i;
+            v[i] = Pj[(r < nj && c < nn) ? r + (int64_t)c * nj : 0];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int c = cb + dc * i;
+            if (r < nj && c < nn) Ps[r + c * ldp] = v[i];
+        }
+    }
+}
+
+// X[xr + c * ldx] (+)= src[c * lds] for c = c0, c0 + dc, ... < nb (RB / dc values)
+template <typename T, int dc, bool add, bool coherent>
+__device__ __forceinline__ void load_rhs(const T* src, int64_t lds, int nb, int c0, bool on, T* X, int xr,
+                                         int ldx) {
+#pragma unroll
+    for (int h = 0; h < RB / dc; h += 8) {
+        constexpr int M = RB / dc < 8 ? RB / dc : 8;
+        T v[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const int c = c0 + dc * (h + i);
+            const T* a = src + (on && c < nb ? c * lds : 0);
+            v[i] = coherent ? ldc(a) : *a;
+        }
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const int c = c0 + dc * (h + i);
+            if (on && c < nb) {
+                if (add) X[xr + c * ldx] += v[i];
+                else X[xr + c * ldx] = v[i];
+            }
+        }
+    }
+}
+
 template <typename T>
 __device__ void small_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S,
                           const Front& f, int J, T* Ps, T* X) {
@@ -226,8 +274,7 @@ __device__ void small_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolv
     const int nn = f.nn, na = f.na, nj = nn + na;
     const bool unit = F.diag != 0;
     const T* Pj = F.panel + f.poff;
-    if (r < nj)
-        for (int c = g; c < nn; c += 4) Ps[r + c * LDS] = Pj[r + c * nj];
+    load_panel<T, LDS, 4>(Pj, nj, nn, r, g, Ps);
     T* MvJ = S.Mv + P.mvoff1[J] * S.nrhs;
     for (int cb = 0; cb < S.nrhs; cb += RB) {
         const int nb = min(RB, S.nrhs - cb);
@@ -237,15 +284,13 @@ __device__ void small_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolv
             const int K = P.chd[q];
             const Front kf = P.fronts[K];
             const int ma = kf.na;
-            if (r < ma) {
-                const int rr = P.rel[kf.reloff + r];
-                const T* MvK = S.Mv + P.mvoff1[K] * S.nrhs;
-                for (int c = g; c < nb; c += 4) X[rr + c * LDS] += ldc(MvK + r + (int64_t)(cb + c) * ma);
-            }
+            const int rr = P.rel[kf.reloff + (r < ma ? r : 0)];
+            const T* MvK = S.Mv + P.mvoff1[K] * S.nrhs;
+            load_rhs<T, 4, true, true>(MvK + (r < ma ? r : 0) + (int64_t)cb * ma, ma, nb, g, r < ma, X, rr, LDS);
             __syncthreads();
         }
-        if (r < nn)
-            for (int c = g; c < nb; c += 4) X[r + c * LDS] += S.B[f.c0 + r + (int64_t)(cb + c) * S.ldb];
+        load_rhs<T, 4, true, false>(S.B + f.c0 + (r < nn ? r : 0) + (int64_t)cb * S.ldb, S.ldb, nb, g, r < nn, X,
+                                    r, LDS);
         __syncthreads();
         for (int k = 0; k < nn; ++k) {
             if (!unit && tid < nb) X[k + tid * LDS] /= Ps[k + k * LDS];
@@ -273,13 +318,12 @@ __device__ void small_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolv
     const int nn = f.nn, na = f.na, nj = nn + na;
     const bool unit = F.diag != 0;
     const T* Pj = F.panel + f.poff;
-    if (r < nj)
-        for (int c = g; c < nn; c += 4) Ps[r + c * LDS] = Pj[r + c * nj];
+    load_panel<T, LDS, 4>(Pj, nj, nn, r, g, Ps);
     const int32_t* rows = P.rows + f.rowoff;
+    const int rowr = rows[r < nj ? r : 0];
     for (int cb = 0; cb < S.nrhs; cb += RB) {
         const int nb = min(RB, S.nrhs - cb);
-        if (r < nj)
-            for (int c = g; c < nb; c += 4) X[r + c * LDS] = ldc(S.B + rows[r] + (int64_t)(cb + c) * S.ldb);
+        load_rhs<T, 4, false, true>(S.B + rowr + (int64_t)cb * S.ldb, S.ldb, nb, g, r < nj, X, r, LDS);
         __syncthreads();
         if (r < nn) {
             for (int c = g; c < nb; c += 4) {
@@ -319,8 +363,7 @@ __device__ void wave_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
     const int nn = f.nn, na = f.na, nj = nn + na;
     const bool unit = F.diag != 0;
     const T* Pj = F.panel + f.poff;
-    if (r < nj)
-        for (int c = 0; c < nn; ++c) Pw[r + c * LDT1] = Pj[r + c * nj];
+    load_panel<T, LDT1, 1>(Pj, nj, nn, r, 0, Pw);
     T* MvJ = S.Mv + P.mvoff1[J] * S.nrhs;
     for (int cb = 0; cb < S.nrhs; cb += RB) {
         const int nb = min(RB, S.nrhs - cb);
@@ -330,14 +373,13 @@ __device__ void wave_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
             const int K = P.chd[q];
             const Front kf = P.fronts[K];
             const int ma = kf.na;
-            if (r < ma) {
-                const int rr = P.rel[kf.reloff + r];
-                const T* MvK = S.Mv + P.mvoff1[K] * S.nrhs;
-                for (int c = 0; c < nb; ++c) Xw[rr + c * TINY_NJ] += ldc(MvK + r + (int64_t)(cb + c) * ma);
-            }
+            const int rr = P.rel[kf.reloff + (r < ma ? r : 0)];
+            const T* MvK = S.Mv + P.mvoff1[K] * S.nrhs;
+            load_rhs<T, 1, true, true>(MvK + (r < ma ? r : 0) + (int64_t)cb * ma, ma, nb, 0, r < ma, Xw, rr,
+                                       TINY_NJ);
         }
-        if (r < nn)
-            for (int c = 0; c < nb; ++c) Xw[r + c * TINY_NJ] += S.B[f.c0 + r + (int64_t)(cb + c) * S.ldb];
+        load_rhs<T, 1, true, false>(S.B + f.c0 + (r < nn ? r : 0) + (int64_t)cb * S.ldb, S.ldb, nb, 0, r < nn, Xw,
+                                    r, TINY_NJ);
         for (int k = 0; k < nn; ++k) {
             if (!unit && r < nb) Xw[k + r * TINY_NJ] /= Pw[k + k * LDT1];
             if (r > k && r < nj) {
@@ -362,13 +404,12 @@ __device__ void wave_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
     const int nn = f.nn, nj = nn + f.na;
     const bool unit = F.diag != 0;
     const T* Pj = F.panel + f.poff;
-    if (r < nj)
-        for (int c = 0; c < nn; ++c) Pw[r + c * LDT1] = Pj[r + c * nj];
+    load_panel<T, LDT1, 1>(Pj, nj, nn, r, 0, Pw);
     const int32_t* rows = P.rows + f.rowoff;
+    const int rowr = rows[r < nj ? r : 0];
     for (int cb = 0; cb < S.nrhs; cb += RB) {
         const int nb = min(RB, S.nrhs - cb);
-        if (r < nj)
-            for (int c = 0; c < nb; ++c) Xw[r + c * TINY_NJ] = ldc(S.B + rows[r] + (int64_t)(cb + c) * S.ldb);
+        load_rhs<T, 1, false, true>(S.B + rowr + (int64_t)cb * S.ldb, S.ldb, nb, 0, r < nj, Xw, r, TINY_NJ);
         if (r < nn) {
             for (int c = 0; c < nb; ++c) {
                 T s = (T)0;
