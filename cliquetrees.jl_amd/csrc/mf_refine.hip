@@ -24,7 +24,24 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, const int64_t* colp
     if (i >= n) return;
     const double* x = X + j * ldx;
     double s = 0.0;
-    for (int64_t p = colptr[i] - 1; p < colptr[i + 1] - 1; ++p) s += val[p] * x[rowval[p] - 1];
+    const int64_t p0 = colptr[i] - 1, p1 = colptr[i + 1] - 1;
+    // eight nonzeros at a time, every load issued before the sums (indices
+    // past the row's end read its first entry); the sum runs in p order
+    for (int64_t pb = p0; pb < p1; pb += 8) {
+        int64_t r[8];
+        double v[8], xv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t pk = pb + k < p1 ? pb + k : p0;
+            r[k] = rowval[pk];
+            v[k] = val[pk];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xv[k] = x[r[k] - 1];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (pb + k < p1) s += v[k] * xv[k];
+    }
     R[i + j * ldr] = B[i + j * ldb] - s;
 }
 
