@@ -365,6 +365,15 @@ __global__ __launch_bounds__(256, 4) void k_small(DevPlan P, DevFactor<T> F, con
 // are fetched while the current one is processed, so a column costs about
 // one memory round trip (its sources' rows) instead of three (records,
 // source records, rows).
+// acc[i] += v in LDS without returning the old value: no wait for a read
+// before the add (a read-add-write per element is one LDS round trip each,
+// since the compiler cannot tell the indices apart).  A wavefront's LDS
+// operations complete in order, so the additions happen in program order.
+template <typename T>
+__device__ __forceinline__ void lds_add(T* p, T v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 template <typename T>
 __device__ __forceinline__ void ea_columns(const DevPlan& P, const DevFactor<T>& F, int e_first, int e_end,
                                            T* acc, int lane) {
@@ -454,7 +463,7 @@ __device__ __forceinline__ void ea_columns(const DevPlan& P, const DevFactor<T>&
                     if (q + 1 < nsb) fetch(q + 1, __builtin_amdgcn_readlane(my_i0, q + 1));
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        if ((cok >> j) & 1u) acc[cr[j] - w0] += cv[j];
+                        if ((cok >> j) & 1u) lds_add(acc + (cr[j] - w0), cv[j]);
                     // rows beyond the first 256 of this source in the window
                     for (int i = i0 + 256; i < i1; i += 256) {
                         const int relo = __builtin_amdgcn_readlane(my_rel, q);
@@ -473,7 +482,7 @@ __device__ __forceinline__ void ea_columns(const DevPlan& P, const DevFactor<T>&
                         }
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            if (i + lane + 64 * j < i1) acc[r2[j] - w0] += v2[j];
+                            if (i + lane + 64 * j < i1) lds_add(acc + (r2[j] - w0), v2[j]);
                     }
                 }
             }
