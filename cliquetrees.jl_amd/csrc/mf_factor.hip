@@ -362,9 +362,7 @@ __global__ __launch_bounds__(256, 4) void k_small(DevPlan P, DevFactor<T> F, con
 // are fetched together with the window bounds, and the next source's rows
 // are fetched while the current source's are added.  A wavefront takes the
 // columns e_first, e_first + 4, ... < e_end; the records of the next column
-// are fetched while the current one is processed, so a column costs about
-// one memory round trip (its sources' rows) instead of three (records,
-// source records, rows).
+// are fetched while the current one is processed.
 // acc[i] += v in LDS without returning the old value: no wait for a read
 // before the add (a read-add-write per element is one LDS round trip each,
 // since the compiler cannot tell the indices apart).  A wavefront's LDS
