@@ -25,7 +25,7 @@ constexpr int NB = 64;          // panel width and GEMM tile edge of big fronts
 constexpr int SMALL_NJ = 64;    // fronts with nJ <= SMALL_NJ use the fused LDS kernel
 constexpr int TINY_NJ = 32;     // small fronts that one wavefront factors (four per ticket)
 constexpr int EA_WIN = 1024;    // parent rows per extend-add window (one wave)
-constexpr int EA_ROWS = 8192;   // parent rows per extend-add task (about)
+constexpr int EA_ROWS = 16384;  // parent rows per extend-add task (about)
 constexpr int WIDE_K = 128;     // update-matrix tiles are 128 x 128 when nn >= WIDE_K
 constexpr int UPD_GROUP = 4;    // panel steps applied by one trailing-update tile
 constexpr int SIM_WORKERS = 512;  // resident workgroups assumed when ordering the DAG tickets

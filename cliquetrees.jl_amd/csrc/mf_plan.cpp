@@ -517,7 +517,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 int32_t J = P.ea[e].front, e1 = e + 1;
                 const bool pan = P.ea[e].col < P.fronts[J].nn;
                 int64_t rows = P.ea[e].row1 - P.ea[e].row0;
-                while (e1 < L.ea1 && e1 - e < 32 && P.ea[e1].front == J && rows < EA_ROWS &&
+                while (e1 < L.ea1 && e1 - e < 64 && P.ea[e1].front == J && rows < EA_ROWS &&
                        (P.ea[e1].col < P.fronts[J].nn) == pan) {
                     rows += P.ea[e1].row1 - P.ea[e1].row0;
                     ++e1;
