@@ -27,7 +27,7 @@ constexpr int TINY_NJ = 32;     // small fronts that one wavefront factors (four
 constexpr int EA_WIN = 1024;    // parent rows per extend-add window (one wave)
 constexpr int EA_ROWS = 16384;  // parent rows per extend-add task (about)
 constexpr int WIDE_K = 128;     // update-matrix tiles are 128 x 128 when nn >= WIDE_K
-constexpr int UPD_GROUP = 4;    // panel steps applied by one trailing-update tile
+constexpr int UPD_GROUP = 8;    // panel steps applied by one trailing-update tile
 constexpr int SIM_WORKERS = 512;  // resident workgroups assumed when ordering the DAG tickets
 
 struct Front {
