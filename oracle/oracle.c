@@ -13,7 +13,7 @@
  *                  chol_factor!      cholesky.jl:694-737
  *                  chol_send!        cholesky.jl:950-983 (addscattertri!, utils.jl:555-579)
  *   potrf_rec      potrf! recursive  blas/potrf.jl:135-178, potrf2! blas/potrf.jl:3-41
- *   regularize     no_regularization.jl:27-42, dynamic_regularization.jl:117-132
+ *   regularize     no_regularization.jl:27-42, dynamic_regularization.jl:75-90
  *   trsm_rlt       trsx2_fwd!(:R)    blas/trsx.jl:11-31 (B <- B A^-T, A lower)
  *   or_div_fwd     div_impl! forward divide.jl:417-450, div_fwd_loop_snd!/nod! 493-821,
  *                  div_fwd_update!   divide.jl:1206-1248
@@ -131,18 +131,28 @@ static void syrk_ln(const double* A, i64 lda, const double* d, double* C, i64 ld
         return;
     }
     if (d && B_gemm && n > 0 && k > 0) {
-        /* W = A diag(d); C_lower -= W A^T, by column blocks of C */
+        /* W = A diag(d); C_lower -= W A^T by column blocks of C: the block
+         * below each diagonal block by GEMM, the diagonal block's lower
+         * triangle by loops (the strict upper triangle stays untouched, as
+         * syr2k!(:L) leaves it, blas/syr2k.jl:4-12) */
         double* W = (double*)malloc(sizeof(double) * (size_t)(n * k));
         for (i64 c = 0; c < k; ++c)
             for (i64 i = 0; i < n; ++i) W[c * n + i] = A_(A, lda, i, c) * d[c];
         const i64 nb = 128;
         for (i64 j0 = 0; j0 < n; j0 += nb) {
             i64 jb = n - j0 < nb ? n - j0 : nb;
-            int mi = (int)(n - j0), ni = (int)jb, ki = (int)k, lw = (int)n, la = (int)lda,
+            for (i64 c = 0; c < k; ++c)
+                for (i64 j = j0; j < j0 + jb; ++j) {
+                    double f = A_(A, lda, j, c);
+                    if (f == 0.0) continue;
+                    for (i64 i = j; i < j0 + jb; ++i) A_(C, ldc, i, j) -= W[c * n + i] * f;
+                }
+            if (n - j0 - jb <= 0) continue;
+            int mi = (int)(n - j0 - jb), ni = (int)jb, ki = (int)k, lw = (int)n, la = (int)lda,
                 lc = (int)ldc;
             double m1 = -1.0, one = 1.0;
-            B_gemm("N", "T", &mi, &ni, &ki, &m1, W + j0, &lw, A + j0, &la, &one,
-                   C + j0 * ldc + j0, &lc);
+            B_gemm("N", "T", &mi, &ni, &ki, &m1, W + j0 + jb, &lw, A + j0, &la, &one,
+                   C + j0 * ldc + j0 + jb, &lc);
         }
         free(W);
         return;
