@@ -7,10 +7,10 @@ The directory name is not a Python identifier; load it with
 from .symbolic import ChordalSymbolic, Permutation, symbolic, flatindices, grid_nd, mmd  # noqa: F401
 from .factorization import (  # noqa: F401
     ChordalCholesky, ChordalLDLt, ChordalFactorization, DynamicRegularization,
-    PosDefException, SingularException, MFError, cholesky_, ldlt_, ldiv_, solve, issuccess, refine,
+    PosDefException, SingularException, MFError, cholesky_, ldlt_, ldiv_, solve, rdiv_, rsolve, issuccess, refine,
     chol_impl, div_impl, device_plan, DevicePlan, DeviceFactor, checkinfo,
     save_factor, load_factor, FACTOR_FORMAT,
-    MF_FP64, MF_FP32, MF_CHOLESKY, MF_LDLT, STAT_NAMES, KC_NAMES,
+    MF_FP64, MF_FP32, MF_CHOLESKY, MF_LDLT, MF_UPLO_L, MF_UPLO_U, STAT_NAMES, KC_NAMES,
 )
 from . import problems  # noqa: F401
 from .batched import BatchedFactorization, BatchFailure, batched_cholesky, batched_ldlt  # noqa: F401

@@ -61,9 +61,11 @@ struct mf_plan {
     int32_t* d_uwait_ptr = nullptr;
     int32_t* d_uwait = nullptr;
     double ea_bytes = 0, small_flops = 0, diag_flops = 0;
-    ~mf_plan() {
-        for (void* p : allocs) (void)hipFree(p);
-    }
+    // reference flops (sum nn^3/3 + na nn^2 + na^2 nn over the reference's own
+    // fronts) of the fronts each launch factors: the small region, the DAG
+    double ref_flops_small = 0, ref_flops_dag = 0;
+    mf_factor* dropin = nullptr;   // FP64 factor object reused by mf_chol_impl / mf_div_impl
+    ~mf_plan();
 };
 
 struct TimedLaunch {
@@ -74,6 +76,8 @@ struct TimedLaunch {
 struct mf_factor {
     mf_plan* plan = nullptr;
     int precision = MF_FP64;
+    int uplo = MF_UPLO_L;       // layout of Dval/Lval at the boundary (:L or :U)
+    double* adiag = nullptr;    // assembled diagonal per permuted column
     hipStream_t own = nullptr, stream = nullptr;
     void* panel = nullptr;
     void* U = nullptr;
@@ -102,7 +106,7 @@ struct mf_factor {
     unsigned long long* trace_buf = nullptr;   // per DAG ticket: start, end
     unsigned long long* strace_buf = nullptr;  // per solve ticket (forward, then backward)
     int64_t* tgt_panel = nullptr;
-    int64_t* tgt_ref = nullptr;
+    int64_t* tgt_diag = nullptr;   // per nonzero: its permuted column if diagonal, else -1
     int64_t nnz = -1;
     double* stage = nullptr;    // staging for host nzval / values
     size_t stage_bytes = 0;
@@ -138,9 +142,16 @@ struct mf_factor {
         if (A_val) (void)hipFree(A_val);
         if (Bws) (void)hipFree(Bws);
         if (stage) (void)hipFree(stage);
+        if (tgt_panel) (void)hipFree(tgt_panel);
+        if (tgt_diag) (void)hipFree(tgt_diag);
         if (own) (void)hipStreamDestroy(own);
     }
 };
+
+mf_plan::~mf_plan() {
+    delete dropin;
+    for (void* p : allocs) (void)hipFree(p);
+}
 
 namespace {
 
@@ -226,7 +237,7 @@ void enqueue_factor(mf_factor* f) {
     launch_fill<unsigned long long>(f->info, 1, ~0ull, s);
     launch_fill<int32_t>(f->flags, std::max<int>(P.nJ, 1), 0, s);
     launch_fill<uint32_t>(f->ticket, 1, 0u, s);
-    timed(f, MF_KC_SMALL, pl->small_flops, H.n_small_region, [&] {
+    timed(f, MF_KC_SMALL, pl->ref_flops_small, H.n_small_region, [&] {
         launch_small<T>(P, F, P.small_order, P.small_units, (int)H.small_units.size(), s);
     });
     if (!H.dag.empty()) {
@@ -237,7 +248,7 @@ void enqueue_factor(mf_factor* f) {
         DevDag D{pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->eacnt + nfr1,
                  f->prof, f->prof ? f->trace_buf : nullptr,
                  (int32_t)H.dag.size()};
-        timed(f, MF_KC_DAG, H.engine_flops - pl->small_flops, 1, [&] { launch_dag<T>(P, F, D, s); });
+        timed(f, MF_KC_DAG, pl->ref_flops_dag, 1, [&] { launch_dag<T>(P, F, D, s); });
     }
 }
 
@@ -249,7 +260,7 @@ void enqueue_copyout(mf_factor* f) {
     const HostPlan& H = f->plan->H;
     DevFactor<T> F = dev_factor<T>(f);
     timed(f, MF_KC_COPYOUT, 8.0 * 2.0 * (double)(H.ndz + H.nlz), H.nfr, [&] {
-        launch_copyout<T>(f->plan->D, F, f->Dref, f->Lref, f->dref, (int)H.nfr, (int)H.n, f->stream);
+        launch_copyout<T>(f->plan->D, F, f->Dref, f->Lref, f->dref, f->uplo, f->adiag, f->stream);
     });
 }
 
@@ -394,6 +405,11 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
             }
     }
     for (const DiagTask& t : H.diag) pl->diag_flops += (double)t.kb * t.kb * t.kb / 3.0;
+    for (const OutFront& o : H.out) {
+        const double a = o.nn, b = o.na, w = a * a * a / 3.0 + b * a * a + b * b * a;
+        if (pl->in_small[o.J]) pl->ref_flops_small += w;
+        else pl->ref_flops_dag += w;
+    }
     *out = pl;
     return MF_OK;
 }
@@ -411,7 +427,8 @@ int mf_plan_stats(const mf_plan* pl, int64_t* st, int64_t nst) {
                    H.w_elems, (int64_t)H.gemm.size(), (int64_t)H.ea.size(),
                    (int64_t)H.diag.size(), (int64_t)llround(H.ref_flops),
                    (int64_t)llround(H.engine_flops), (int64_t)llround(H.gemm_flops), H.ndz,
-                   H.nlz, (int64_t)H.dag.size()};
+                   H.nlz, (int64_t)H.dag.size(), (int64_t)llround(pl->ref_flops_dag),
+                   (int64_t)llround(pl->ref_flops_small)};
     int64_t m = (int64_t)(sizeof(v) / sizeof(v[0]));
     for (int64_t i = 0; i < nst && i < m; ++i) st[i] = v[i];
     return MF_OK;
@@ -435,6 +452,7 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
               alloc(&f->W, es * (size_t)H.w_elems) && alloc(&f->d, es * (size_t)(H.n + 1)) &&
               alloc((void**)&f->Dref, 8 * (size_t)H.ndz) && alloc((void**)&f->Lref, 8 * (size_t)H.nlz) &&
               alloc((void**)&f->dref, 8 * (size_t)(H.n + 1)) &&
+              alloc((void**)&f->adiag, 8 * (size_t)(H.n + 1)) &&
               alloc((void**)&f->signs, 8 * (size_t)(H.n + 1)) &&
               alloc((void**)&f->info, 64) && alloc((void**)&f->flags, 4 * (H.fronts.size() + 1)) &&
               alloc((void**)&f->prog, 4 * (H.fronts.size() + 1)) &&
@@ -493,12 +511,12 @@ int mf_factor_set_values(mf_factor* f, const double* Dval, const double* Lval, i
     timed(f, MF_KC_ASSEMBLE, 8.0 * (double)(H.ndz + H.nlz), 1, [&] {
         if (f->precision == MF_FP32) {
             launch_fill<float>((float*)f->panel, H.panel_elems, 0.0f, f->stream);
-            launch_assemble_ref<float>(f->plan->D, dev_factor<float>(f), f->Dref, f->Lref,
-                                       (int)H.nfr, f->stream);
+            launch_assemble_ref<float>(f->plan->D, dev_factor<float>(f), f->Dref, f->Lref, f->uplo,
+                                       f->adiag, f->stream);
         } else {
             launch_fill<double>((double*)f->panel, H.panel_elems, 0.0, f->stream);
-            launch_assemble_ref<double>(f->plan->D, dev_factor<double>(f), f->Dref, f->Lref,
-                                        (int)H.nfr, f->stream);
+            launch_assemble_ref<double>(f->plan->D, dev_factor<double>(f), f->Dref, f->Lref, f->uplo,
+                                        f->adiag, f->stream);
         }
     });
     HIPCHK(hipGetLastError());
@@ -539,13 +557,12 @@ int mf_factor_load(mf_factor* f, const double* Dval, const double* Lval, const d
 int mf_factor_set_pattern(mf_factor* f, int64_t nnz, const int64_t* flat) {
     if (!f || nnz < 0 || (nnz > 0 && !flat)) return fail(MF_EARG, "bad argument");
     const HostPlan& H = f->plan->H;
-    std::vector<int64_t> tp(nnz, -1), tr(nnz, -1);
+    std::vector<int64_t> tp(nnz, -1), td(nnz, -1);
     for (int64_t k = 0; k < nnz; ++k) {
         int64_t p = flat[k];
         if (p == 0) continue;
         if (p < 0 || p > H.ndz + H.nlz) return fail(MF_EARG, "flat index out of range");
         int64_t q = p - 1;
-        tr[k] = q;
         int64_t j, e, r, c;
         const OutFront* o;
         if (q < H.ndz) {
@@ -554,7 +571,9 @@ int mf_factor_set_pattern(mf_factor* f, int64_t nnz, const int64_t* flat) {
             e = q - H.Dptr[j];
             r = e % o->nn;
             c = e / o->nn;
-            if (r < c) return fail(MF_EARG, "flat index in the upper triangle");
+            if (f->uplo == MF_UPLO_U) std::swap(r, c);   // :U stores the transposed block
+            if (r < c) return fail(MF_EARG, "flat index outside the stored triangle");
+            if (r == c) td[k] = o->r0 + c;
             const Front& F = H.fronts[o->J];
             int64_t cl = o->r0 - F.c0, nj = F.nn + F.na;
             tp[k] = F.poff + (cl + r) + (cl + c) * nj;
@@ -565,21 +584,27 @@ int mf_factor_set_pattern(mf_factor* f, int64_t nnz, const int64_t* flat) {
             while (j > 0 && H.Lptr[j] == H.Lptr[j + 1]) --j;
             o = &H.out[j];
             e = q - H.Lptr[j];
-            r = e % o->na;
-            c = e / o->na;
+            if (f->uplo == MF_UPLO_U) {   // nn x na block above the diagonal block
+                c = e % o->nn;
+                r = e / o->nn;
+            } else {
+                r = e % o->na;
+                c = e / o->na;
+            }
             const Front& F = H.fronts[o->J];
             int64_t cl = o->r0 - F.c0, nj = F.nn + F.na;
             tp[k] = F.poff + H.septpos[o->sepoff + r] + (cl + c) * nj;
         }
     }
     if (f->tgt_panel) (void)hipFree(f->tgt_panel);
-    if (f->tgt_ref) (void)hipFree(f->tgt_ref);
-    f->tgt_panel = f->tgt_ref = nullptr;
+    if (f->tgt_diag) (void)hipFree(f->tgt_diag);
+    f->tgt_panel = f->tgt_diag = nullptr;
+    f->nnz = -1;
     HIPCHK(hipMalloc(&f->tgt_panel, 8 * std::max<int64_t>(nnz, 2)));
-    HIPCHK(hipMalloc(&f->tgt_ref, 8 * std::max<int64_t>(nnz, 2)));
+    HIPCHK(hipMalloc(&f->tgt_diag, 8 * std::max<int64_t>(nnz, 2)));
     if (nnz) {
         HIPCHK(hipMemcpy(f->tgt_panel, tp.data(), 8 * nnz, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(f->tgt_ref, tr.data(), 8 * nnz, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(f->tgt_diag, td.data(), 8 * nnz, hipMemcpyHostToDevice));
     }
     f->nnz = nnz;
     return MF_OK;
@@ -595,17 +620,17 @@ int mf_factor_refresh(mf_factor* f, const double* nzval, int device) {
         HIPCHK(hipMemcpyAsync(f->stage, nzval, 8 * f->nnz, hipMemcpyHostToDevice, f->stream));
         src = f->stage;
     }
-    timed(f, MF_KC_ASSEMBLE, 8.0 * (double)(H.panel_elems + H.ndz + H.nlz) + 24.0 * f->nnz, 1, [&] {
-        launch_fill<double>(f->Dref, H.ndz, 0.0, f->stream);
-        launch_fill<double>(f->Lref, H.nlz, 0.0, f->stream);
+    // fill!(F, 0) + setflatindex! into the panels only: the reference-layout
+    // arrays are written from the panels when a caller asks for them
+    timed(f, MF_KC_ASSEMBLE, (double)elt(f) * (double)H.panel_elems + 24.0 * f->nnz, 1, [&] {
         if (f->precision == MF_FP32) {
             launch_fill<float>((float*)f->panel, H.panel_elems, 0.0f, f->stream);
-            launch_refresh<float>(dev_factor<float>(f), f->tgt_panel, f->tgt_ref, src, f->nnz,
-                                  f->Dref, H.ndz, f->Lref, f->stream);
+            launch_refresh<float>(dev_factor<float>(f), f->tgt_panel, f->tgt_diag, src, f->nnz,
+                                  f->adiag, f->stream);
         } else {
             launch_fill<double>((double*)f->panel, H.panel_elems, 0.0, f->stream);
-            launch_refresh<double>(dev_factor<double>(f), f->tgt_panel, f->tgt_ref, src, f->nnz,
-                                   f->Dref, H.ndz, f->Lref, f->stream);
+            launch_refresh<double>(dev_factor<double>(f), f->tgt_panel, f->tgt_diag, src, f->nnz,
+                                   f->adiag, f->stream);
         }
     });
     if (f->A_val && f->A_nnz == f->nnz && f->nnz > 0)
@@ -613,7 +638,7 @@ int mf_factor_refresh(mf_factor* f, const double* nzval, int device) {
     HIPCHK(hipGetLastError());
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     f->factored = false;
-    f->ref_stale = false;
+    f->ref_stale = true;
     return MF_OK;
 }
 
@@ -911,38 +936,58 @@ int mf_refine(mf_factor* f, const int64_t* perm, const double* B, int64_t ldb, d
                                relres);
 }
 
+// The drop-in entry points keep one FP64 factor object per plan, created on
+// first use, so repeated calls allocate nothing new.
+static int dropin_factor(mf_plan* plan, int uplo, mf_factor** out) {
+    if (!plan) return fail(MF_EARG, "null plan");
+    if (uplo != MF_UPLO_L && uplo != MF_UPLO_U) return fail(MF_EARG, "bad uplo");
+    if (!plan->dropin) {
+        int rc = mf_factor_create(plan, MF_FP64, &plan->dropin);
+        if (rc) return rc;
+    }
+    plan->dropin->uplo = uplo;
+    plan->dropin->stream = plan->dropin->own;
+    *out = plan->dropin;
+    return MF_OK;
+}
+
 int mf_chol_impl(mf_plan* plan, double* Dval, double* Lval, double* d, const double* signs,
-                 int diag, int reg_kind, double delta, double eps, int64_t* info) {
+                 int diag, int uplo, int reg_kind, double delta, double eps, int64_t* info) {
     mf_factor* f = nullptr;
-    int rc = mf_factor_create(plan, MF_FP64, &f);
+    int rc = dropin_factor(plan, uplo, &f);
     if (rc) return rc;
     if (!(rc = mf_factor_set_values(f, Dval, Lval, 0)) &&
         !(rc = mf_factorize(f, diag, signs, reg_kind, delta, eps, info)))
         rc = mf_factor_get(f, Dval, Lval, diag == MF_LDLT ? d : nullptr, 0);
-    mf_factor_destroy(f);
     return rc;
 }
 
 int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const double* d,
-                double* C, int64_t nrhs, int64_t ldc, int diag) {
+                double* C, int64_t nrhs, int64_t ldc, int diag, int uplo) {
+    if (diag != MF_CHOLESKY && diag != MF_LDLT) return fail(MF_EARG, "bad diag");
+    if (diag == MF_LDLT && !d && plan && plan->H.n) return fail(MF_EARG, "LDLt solve without d");
     mf_factor* f = nullptr;
-    int rc = mf_factor_create(plan, MF_FP64, &f);
+    int rc = dropin_factor(plan, uplo, &f);
     if (rc) return rc;
     f->diag = diag;
-    if (!(rc = mf_factor_set_values(f, Dval, Lval, 0))) {
-        if (diag == MF_LDLT && d) {
-            if (hipMemcpy(f->d, d, 8 * plan->H.n, hipMemcpyHostToDevice) != hipSuccess)
-                rc = fail(MF_EHIP, "hipMemcpy d");
-        }
-        // the big-front solves use the inverted 64x64 diagonal blocks
-        if (!rc) {
-            DevFactor<double> F = dev_factor<double>(f);
-            launch_diag<double>(plan->D, F, 0, (int)plan->H.diag.size(), true, f->stream);
-            rc = mf_solve_natural(f, C, nrhs, ldc, 0);
-        }
+    if ((rc = mf_factor_set_values(f, Dval, Lval, 0))) return rc;
+    if (diag == MF_LDLT && plan->H.n)
+        HIPCHK(hipMemcpyAsync(f->d, d, 8 * plan->H.n, hipMemcpyHostToDevice, f->stream));
+    // the big-front solves use the inverted 64x64 diagonal blocks
+    launch_diag<double>(plan->D, dev_factor<double>(f), 0, (int)plan->H.diag.size(), true, f->stream);
+    f->factored = true;
+    f->ref_stale = false;
+    return mf_solve_natural(f, C, nrhs, ldc, 0);
+}
+
+int mf_factor_set_uplo(mf_factor* f, int uplo) {
+    if (!f || (uplo != MF_UPLO_L && uplo != MF_UPLO_U)) return fail(MF_EARG, "bad argument");
+    if (uplo != f->uplo) {
+        f->nnz = -1;           // flat indices registered for the other layout no longer apply
+        f->ref_stale = true;   // the reference arrays are rewritten in the new layout on demand
+        f->uplo = uplo;
     }
-    mf_factor_destroy(f);
-    return rc;
+    return MF_OK;
 }
 
 int mf_timing(mf_factor* f, int enable) {

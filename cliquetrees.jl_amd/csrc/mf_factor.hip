@@ -37,7 +37,7 @@ __device__ __forceinline__ double sign_of(const DevFactor<T>& F, int gcol) {
 }
 
 // regularize(R, S, Djj, j): no_regularization.jl:27-33 and
-// dynamic_regularization.jl:117-123.  0 signals a failed pivot.
+// dynamic_regularization.jl:75-90.  0 signals a failed pivot.
 template <typename T>
 __device__ __forceinline__ double regularize(const DevFactor<T>& F, double s, double Djj) {
     if (F.reg_kind == 1) return (s * Djj < F.eps) ? F.delta * s : Djj;
@@ -1300,55 +1300,78 @@ __global__ void k_fill(T* p, int64_t count, T value) {
     for (; i < count; i += stride) p[i] = value;
 }
 
-// copyto!-layout Dval/Lval (assembled A) -> internal panels (zeroed before);
-// one workgroup per global column.
+// Reference-layout Dval/Lval (assembled A, or a factor) -> internal panels
+// (zeroed before); one workgroup per global column.  uplo 0 (:L): the
+// diagonal block is nn x nn lower and the off-diagonal block na x nn;
+// uplo 1 (:U): both transposed, the block above being nn x na
+// (chordal_triangular.jl:852-926, 1015-1055).  adiag[g] receives the
+// diagonal entry of column g.
 template <typename T>
 __global__ __launch_bounds__(256) void k_assemble_ref(DevPlan P, DevFactor<T> F,
-                                                      const double* Dval, const double* Lval) {
+                                                      const double* Dval, const double* Lval,
+                                                      int uplo, double* adiag) {
     const int g = blockIdx.x;
     const OutFront o = P.out[P.col2ref[g]];
     const Front f = P.fronts[o.J];
     const int nj = f.nn + f.na, c = g - o.r0, cl = o.r0 - f.c0;
     T* col = F.panel + f.poff + (int64_t)(cl + c) * nj;
-    for (int r = c + threadIdx.x; r < o.nn; r += blockDim.x)
-        col[cl + r] = (T)Dval[o.Dp + r + (int64_t)c * o.nn];
-    for (int i = threadIdx.x; i < o.na; i += blockDim.x)
-        col[P.septpos[o.sepoff + i]] = (T)Lval[o.Lp + i + (int64_t)c * o.na];
+    if (uplo == 0) {
+        for (int r = c + threadIdx.x; r < o.nn; r += blockDim.x)
+            col[cl + r] = (T)Dval[o.Dp + r + (int64_t)c * o.nn];
+        for (int i = threadIdx.x; i < o.na; i += blockDim.x)
+            col[P.septpos[o.sepoff + i]] = (T)Lval[o.Lp + i + (int64_t)c * o.na];
+    } else {
+        for (int r = c + threadIdx.x; r < o.nn; r += blockDim.x)
+            col[cl + r] = (T)Dval[o.Dp + c + (int64_t)r * o.nn];
+        for (int i = threadIdx.x; i < o.na; i += blockDim.x)
+            col[P.septpos[o.sepoff + i]] = (T)Lval[o.Lp + c + (int64_t)i * o.nn];
+    }
+    if (threadIdx.x == 0) adiag[g] = Dval[o.Dp + c + (int64_t)c * o.nn];
 }
 
-// setflatindex! for every nonzero: panel slot and reference slot.
+// setflatindex! for every nonzero into the panels (tp: panel slot or -1);
+// td: the permuted column of a diagonal entry or -1 (adiag keeps the
+// assembled diagonal for the LDLt copy-out of nn = 1 fronts).
 template <typename T>
-__global__ void k_refresh(DevFactor<T> F, const int64_t* tp, const int64_t* tr, const double* x,
-                          int64_t nnz, double* Dref, int64_t ndz, double* Lref) {
+__global__ void k_refresh(DevFactor<T> F, const int64_t* tp, const int64_t* td, const double* x,
+                          int64_t nnz, double* adiag) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nnz) return;
     const double v = x[k];
     const int64_t p = tp[k];
     if (p >= 0) F.panel[p] = (T)v;
-    const int64_t q = tr[k];
-    if (q >= 0) {
-        if (q < ndz) Dref[q] = v;
-        else Lref[q - ndz] = v;
-    }
+    const int64_t q = td[k];
+    if (q >= 0) adiag[q] = v;
 }
 
-// Internal panels -> the reference's Dval/Lval/d; one workgroup per global column.
+// Internal panels -> the reference's Dval/Lval/d; one workgroup per global
+// column.  Every entry of the column's blocks is written (zeros outside the
+// stored triangle), so the arrays need no clearing before.  uplo as for
+// k_assemble_ref.
 template <typename T>
 __global__ __launch_bounds__(256) void k_copyout(DevPlan P, DevFactor<T> F, double* Dval,
-                                                 double* Lval, double* dout) {
+                                                 double* Lval, double* dout, int uplo,
+                                                 const double* adiag) {
     const int g = blockIdx.x;
     const OutFront o = P.out[P.col2ref[g]];
     const Front f = P.fronts[o.J];
     const int nj = f.nn + f.na, c = g - o.r0, cl = o.r0 - f.c0;
     const T* col = F.panel + f.poff + (int64_t)(cl + c) * nj;
     const bool keep_diag = (F.diag != 0 && o.nn == 1);  // chol_loop_nod!: Dval keeps A's entry
-    if (Dval && !keep_diag) {
-        for (int r = threadIdx.x; r < o.nn; r += blockDim.x)
-            Dval[o.Dp + r + (int64_t)c * o.nn] = r >= c ? (double)col[cl + r] : 0.0;
+    if (Dval) {
+        for (int r = threadIdx.x; r < o.nn; r += blockDim.x) {
+            double v = r >= c ? (double)col[cl + r] : 0.0;
+            if (r == c && keep_diag) v = adiag[g];
+            if (uplo == 0) Dval[o.Dp + r + (int64_t)c * o.nn] = v;
+            else Dval[o.Dp + c + (int64_t)r * o.nn] = v;
+        }
     }
     if (Lval) {
-        for (int i = threadIdx.x; i < o.na; i += blockDim.x)
-            Lval[o.Lp + i + (int64_t)c * o.na] = (double)col[P.septpos[o.sepoff + i]];
+        for (int i = threadIdx.x; i < o.na; i += blockDim.x) {
+            const double v = (double)col[P.septpos[o.sepoff + i]];
+            if (uplo == 0) Lval[o.Lp + i + (int64_t)c * o.na] = v;
+            else Lval[o.Lp + c + (int64_t)i * o.nn] = v;
+        }
     }
     if (dout && F.diag != 0 && threadIdx.x == 0) dout[g] = (double)F.d[g];
 }
@@ -1369,18 +1392,18 @@ void launch_fill(T* p, int64_t count, T value, hipStream_t s) {
 
 template <typename T>
 void launch_assemble_ref(const DevPlan& P, const DevFactor<T>& F, const double* Dval,
-                         const double* Lval, int nfr, hipStream_t s) {
-    if (nfr <= 0 || P.n <= 0) return;
-    hipLaunchKernelGGL(k_assemble_ref<T>, dim3(P.n), dim3(256), 0, s, P, F, Dval, Lval);
+                         const double* Lval, int uplo, double* adiag, hipStream_t s) {
+    if (P.n <= 0) return;
+    hipLaunchKernelGGL(k_assemble_ref<T>, dim3(P.n), dim3(256), 0, s, P, F, Dval, Lval, uplo,
+                       adiag);
 }
 
 template <typename T>
-void launch_refresh(const DevFactor<T>& F, const int64_t* tp, const int64_t* tr,
-                    const double* nzval, int64_t nnz, double* Dref, int64_t ndz, double* Lref,
-                    hipStream_t s) {
+void launch_refresh(const DevFactor<T>& F, const int64_t* tp, const int64_t* td,
+                    const double* nzval, int64_t nnz, double* adiag, hipStream_t s) {
     if (nnz <= 0) return;
     hipLaunchKernelGGL(k_refresh<T>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s, F, tp,
-                       tr, nzval, nnz, Dref, ndz, Lref);
+                       td, nzval, nnz, adiag);
 }
 
 template <typename T>
@@ -1412,24 +1435,24 @@ void launch_dag(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, hipStr
 
 template <typename T>
 void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, double* Lval,
-                    double* dout, int nfr, int n, hipStream_t s) {
-    if (nfr <= 0 || n <= 0) return;
-    hipLaunchKernelGGL(k_copyout<T>, dim3(n), dim3(256), 0, s, P, F, Dval, Lval, dout);
+                    double* dout, int uplo, const double* adiag, hipStream_t s) {
+    if (P.n <= 0) return;
+    hipLaunchKernelGGL(k_copyout<T>, dim3(P.n), dim3(256), 0, s, P, F, Dval, Lval, dout, uplo,
+                       adiag);
 }
 
 #define MF_INST(T)                                                                              \
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                  \
     template void launch_assemble_ref<T>(const DevPlan&, const DevFactor<T>&, const double*,    \
-                                         const double*, int, hipStream_t);                      \
+                                         const double*, int, double*, hipStream_t);             \
     template void launch_refresh<T>(const DevFactor<T>&, const int64_t*, const int64_t*,        \
-                                    const double*, int64_t, double*, int64_t, double*,          \
-                                    hipStream_t);                                               \
+                                    const double*, int64_t, double*, hipStream_t);              \
     template void launch_small<T>(const DevPlan&, const DevFactor<T>&, const int32_t*,          \
                                   const int32_t*, int, hipStream_t);                            \
     template void launch_diag<T>(const DevPlan&, const DevFactor<T>&, int, int, bool,           \
                                  hipStream_t);                                                 \
     template void launch_copyout<T>(const DevPlan&, const DevFactor<T>&, double*, double*,      \
-                                    double*, int, int, hipStream_t);                            \
+                                    double*, int, const double*, hipStream_t);                  \
     template void launch_dag<T>(const DevPlan&, const DevFactor<T>&, const DevDag&, hipStream_t);
 MF_INST(double)
 MF_INST(float)

@@ -77,13 +77,14 @@ struct DevDag {
 // ---- factorization (mf_factor.hip) ----
 template <typename T>
 void launch_fill(T* p, int64_t count, T value, hipStream_t s);
+// uplo: 0 = :L, 1 = :U layout of Dval/Lval; adiag: assembled diagonal per
+// permuted column (written by assembly and refresh, read by the copy-out).
 template <typename T>
 void launch_assemble_ref(const DevPlan& P, const DevFactor<T>& F, const double* Dval,
-                         const double* Lval, int nfr, hipStream_t s);
+                         const double* Lval, int uplo, double* adiag, hipStream_t s);
 template <typename T>
-void launch_refresh(const DevFactor<T>& F, const int64_t* tgt_panel, const int64_t* tgt_ref,
-                    const double* nzval, int64_t nnz, double* Dref, int64_t ndz, double* Lref,
-                    hipStream_t s);
+void launch_refresh(const DevFactor<T>& F, const int64_t* tgt_panel, const int64_t* tgt_diag,
+                    const double* nzval, int64_t nnz, double* adiag, hipStream_t s);
 template <typename T>
 void launch_small(const DevPlan& P, const DevFactor<T>& F, const int32_t* list, const int32_t* units,
                   int count, hipStream_t s);
@@ -95,7 +96,7 @@ template <typename T>
 void launch_dag(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, hipStream_t s);
 template <typename T>
 void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, double* Lval,
-                    double* dout, int nfr, int n, hipStream_t s);
+                    double* dout, int uplo, const double* adiag, hipStream_t s);
 
 // ---- solves (mf_solve.hip) ----
 template <typename T>

@@ -10,6 +10,7 @@ carry a trailing underscore):
   ``cholesky_(F; check, reg)``   cholesky!  cholesky.jl:80-101
   ``ldlt_(F; signs, reg, check)`` ldlt!     cholesky.jl:181-203
   ``ldiv_(F, B)`` / ``solve(F, B)`` (= ``F \\ B``)  ldiv! divide.jl:201-206
+  ``rdiv_(B, F)`` / ``rsolve(B, F)`` (= ``B / F``)   rdiv! divide.jl:71-74,277-285
   ``issuccess(F)``               abstract_factorization.jl:100-102
   ``chol_impl(S, Dval, Lval, ...)`` / ``div_impl(...)``: the flat-array seam
         (cholesky.jl:361-392, divide.jl:417-450) on host arrays.
@@ -30,11 +31,13 @@ from .symbolic import ChordalSymbolic, Permutation, flatindices, symbolic
 MF_FP64, MF_FP32 = 0, 1
 MF_CHOLESKY, MF_LDLT = 0, 1
 MF_REG_NONE, MF_REG_DYNAMIC = 0, 1
+MF_UPLO_L, MF_UPLO_U = 0, 1
+_UPLO = {"L": MF_UPLO_L, "U": MF_UPLO_U}
 
 STAT_NAMES = ["n", "nfr", "internal_fronts", "small_region", "big_fronts", "big_levels",
               "small_levels", "max_nj", "panel_elems", "u_elems", "w_elems", "gemm_tasks",
               "ea_tasks", "diag_tasks", "ref_flops", "engine_flops", "gemm_flops", "ndz", "nlz",
-              "dag_tasks"]
+              "dag_tasks", "ref_flops_dag", "ref_flops_small"]
 KC_NAMES = ["small", "ea", "diag", "gemm", "copyout", "solve", "assemble", "dag"]
 
 
@@ -73,7 +76,7 @@ def checkinfo(info, diag):
 
 
 class DynamicRegularization:
-    """DynamicRegularization (dynamic_regularization.jl:80-115): pivots with
+    """DynamicRegularization (dynamic_regularization.jl:38-90): pivots with
     sign*Djj < epsilon are replaced by delta*sign; negative parameters take the
     defaults cbrt(eps(T)) and sqrt(eps(T))."""
 
@@ -146,13 +149,15 @@ class DeviceFactor:
 
 
 class ChordalFactorization:
-    """ChordalFactorization{DIAG, :L} (chordal_factorization.jl:1-21) whose
-    values live on the GPU."""
+    """ChordalFactorization{DIAG, UPLO} (chordal_factorization.jl:1-21) whose
+    values live on the GPU.  ``uplo`` ("L" or "U") is the stored triangle of
+    ``Dval``/``Lval``: "U" holds the transposed blocks (U = L',
+    chordal_triangular.jl:895-926, 1015-1055)."""
 
     DIAG = MF_CHOLESKY
 
     def __init__(self, A, P: Permutation | None = None, S: ChordalSymbolic | None = None, *,
-                 alg=None, snd="maximal", precision=MF_FP64):
+                 alg=None, snd="maximal", precision=MF_FP64, uplo="L"):
         A = sp.csc_matrix(A, dtype=np.float64)
         A.sort_indices()
         if P is None or S is None:
@@ -163,9 +168,16 @@ class ChordalFactorization:
         self.info = 0
         self._plan = device_plan(S)
         self._f = DeviceFactor(self._plan, precision)
+        self._set_uplo(uplo)
         self._pattern = None
         self._matrix_key = None
         self.copyto_(A)
+
+    def _set_uplo(self, uplo):
+        if uplo not in _UPLO:
+            raise ValueError("uplo must be 'L' or 'U'")
+        self.uplo = uplo
+        _check(_native.hip().mf_factor_set_uplo(self._f.handle, _UPLO[uplo]))
 
     # copyto!(F, A) with a pattern reuse fast path (flatindices + setflatindex!)
     def copyto_(self, A):
@@ -174,7 +186,7 @@ class ChordalFactorization:
         key = (A.shape, A.indptr.tobytes(), A.indices.tobytes())
         lib = _native.hip()
         if self._pattern != key:
-            flat = flatindices(self.S, A)
+            flat = flatindices(self.S, A, self.uplo)
             _check(lib.mf_factor_set_pattern(self._f.handle, len(flat),
                                              flat.ctypes.data_as(_native.i64p)))
             self._pattern = key
@@ -213,7 +225,7 @@ class ChordalFactorization:
         return self._plan.stats()
 
     @classmethod
-    def from_factor(cls, P: Permutation, S: ChordalSymbolic, Dval, Lval, d=None, info=0):
+    def from_factor(cls, P: Permutation, S: ChordalSymbolic, Dval, Lval, d=None, info=0, uplo="L"):
         """A factorization given by its values, as the reference constructs a
         ChordalFactorization from stored Dval/Lval/d (chordal_factorization.jl:1-21):
         ready for ldiv_/solve without refactoring (mf_factor_load)."""
@@ -224,6 +236,7 @@ class ChordalFactorization:
         F.info = int(info)
         F._plan = device_plan(S)
         F._f = DeviceFactor(F._plan, MF_FP64)
+        F._set_uplo(uplo)
         F._pattern = None
         F._matrix_key = None
         F._A = None
@@ -316,6 +329,28 @@ def solve(F: ChordalFactorization, B):
     return ldiv_(F, X)
 
 
+def rdiv_(B, F: ChordalFactorization):
+    """rdiv!(B, F): B <- B / F in place (B float64, n or m x n).  A is
+    symmetric, so B A^-1 = (A^-1 B')' and the right division is the left
+    solve of the transpose (rdiv!, divide.jl:277-285)."""
+    if not isinstance(B, np.ndarray) or B.dtype != np.float64:
+        raise TypeError("B must be a float64 numpy array")
+    if B.shape[-1] != F.n:
+        raise ValueError("dimension mismatch")
+    if B.ndim == 1:
+        return ldiv_(F, B)
+    T = np.asfortranarray(B.T)
+    ldiv_(F, T)
+    B[...] = T.T
+    return B
+
+
+def rsolve(B, F: ChordalFactorization):
+    """B / F."""
+    X = np.array(B, dtype=np.float64, copy=True)
+    return rdiv_(X, F)
+
+
 def refine(F: ChordalFactorization, B, tol=1e-10, maxit=20, segments=None):
     """X = A \\ B by iterative refinement around F (typically FP32 fronts): the
     residual B - A X and the update are FP64 on the GPU.  Returns X (float64,
@@ -352,8 +387,9 @@ def refine(F: ChordalFactorization, B, tol=1e-10, maxit=20, segments=None):
     return X.reshape(-1) if vec else X
 
 
-def chol_impl(S: ChordalSymbolic, Dval, Lval, diag=MF_CHOLESKY, signs=None, reg=None):
-    """chol_impl! on host arrays through mf_chol_impl: returns (Dval, Lval, d, info)."""
+def chol_impl(S: ChordalSymbolic, Dval, Lval, diag=MF_CHOLESKY, signs=None, reg=None, uplo="L"):
+    """chol_impl! on host arrays through mf_chol_impl: returns (Dval, Lval, d, info).
+    For Cholesky d stays ones (the reference's d is Ones) and no d is passed."""
     Dval = np.array(Dval, dtype=np.float64, copy=True)
     Lval = np.array(Lval, dtype=np.float64, copy=True)
     d = np.ones(S.n) if diag == MF_CHOLESKY else np.zeros(S.n)
@@ -367,14 +403,14 @@ def chol_impl(S: ChordalSymbolic, Dval, Lval, diag=MF_CHOLESKY, signs=None, reg=
         delta, eps = reg.initialize(np.finfo(np.float64).eps)
     info = ctypes.c_int64(0)
     plan = device_plan(S)
+    dp = None if diag == MF_CHOLESKY else _ptr(d)
     _check(_native.hip().mf_chol_impl(plan.handle, Dval.ctypes.data_as(_native.f64p),
-                                      Lval.ctypes.data_as(_native.f64p),
-                                      d.ctypes.data_as(_native.f64p), sp_, diag, kind, delta, eps,
-                                      ctypes.byref(info)))
+                                      Lval.ctypes.data_as(_native.f64p), dp, sp_, diag, _UPLO[uplo],
+                                      kind, delta, eps, ctypes.byref(info)))
     return Dval, Lval, d, int(info.value)
 
 
-def div_impl(S: ChordalSymbolic, Dval, Lval, d, C, diag=MF_CHOLESKY):
+def div_impl(S: ChordalSymbolic, Dval, Lval, d, C, diag=MF_CHOLESKY, uplo="L"):
     """Natural solve C <- L^-T [D^-1] L^-1 C on host arrays (mf_div_impl)."""
     C = np.array(C, dtype=np.float64, order="F", copy=True)
     vec = C.ndim == 1
@@ -389,7 +425,8 @@ def div_impl(S: ChordalSymbolic, Dval, Lval, d, C, diag=MF_CHOLESKY):
     plan = device_plan(S)
     _check(_native.hip().mf_div_impl(plan.handle, Dval.ctypes.data_as(_native.f64p),
                                      Lval.ctypes.data_as(_native.f64p), dp,
-                                     C.ctypes.data_as(_native.f64p), C.shape[1], S.n, diag))
+                                     C.ctypes.data_as(_native.f64p), C.shape[1], S.n, diag,
+                                     _UPLO[uplo]))
     return C[:, 0] if vec else C
 
 
@@ -410,6 +447,7 @@ def save_factor(F: ChordalFactorization, path):
         raise ValueError("the factorization failed; nothing to save")
     S = F.S
     out = {"format": np.array(FACTOR_FORMAT), "diag": np.int64(F.DIAG), "info": np.int64(F.info),
+           "uplo": np.array(F.uplo),
            "perm": np.asarray(F.perm, dtype=np.int64), "invp": np.asarray(F.invp, dtype=np.int64),
            "Dval": F.Dval, "Lval": F.Lval, "d": F.d}
     for k in _SYM_ARRAYS:
@@ -434,5 +472,6 @@ def load_factor(path) -> ChordalFactorization:
             raise ValueError("inconsistent permutation in the factor file")
         diag = int(z["diag"])
         cls = ChordalLDLt if diag == MF_LDLT else ChordalCholesky
-        return cls.from_factor(P, S, z["Dval"], z["Lval"], z["d"], info=int(z["info"]))
+        uplo = str(z["uplo"]) if "uplo" in z.files else "L"
+        return cls.from_factor(P, S, z["Dval"], z["Lval"], z["d"], info=int(z["info"]), uplo=uplo)
 

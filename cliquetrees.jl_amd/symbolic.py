@@ -204,11 +204,27 @@ def symbolic(A, alg=None, snd="maximal"):
     return Permutation.from_perm(arr["perm"]), S
 
 
-def flatindices(S: ChordalSymbolic, A):
+def flatindices(S: ChordalSymbolic, A, uplo="L"):
     """``flatindices(S, A)`` (chordal_symbolic.jl:595-752): 1-based slot of each
-    stored entry of A in [Dval; Lval], 0 if not in the lower triangle.  Indices
-    follow the order of ``A.data`` of the CSC matrix with sorted indices."""
+    stored entry of A in [Dval; Lval], 0 if not in the stored triangle.  Indices
+    follow the order of ``A.data`` of the CSC matrix with sorted indices.
+    ``uplo="U"`` numbers the slots of the :U layout (the transposed blocks),
+    where the stored triangle is the upper one."""
     A, colptr, rowval = _csc(A)
+    if uplo == "U":
+        # the upper entry (i, j) is stored where the :L layout stores (j, i),
+        # at the transposed position inside its block
+        coo = A.tocoo()
+        T = sp.csc_matrix((np.arange(1, A.nnz + 1, dtype=np.float64), (coo.col, coo.row)),
+                          shape=A.shape)
+        T.sort_indices()
+        flatT = flatindices(S, T, "L")
+        src = T.data.astype(np.int64) - 1       # T's k-th entry is A's entry src[k]
+        flat = np.zeros(A.nnz, dtype=np.int64)
+        flat[src] = transpose_slots(S, flatT)
+        return flat
+    if uplo != "L":
+        raise ValueError("uplo must be 'L' or 'U'")
     flat = np.zeros(len(rowval), dtype=np.int64)
     rc = _native.sym().mfs_flatindices(S._handle, A.shape[0],
                                        colptr.ctypes.data_as(_native.i64p),
@@ -217,3 +233,29 @@ def flatindices(S: ChordalSymbolic, A):
     if rc != 0:
         raise ValueError(f"mfs_flatindices failed ({rc})")
     return flat
+
+
+def transpose_slots(S: ChordalSymbolic, flat):
+    """Map 1-based slots of the :L layout to the slots of the same factor
+    entries in the :U layout (0 stays 0); the map is its own inverse only
+    on diagonal blocks, so it is used in the :L -> :U direction."""
+    flat = np.asarray(flat, dtype=np.int64)
+    out = np.zeros_like(flat)
+    nn = np.diff(S.resptr)
+    na = np.diff(S.sepptr)
+    ndz = int(S.Dptr[-1] - 1)
+    q = flat - 1
+    inD = (flat > 0) & (q < ndz)
+    if inD.any():
+        j = np.searchsorted(S.Dptr - 1, q[inD], side="right") - 1
+        e = q[inD] - (S.Dptr[j] - 1)
+        r, c = e % nn[j], e // nn[j]
+        out[inD] = S.Dptr[j] + c + r * nn[j]
+    inL = (flat > 0) & (q >= ndz)
+    if inL.any():
+        ql = q[inL] - ndz
+        j = np.searchsorted(S.Lptr - 1, ql, side="right") - 1
+        e = ql - (S.Lptr[j] - 1)
+        r, c = e % na[j], e // na[j]
+        out[inL] = ndz + S.Lptr[j] + c + r * nn[j]
+    return out

@@ -41,6 +41,7 @@ enum {
 enum { MF_FP64 = 0, MF_FP32 = 1 };          /* precision of the fronts       */
 enum { MF_CHOLESKY = 0, MF_LDLT = 1 };      /* diag = :N / :U                */
 enum { MF_REG_NONE = 0, MF_REG_DYNAMIC = 1 };
+enum { MF_UPLO_L = 0, MF_UPLO_U = 1 };     /* uplo = :L / :U                */
 
 /* Message of the last failing call on this thread. */
 const char* mf_last_error(void);
@@ -59,7 +60,14 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
                    int64_t flags, mf_plan** out);
 int mf_plan_destroy(mf_plan* plan);
 
-/* Plan statistics (see MF_STAT_* in the package's Python mirror). */
+/* Plan statistics, in this order: n, nfr, internal fronts, fronts of the
+ * small region, big fronts, big levels, small levels, max front size,
+ * panel / U / W pool elements, GEMM tasks, extend-add tasks, diagonal-block
+ * tasks, reference flops (sum nn^3/3 + na nn^2 + na^2 nn over the
+ * reference's fronts), engine flops (the same over internal fronts), GEMM
+ * flops, ndz, nlz, DAG tasks, reference flops of the fronts the tile-DAG
+ * launch factors, reference flops of the small region.  nstats may be
+ * smaller than the list. */
 int mf_plan_stats(const mf_plan* plan, int64_t* stats, int64_t nstats);
 
 /*
@@ -70,6 +78,18 @@ int mf_plan_stats(const mf_plan* plan, int64_t* stats, int64_t nstats);
  */
 int mf_factor_create(mf_plan* plan, int precision, mf_factor** out);
 int mf_factor_destroy(mf_factor* f);
+
+/*
+ * Storage triangle of the reference-layout arrays at this factor's boundary
+ * (uplo of ChordalTriangular, cholesky.jl:373): MF_UPLO_L (default) keeps per
+ * front the nn x nn lower block in Dval and the na x nn block below it in
+ * Lval; MF_UPLO_U keeps their transposes, the nn x nn upper block and the
+ * nn x na block to its right (chordal_triangular.jl:895-926, 1015-1055),
+ * i.e. U = L'.  It applies to mf_factor_set_values, mf_factor_load,
+ * mf_factor_get and the flat indices of mf_factor_set_pattern (register the
+ * pattern after choosing the layout; a change drops the registered pattern).
+ */
+int mf_factor_set_uplo(mf_factor* f, int uplo);
 
 /* All work of a factor object is ordered on this stream (a hipStream_t;
  * NULL restores the factor's own stream). */
@@ -112,7 +132,7 @@ int mf_factor_refresh(mf_factor* f, const double* nzval, int device);
  * order (NULL = Ones for Cholesky, Zeros for LDLt), host pointer.
  * reg_kind: MF_REG_NONE (NoRegularization, no_regularization.jl:27-42) or
  * MF_REG_DYNAMIC (DynamicRegularization(delta, eps),
- * dynamic_regularization.jl:117-132).  *info as chol_impl!.  The call
+ * dynamic_regularization.jl:75-90).  *info as chol_impl!.  The call
  * returns once the work is enqueued and *info is known (it synchronizes).
  */
 int mf_factorize(mf_factor* f, int diag, const double* signs, int reg_kind, double delta,
@@ -176,14 +196,19 @@ int mf_refine_steps(mf_factor* f, const int64_t* perm, const double* B, int64_t 
                     double* nrm2);
 
 /*
- * Drop-in forms with chol_impl!'s own arguments: Dval/Lval in (assembled A)
- * and out (factor), d out (LDLt pivots), host memory, one call.
+ * Drop-in forms with chol_impl!'s own arguments (cholesky.jl:361-377):
+ * Dval/Lval in (assembled A) and out (factor), d out (LDLt pivots; NULL is
+ * accepted for Cholesky, whose d is Ones), signs in (NULL = the default),
+ * uplo as for mf_factor_set_uplo, host memory, one call.  The plan keeps one
+ * FP64 factor object for these calls (created on the first), so repeated
+ * calls allocate nothing new; they are not re-entrant on one plan.
  */
 int mf_chol_impl(mf_plan* plan, double* Dval, double* Lval, double* d, const double* signs,
-                 int diag, int reg_kind, double delta, double eps, int64_t* info);
-/* div_impl! forward + (LDLt) diagonal + backward on host arrays. */
+                 int diag, int uplo, int reg_kind, double delta, double eps, int64_t* info);
+/* div_impl! forward + (LDLt) diagonal + backward on host arrays
+ * (divide.jl:417-435); d may be NULL for Cholesky. */
 int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const double* d,
-                double* C, int64_t nrhs, int64_t ldc, int diag);
+                double* C, int64_t nrhs, int64_t ldc, int diag, int uplo);
 
 /* Kernel timing (HIP events around each launch, on the factor's stream).
  * times[k] = total milliseconds, counts[k] = launches, flops[k] = algorithmic

@@ -22,7 +22,7 @@
  *
  * The dense kernels are the reference's own pure-Julia fallbacks restated in
  * C.  For Float64 the reference calls OpenBLAS instead (potrf.jl:87-103,
- * trsx.jl:156-177, syrk.jl:137-159, gemx.jl:3-41); or_set_blas() installs
+ * trsx.jl:156-177, syrk.jl:137-159, gemx.jl:3-41, syr.jl:3-11); or_set_blas() installs
  * host BLAS/LAPACK entry points (Fortran ABI, 32-bit integers) so the same
  * loop can run at BLAS speed for the CPU baseline.  All index arrays are the
  * reference's 1-based Int64 arrays.
@@ -48,7 +48,14 @@ typedef void (*f_dgemm)(const char*, const char*, const int*, const int*, const 
 static f_dpotrf B_potrf = 0;
 static f_dtrsm B_trsm = 0;
 static f_dsyrk B_syrk = 0;
+typedef void (*f_dsyr)(const char*, const int*, const double*, const double*, const int*,
+                       double*, const int*);
 static f_dgemm B_gemm = 0;
+static f_dsyr B_syr = 0;
+
+/* syr (may be NULL) serves the rank-1 update of nn = 1 fronts, as syr! calls
+ * BLAS.syr! for Float64 (blas/syr.jl:3-11). */
+void or_set_blas_syr(void* syr) { B_syr = (f_dsyr)syr; }
 
 void or_set_blas(void* potrf, void* trsm, void* syrk, void* gemm) {
     B_potrf = (f_dpotrf)potrf;
@@ -282,9 +289,14 @@ i64 or_chol_range(i64 n, i64 nfr, const i64* resptr, const i64* sepptr, const i6
                     for (i64 r = c; r < na; ++r) A_(M, na, r, c) = A_(F, nj, 1 + r, 1 + c);
                 for (i64 r = 0; r < na; ++r) L21[r] /= d11;
                 double alpha = diag ? -d11 : -1.0;
-                for (i64 c = 0; c < na; ++c) {
-                    double f = alpha * L21[c];
-                    for (i64 r = c; r < na; ++r) A_(M, na, r, c) += L21[r] * f;
+                if (B_syr) {
+                    int ni = (int)na, one = 1;
+                    B_syr("L", &ni, &alpha, L21, &one, M, &ni);
+                } else {
+                    for (i64 c = 0; c < na; ++c) {
+                        double f = alpha * L21[c];
+                        for (i64 r = c; r < na; ++r) A_(M, na, r, c) += L21[r] * f;
+                    }
                 }
             }
             continue;

@@ -37,6 +37,8 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.or_set_blas.argtypes = [ctypes.c_void_p] * 4
         L.or_set_blas.restype = None
+        L.or_set_blas_syr.argtypes = [ctypes.c_void_p]
+        L.or_set_blas_syr.restype = None
         L.or_chol_impl.argtypes = ([ctypes.c_int64] * 2 + [i64p] * 7 + [f64p] * 4 +
                                    [ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, f64p, i64p, f64p])
@@ -62,15 +64,18 @@ def _capsule_ptr(capsule):
 
 
 def use_host_blas(on=True):
-    """Route the oracle's dense kernels to scipy's LAPACK/BLAS (OpenBLAS, LP64),
-    as the reference does for Float64; ``on=False`` restores the C loops."""
+    """Route the oracle's dense kernels (dpotrf, dtrsm, dsyrk, dgemm and the
+    nn = 1 fronts' dsyr) to scipy's LAPACK/BLAS (OpenBLAS, LP64), as the
+    reference does for Float64; ``on=False`` restores the C loops."""
     if not on:
         lib().or_set_blas(None, None, None, None)
+        lib().or_set_blas_syr(None)
         return
     from scipy.linalg import cython_blas, cython_lapack
     b, l = cython_blas.__pyx_capi__, cython_lapack.__pyx_capi__
     lib().or_set_blas(_capsule_ptr(l["dpotrf"]), _capsule_ptr(b["dtrsm"]),
                       _capsule_ptr(b["dsyrk"]), _capsule_ptr(b["dgemm"]))
+    lib().or_set_blas_syr(_capsule_ptr(b["dsyr"]))
 
 
 def _p(a, t):

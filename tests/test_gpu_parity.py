@@ -221,7 +221,7 @@ def test_pattern_reuse(ct, oracle):
 
 
 def test_dynamic_regularization(ct, oracle):
-    """DynamicRegularization example (dynamic_regularization.jl:52-73)."""
+    """DynamicRegularization example (dynamic_regularization.jl:12-30)."""
     A = np.array([[1, 1, 2], [1, 1, 3], [2, 3, 1]], float)
     P, S = ct.symbolic(A)
     Dv, Lv = oracle.assemble(S, P, A)
@@ -325,10 +325,11 @@ def test_batched_failure_names_instance(ct):
     assert ei.value.instance == 3
 
 
-def test_serving_loop_same_alone_and_in_the_mix(ct):
+def test_serving_loop_same_alone_and_in_the_mix(ct, oracle):
     """Continuous batching of KKT LDLt requests (3 patterns x 6 value updates,
     Poisson arrivals): every result equals the request served alone (one at
-    a time, no batching), the refined solves meet 1e-10 in relative residual,
+    a time, no batching), the refined solves meet 1e-10 in relative residual
+    and agree with the oracle's LDLt solve (same MMD order) and a dense solve,
     the device-reported residuals agree with the host's, and one correction
     improves on the plain drop-in solve."""
     srv = ct.serving
@@ -364,6 +365,12 @@ def test_serving_loop_same_alone_and_in_the_mix(ct):
         x0 = ct.solve(F, r.B)
         res0 = np.linalg.norm(r.K @ x0 - r.B) / np.linalg.norm(r.B)
         assert res <= max(1e-14, res0)
+        Dv, Lv = oracle.assemble(S, P, r.K)
+        D0, L0, d0, info0 = oracle.chol_impl(S, Dv, Lv, diag=1)
+        assert info0 == 0
+        xo = oracle.solve(S, P, D0, L0, d0, r.B, diag=1)
+        assert rel(r.X, xo) <= 1e-8, (r.tag, rel(r.X, xo))
+        assert rel(r.X, np.linalg.solve(r.K.toarray(), r.B)) <= 1e-8
     st = srv.latency_stats(out)
     assert st["p99_ms"] >= st["p50_ms"] > 0
 
