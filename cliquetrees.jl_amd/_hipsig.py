@@ -27,6 +27,7 @@ SIGNATURES = {
     "mf_factor_get": ([vp, vp, vp, vp, ci], ci),
     "mf_solve_natural": ([vp, vp, i64, i64, ci], ci),
     "mf_solve": ([vp, i64p, vp, i64, i64, ci], ci),
+    "mf_solve_noalloc": ([vp, i64p, vp, i64, i64], ci),
     "mf_chol_impl": ([vp, f64p, f64p, vp, vp, ci, ci, ci, cd, cd, i64p], ci),
     "mf_div_impl": ([vp, f64p, f64p, vp, f64p, i64, i64, ci, ci], ci),
     "mf_factor_set_uplo": ([vp, ci], ci),

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,39 @@ int upload(const std::vector<V>& h, V** d, std::vector<void*>& allocs) {
 
 }  // namespace
 
+// Device scratch of mf_solve_noalloc: the permuted right-hand sides and the
+// solves' update vectors, for callers that cannot allocate device memory.
+// Every device holds its own copy for the life of the process.  Host code
+// never uses this name as an address: it takes the device address once per
+// device with hipGetSymbolAddress (scratch_address below).
+static __device__ __align__(256) unsigned char scratch[MF_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (taken on the first call
+// per device, with that device current).
+int scratch_address(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(MF_EARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MF_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MF_OK;
+}
+
+}  // namespace
+
 struct mf_plan {
     HostPlan H;
     DevPlan D{};
@@ -76,6 +110,7 @@ struct TimedLaunch {
 struct mf_factor {
     mf_plan* plan = nullptr;
     int precision = MF_FP64;
+    int device = 0;             // the device current at mf_factor_create
     int uplo = MF_UPLO_L;       // layout of Dval/Lval at the boundary (:L or :U)
     double* adiag = nullptr;    // assembled diagonal per permuted column
     hipStream_t own = nullptr, stream = nullptr;
@@ -271,13 +306,15 @@ void sync_ref(mf_factor* f) {
     f->ref_stale = false;
 }
 
+// mv: the update-vector workspace (mv_elems1 x nrhs elements); NULL takes
+// the factor's own.
 template <typename T>
-void enqueue_solve(mf_factor* f, T* B, int64_t ldb, int nrhs) {
+void enqueue_solve(mf_factor* f, T* B, int64_t ldb, int nrhs, void* mv = nullptr) {
     mf_plan* pl = f->plan;
     const HostPlan& H = pl->H;
     const DevPlan& P = pl->D;
     DevFactor<T> F = dev_factor<T>(f);
-    DevSolve<T> S{B, ldb, (T*)f->Mv, nrhs};
+    DevSolve<T> S{B, ldb, (T*)(mv ? mv : f->Mv), nrhs};
     double bytes = (double)sizeof(T) * (2.0 * (double)(H.ndz + H.nlz) + 4.0 * H.n * nrhs);
     timed(f, MF_KC_SOLVE, bytes, 1, [&] {
         launch_solve<T>(P, F, S, P.nJ, f->flags, f->prog, pl->d_ntf, pl->d_ntb, pl->d_ftask,
@@ -438,9 +475,12 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
     if (!pl || !out || (precision != MF_FP64 && precision != MF_FP32))
         return fail(MF_EARG, "bad argument");
     *out = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(MF_EHIP, "hipGetDevice failed");
     mf_factor* f = new mf_factor();
     f->plan = pl;
     f->precision = precision;
+    f->device = dev;
     const HostPlan& H = pl->H;
     size_t es = precision == MF_FP32 ? 4 : 8;
     auto alloc = [&](void** p, size_t bytes) -> bool {
@@ -796,6 +836,45 @@ int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t
     }
     HIPCHK(hipGetLastError());
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
+    return MF_OK;
+}
+
+int mf_solve_noalloc(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t ldb) {
+    if (!f || !perm || nrhs < 0 || (nrhs > 0 && !B)) return fail(MF_EARG, "bad argument");
+    const HostPlan& H = f->plan->H;
+    if (ldb < H.n) return fail(MF_EARG, "ldb < n");
+    if (nrhs == 0 || H.n == 0) return MF_OK;
+    const size_t es = elt(f);
+    const size_t mv_bytes = (es * (size_t)H.mv_elems1 * (size_t)nrhs + 255) & ~(size_t)255;
+    const size_t need = mv_bytes + es * (size_t)H.n * (size_t)nrhs;
+    if (need > MF_NOALLOC_SCRATCH_BYTES)
+        return fail(MF_EARG, "mf_solve_noalloc: " + std::to_string(need) + " bytes of scratch needed, limit " +
+                                 std::to_string((unsigned long long)MF_NOALLOC_SCRATCH_BYTES));
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != f->device) return fail(MF_EARG, "the factor's device is not the current device");
+    void* base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    void* mv = base;
+    void* bws = (unsigned char*)base + mv_bytes;
+    if (f->perm_host.size() != (size_t)H.n ||
+        std::memcmp(f->perm_host.data(), perm, 8 * H.n) != 0) {
+        f->perm_host.assign(perm, perm + H.n);
+        HIPCHK(hipMemcpyAsync(f->perm_dev, f->perm_host.data(), 8 * H.n, hipMemcpyHostToDevice,
+                              f->stream));
+    }
+    if (f->precision == MF_FP32) {
+        launch_permute<float>(B, ldb, (float*)bws, H.n, f->perm_dev, (int)H.n, (int)nrhs, false,
+                              f->stream);
+        enqueue_solve<float>(f, (float*)bws, H.n, (int)nrhs, mv);
+        launch_permute_back(bws, H.n, true, B, ldb, f->perm_dev, (int)H.n, (int)nrhs, f->stream);
+    } else {
+        launch_permute<double>(B, ldb, (double*)bws, H.n, f->perm_dev, (int)H.n, (int)nrhs, false,
+                               f->stream);
+        enqueue_solve<double>(f, (double*)bws, H.n, (int)nrhs, mv);
+        launch_permute_back(bws, H.n, false, B, ldb, f->perm_dev, (int)H.n, (int)nrhs, f->stream);
+    }
+    HIPCHK(hipGetLastError());
     return MF_OK;
 }
 

@@ -164,6 +164,22 @@ int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t
              int device);
 
 /*
+ * mf_solve without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  B is a
+ * device pointer; perm as for mf_solve.  The permuted right-hand sides and
+ * the update vectors live in a fixed device array of the library
+ * (MF_NOALLOC_SCRATCH_BYTES per device, held until the process ends); a
+ * solve that needs more returns MF_EARG.  The call only enqueues work on the
+ * factor's stream and must be made with the factor's device current.  All
+ * calls on one device share that array: calls that overlap on one GPU (other
+ * streams, other factors, other host threads) race and must be serialized
+ * by the caller.
+ */
+#define MF_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int mf_solve_noalloc(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t ldb);
+
+/*
  * Iterative refinement (mixed precision: FP32 fronts, FP64 residual and
  * update; no counterpart in the reference, which factors in one precision).
  * mf_factor_set_matrix registers A (full symmetric CSC, 1-based, original
