@@ -14,10 +14,17 @@ numeric factorization + a 16-RHS solve, all on the GPU.
   roofline     = the dominant kernel (k_dag, the persistent tile-DAG of the
                  big fronts, FP64 MFMA) timed with HIP events on the
                  engine's stream over the timed region; its algorithmic work
-                 is the engine flop count of the fronts it factors.
+                 is the reference flop count of the fronts it factors (the
+                 reference's own fronts, not the engine's merged ones).
   cpu_baseline = the CPU oracle (restatement of the reference's serial front
-                 loop) with host OpenBLAS on a bounded sample of the same
-                 factorization, timed on the host cores.
+                 loop, dense kernels in host OpenBLAS on all host threads) on
+                 a bounded sample of the same factorization (a subtree with
+                 1/8 of the flops), median of the repetitions; the whole
+                 factorization + solve measured by --workload cpu_reference
+                 on the same host is attached from profiles/.
+  workloads    = the other configurations measured in the same run
+                 (lap7_64_mixed, st27_48, st27_48_mixed, batch4096,
+                 kkt_serving), fewer steps each; --headline-only skips them.
 
 Other workloads (--workload):
   lap7_64_mixed  the lap7_64 matrix with FP32 fronts + FP64 iterative
@@ -33,6 +40,9 @@ Other workloads (--workload):
                  value updates, continuous batching; p50/p99 latency, solves/s
                  against one-at-a-time serving
   copy           a 1 GiB device copy (calibration of the HBM traffic counters)
+  cpu_reference  the CPU reference path (oracle + host OpenBLAS) on the whole
+                 lap7_64 factorization and 16-RHS solve, median / p99 of
+                 at least 5 repetitions (no GPU)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -106,18 +116,47 @@ def workload(ct, name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(ct, A, P, S, frac=0.125, min_s=8.0):
-    """The oracle (restated serial front loop, its dense kernels routed to host
-    OpenBLAS as the reference does for Float64) on a bounded sample of the
-    same factorization: the largest elimination subtree holding at most
-    ``frac`` of the reference flops (a sub-box of the nested dissection),
-    factored repeatedly until ``min_s`` seconds have been timed.  Run with
-    one BLAS thread and with all of them; the faster is reported (many tiny
-    fronts make a multithreaded BLAS slower than a serial one)."""
-    import contextlib
+def host_info():
+    """Host CPU of the run: model name, logical CPUs, and the BLAS threads the
+    oracle uses (all of them)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = ncpu
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max(int(i.get("num_threads", 1)) for i in threadpool_info()) or ncpu
+    except Exception:
+        pass
+    return model, ncpu, threads
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.use_host_blas(True)
+    return oracle
+
+
+def cpu_baseline(ct, A, P, S, frac=0.125, min_s=10.0, min_reps=3):
+    """The oracle (restated serial front loop of chol_impl!, its dense kernels
+    and the nn = 1 rank-1 updates routed to host OpenBLAS with all host
+    threads, as the reference does for Float64) on a bounded sample of the
+    same factorization: the largest elimination subtree holding at most
+    ``frac`` of the reference flops (for a nested-dissection order a sub-box
+    with the whole problem's mix of front sizes), factored until ``min_s``
+    seconds and ``min_reps`` repetitions; median and p99 of the repetitions.
+    The whole factorization and solve are timed by ``--workload cpu_reference``."""
+    oracle = _oracle()
     Dv, Lv = oracle.assemble(S, P, A)
     nn = S.nn().astype(np.float64)
     na = S.na().astype(np.float64)
@@ -130,27 +169,63 @@ def cpu_baseline(ct, A, P, S, frac=0.125, min_s=8.0):
     total = float(fl.sum())
     ok = np.nonzero(sub <= frac * total)[0]
     root = int(ok[np.argmax(sub[ok])]) + 1
-    try:
-        from threadpoolctl import threadpool_info, threadpool_limits
-        all_threads = max(int(i.get("num_threads", 1)) for i in threadpool_info()) or 1
-    except Exception:
-        threadpool_limits, all_threads = None, os.cpu_count() or 1
-    runs = []
-    for nthreads in ([1, all_threads] if threadpool_limits and all_threads > 1 else [all_threads]):
-        ctx = threadpool_limits(limits=nthreads) if threadpool_limits else contextlib.nullcontext()
-        with ctx:
-            t, reps, flops, info = 0.0, 0, 0.0, 0
-            while t < min_s and reps < 50:
-                dt, flops, info = oracle.chol_subtree(S, Dv, Lv, root)
-                t += dt
-                reps += 1
-        runs.append((flops * reps / t, nthreads, reps, t, flops, info))
-    rate, cores, reps, t, flops, info = max(runs)
-    others = ", ".join(f"{r[1]} thread(s): {r[0] / 1e9:.1f} GFLOP/s" for r in runs)
-    return {"value": rate / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+    model, ncpu, threads = host_info()
+    times, flops, info = [], 0.0, 0
+    while sum(times) < min_s or len(times) < min_reps:
+        dt, flops, info = oracle.chol_subtree(S, Dv, Lv, root)
+        times.append(dt)
+        if len(times) >= 50:
+            break
+    med = float(np.median(times))
+    return {"value": flops / med / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "host": model, "nproc": ncpu,
+            "median_s": round(med, 4), "p99_s": round(float(np.percentile(times, 99)), 4),
             "sample": f"subtree of front {root} ({flops / total:.1%} of the factor flops, "
-                      f"{flops:.3e} flops) factored {reps} times in {t:.2f} s, info={info}: oracle.c "
-                      f"serial front loop + OpenBLAS dpotrf/dtrsm/dsyrk/dgemm ({others})"}
+                      f"{flops:.3e} flops) factored {len(times)} times ({sum(times):.1f} s), "
+                      f"info={info}: oracle.c serial front loop + OpenBLAS dpotrf/dtrsm/dsyrk/"
+                      f"dgemm/dsyr with {threads} threads; median per factorization"}
+
+
+def run_cpu_reference(args, ct):
+    """The CPU reference path on the whole default workload: the oracle's
+    factorization of the 64^3 Laplacian (restated chol_impl! + host OpenBLAS
+    on all host threads) and its 16-RHS solve (div_impl! + permutations),
+    1 warm-up then ``--steps`` repetitions (at least 5), median and p99 of
+    each.  Symbolic analysis and assembly are excluded."""
+    W = workload(ct, "lap7_64")
+    A = W["A"]
+    P, S = ct.symbolic(A, alg=W["order"])
+    oracle = _oracle()
+    Dv, Lv = oracle.assemble(S, P, A)
+    B0 = np.random.default_rng(W["seed"]).standard_normal((A.shape[0], W["nrhs"]))
+    model, ncpu, threads = host_info()
+    reps = max(5, args.steps)
+    tf, ts, res = [], [], None
+    for k in range(reps + 1):
+        t0 = time.perf_counter()
+        D, L, d, info = oracle.chol_impl(S, Dv, Lv)
+        t1 = time.perf_counter()
+        X = oracle.solve(S, P, D, L, d, B0)
+        t2 = time.perf_counter()
+        log(f"cpu_reference rep {k}: factor {t1 - t0:.2f} s, solve {t2 - t1:.2f} s, info {info}")
+        if k > 0:
+            tf.append(t1 - t0)
+            ts.append(t2 - t1)
+        res = float(np.linalg.norm(A @ X - B0) / np.linalg.norm(B0))
+    flops = S.factor_flops()
+    mf, ms = float(np.median(tf)), float(np.median(ts))
+    return {"metric": "CPU reference path: factor TFLOP/s and 16-RHS solve time",
+            "value": round(flops / mf / 1e12, 5), "unit": "TFLOP/s", "n_gpus": 0,
+            "steps": reps, "warmup": 1, "ms_per_step": round((mf + ms) * 1e3, 1),
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": W["desc"], "n": int(A.shape[0]), "ref_flops": flops},
+            "factor_median_s": round(mf, 3), "factor_p99_s": round(float(np.percentile(tf, 99)), 3),
+            "solve_median_s": round(ms, 3), "solve_p99_s": round(float(np.percentile(ts, 99)), 3),
+            "rel_residual": res, "cores": threads, "nproc": ncpu, "host": model,
+            "kind": "port",
+            "path": "oracle.c (restated chol_impl!/div_impl! serial front loops) + OpenBLAS "
+                    "dpotrf/dtrsm/dsyrk/dgemm/dsyr through scipy"}
 
 
 class Engine:
@@ -369,7 +444,38 @@ def run_single(args, ct, torch, name):
     if not args.no_cpu and not fp32:
         log("CPU baseline (oracle + host BLAS on a subtree sample)")
         out["cpu_baseline"] = cpu_baseline(ct, A, P, S)
+        full = cpu_reference_from_profile(W["tag"])
+        if full:
+            out["cpu_baseline"]["whole_factor_and_solve"] = full
         log("CPU baseline done")
+    return out
+
+
+def cpu_reference_from_profile(tag):
+    """The whole-workload CPU reference measurement committed under profiles/
+    (``bench.py --workload cpu_reference`` on the GPU box's host), or None."""
+    path = os.path.join(ROOT, "profiles", f"cpu_reference_{tag}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("json:"):
+            d = json.loads(line.split(":", 1)[1])
+            keep = ["factor_median_s", "factor_p99_s", "solve_median_s", "solve_p99_s", "value",
+                    "unit", "cores", "nproc", "host", "steps"]
+            out = {k: d[k] for k in keep if k in d}
+            out["source"] = os.path.relpath(path, ROOT)
+            return out
+    return None
+
+
+def compact(d):
+    """The figures of a sub-workload's line that the headline line carries."""
+    keep = ["value", "unit", "ms_per_step", "p50_ms", "p99_ms", "dtype", "speedup_vs_plain_fp64",
+            "error_vs_fp64", "rel_residual", "refinement", "plain_fp64", "factor_ms_per_step",
+            "solve_ms_per_step", "roofline", "fp64", "fp32_ir", "one_at_a_time",
+            "worst_rel_residual", "max_rel_diff_batched_vs_alone", "kernel_ms_per_step"]
+    out = {k: d[k] for k in keep if k in d}
+    out["workload"] = d["config"]["workload"]
     return out
 
 
@@ -520,6 +626,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="lap7_64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="lap7_64 alone (without the other workloads in the same line)")
     ap.add_argument("--dag-profile", action="store_true",
                     help="per task kind wait/work time of the tile-DAG kernel (stderr)")
     ap.add_argument("--trace-out", default=None,
@@ -530,16 +638,38 @@ def main():
               f"on a single MI355X and has no multi-GPU path)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     ct = load_package()
+    if args.workload == "cpu_reference":
+        print(json.dumps(run_cpu_reference(args, ct)))
+        return
+    import torch
     if args.workload == "copy":
         out = run_copy(args, ct, torch)
     elif args.workload == "batch4096":
         out = run_batched(args, ct, torch)
     elif args.workload == "kkt_serving":
         out = run_serving(args, ct, torch)
+
     else:
         out = run_single(args, ct, torch, args.workload)
+        if args.workload == "lap7_64" and not args.headline_only:
+            # the other configurations, measured in the same run (fewer steps)
+            sub = argparse.Namespace(**vars(args))
+            sub.steps, sub.warmup, sub.no_cpu = max(3, args.steps // 2), 1, True
+            sub.dag_profile, sub.trace_out = False, None
+            out["workloads"] = {}
+            for name in ["lap7_64_mixed", "st27_48", "st27_48_mixed", "batch4096", "kkt_serving"]:
+                t0 = time.perf_counter()
+                log(f"sub-workload {name}")
+                if name == "batch4096":
+                    r = run_batched(sub, ct, torch)
+                elif name == "kkt_serving":
+                    r = run_serving(sub, ct, torch)
+                else:
+                    r = run_single(sub, ct, torch, name)
+                out["workloads"][name] = compact(r)
+                out["workloads"][name]["wall_s"] = round(time.perf_counter() - t0, 1)
+                torch.cuda.empty_cache()
     print(json.dumps(out))
 
 

@@ -12,9 +12,12 @@ SIGNATURES = {
     "mf_last_error": ([], ctypes.c_char_p),
     "mf_plan_create": ([i64, i64] + [i64p] * 8 + [i64, ctypes.POINTER(vp)], ci),
     "mf_plan_destroy": ([vp], ci),
+    "mf_plan_create_tuned": ([i64, i64] + [i64p] * 8 + [i64p, i64, ctypes.POINTER(vp)], ci),
+    "mf_plan_tuning": ([vp, i64p, i64], ci),
     "mf_plan_stats": ([vp, i64p, i64], ci),
     "mf_factor_create": ([vp, ci, ctypes.POINTER(vp)], ci),
     "mf_factor_destroy": ([vp], ci),
+    "mf_factor_bytes": ([vp, i64p], ci),
     "mf_factor_set_stream": ([vp, vp], ci),
     "mf_factor_sync": ([vp], ci),
     "mf_factor_set_values": ([vp, vp, vp, ci], ci),

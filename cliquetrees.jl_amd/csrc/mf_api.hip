@@ -112,6 +112,7 @@ struct mf_factor {
     int precision = MF_FP64;
     int device = 0;             // the device current at mf_factor_create
     int uplo = MF_UPLO_L;       // layout of Dval/Lval at the boundary (:L or :U)
+    unsigned long long stall = 0;  // wall-clock ticks of the bounded waits' watchdog (10 s)
     double* adiag = nullptr;    // assembled diagonal per permuted column
     hipStream_t own = nullptr, stream = nullptr;
     void* panel = nullptr;
@@ -163,6 +164,7 @@ struct mf_factor {
     int64_t t_cnt[MF_KC_COUNT] = {0};
     double t_work[MF_KC_COUNT] = {0};
     std::vector<void*> allocs;
+    size_t alloc_bytes = 0;     // bytes of the allocations in allocs
     ~mf_factor() {
         for (auto& t : pending) {
             (void)hipEventDestroy(t.a);
@@ -244,6 +246,7 @@ DevFactor<T> dev_factor(mf_factor* f) {
     F.flags = f->flags;
     F.ticket = f->ticket;
     F.hang = f->hang;
+    F.stall = f->stall;
     F.diag = f->diag;
     F.reg_kind = f->reg_kind;
     F.delta = f->delta;
@@ -282,7 +285,7 @@ void enqueue_factor(mf_factor* f) {
         launch_fill<uint32_t>(f->ticket, 1, 0u, s);
         DevDag D{pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->eacnt + nfr1,
                  f->prof, f->prof ? f->trace_buf : nullptr,
-                 (int32_t)H.dag.size()};
+                 (int32_t)H.dag.size(), H.tune.prio};
         timed(f, MF_KC_DAG, pl->ref_flops_dag, 1, [&] { launch_dag<T>(P, F, D, s); });
     }
 }
@@ -342,17 +345,16 @@ extern "C" {
 
 const char* mf_last_error(void) { return g_err.c_str(); }
 
-int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
-                   const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
-                   const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
-                   int64_t flags, mf_plan** out) {
-    (void)flags;
+static int plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                       const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                       const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
+                       const PlanTuning& tune, mf_plan** out) {
     if (!out || !resptr || !sepptr || !chdptr || !Dptr || !Lptr || (nfr > 0 && !chdtgt))
         return fail(MF_EARG, "null argument");
     *out = nullptr;
     mf_plan* pl = new mf_plan();
     std::string err = build_plan(pl->H, n, nfr, resptr, sepptr, septgt, reltgt, chdptr, chdtgt,
-                                 Dptr, Lptr, 0.05);
+                                 Dptr, Lptr, tune);
     if (!err.empty()) {
         delete pl;
         return fail(MF_EARG, "plan: " + err);
@@ -451,6 +453,35 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
     return MF_OK;
 }
 
+int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                   const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                   const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
+                   int64_t flags, mf_plan** out) {
+    (void)flags;
+    return plan_create(n, nfr, resptr, sepptr, septgt, reltgt, chdptr, chdtgt, Dptr, Lptr,
+                       PlanTuning(), out);
+}
+
+int mf_plan_create_tuned(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                         const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                         const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
+                         const int64_t* tuning, int64_t ntuning, mf_plan** out) {
+    PlanTuning t;
+    std::string err = tuning_from(tuning, ntuning, &t);
+    if (!err.empty()) return fail(MF_EARG, "tuning: " + err);
+    return plan_create(n, nfr, resptr, sepptr, septgt, reltgt, chdptr, chdtgt, Dptr, Lptr, t, out);
+}
+
+int mf_plan_tuning(const mf_plan* pl, int64_t* tuning, int64_t ntuning) {
+    if (!pl || !tuning) return fail(MF_EARG, "null argument");
+    const PlanTuning& t = pl->H.tune;
+    const int64_t v[] = {t.upd_group, t.ea_rows, t.wide_k, t.sim_workers,
+                         (int64_t)llround(t.zero_frac * 1e6), t.prio,
+                         (int64_t)llround(pl->H.sim_makespan_us)};
+    for (int64_t i = 0; i < ntuning && i < (int64_t)(sizeof(v) / sizeof(v[0])); ++i) tuning[i] = v[i];
+    return MF_OK;
+}
+
 int mf_plan_destroy(mf_plan* plan) {
     delete plan;
     return MF_OK;
@@ -465,7 +496,7 @@ int mf_plan_stats(const mf_plan* pl, int64_t* st, int64_t nst) {
                    (int64_t)H.diag.size(), (int64_t)llround(H.ref_flops),
                    (int64_t)llround(H.engine_flops), (int64_t)llround(H.gemm_flops), H.ndz,
                    H.nlz, (int64_t)H.dag.size(), (int64_t)llround(pl->ref_flops_dag),
-                   (int64_t)llround(pl->ref_flops_small)};
+                   (int64_t)llround(pl->ref_flops_small), H.mv_elems1};
     int64_t m = (int64_t)(sizeof(v) / sizeof(v[0]));
     for (int64_t i = 0; i < nst && i < m; ++i) st[i] = v[i];
     return MF_OK;
@@ -481,11 +512,18 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
     f->plan = pl;
     f->precision = precision;
     f->device = dev;
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            khz = 100000;
+        f->stall = (unsigned long long)khz * 1000ull * 10ull;
+    }
     const HostPlan& H = pl->H;
     size_t es = precision == MF_FP32 ? 4 : 8;
     auto alloc = [&](void** p, size_t bytes) -> bool {
         if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
         f->allocs.push_back(*p);
+        f->alloc_bytes += std::max<size_t>(bytes, 16);
         return true;
     };
     bool ok = alloc(&f->panel, es * (size_t)H.panel_elems) && alloc(&f->U, es * (size_t)H.u_elems) &&
@@ -1057,6 +1095,15 @@ int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const dou
     f->factored = true;
     f->ref_stale = false;
     return mf_solve_natural(f, C, nrhs, ldc, 0);
+}
+
+int mf_factor_bytes(const mf_factor* f, int64_t* bytes) {
+    if (!f || !bytes) return fail(MF_EARG, "null argument");
+    size_t b = f->alloc_bytes + f->Mv_bytes + f->Bws_bytes + f->ir_ws_bytes + f->stage_bytes;
+    if (f->A_n >= 0) b += 8 * (size_t)(f->A_n + 1) + 16 * (size_t)std::max<int64_t>(f->A_nnz, 1);
+    if (f->nnz >= 0) b += 16 * (size_t)std::max<int64_t>(f->nnz, 2);
+    *bytes = (int64_t)b;
+    return MF_OK;
 }
 
 int mf_factor_set_uplo(mf_factor* f, int uplo) {

@@ -25,6 +25,7 @@
 
 #include "mf_kernels.h"
 #include "mf_mfma.h"
+#include "mf_sync.h"
 
 namespace mf {
 
@@ -51,21 +52,10 @@ __device__ __forceinline__ double regularize(const DevFactor<T>& F, double s, do
 // follows all of the task's data.
 __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0); }
 
-// Bounded wait until *flag >= target; gives up (and records it) after ~1 s.
-__device__ __forceinline__ void wait_ge(int32_t* flag, int32_t target, int32_t* hang) {
-    int spins = 0;
-    // poll with relaxed loads; the data the flag guards is read with
-    // device-coherent loads afterwards
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++spins > (1 << 24)) {
-            atomicExch(hang, 1);
-            break;
-        }
-        // once any wait has given up, every other wait returns at once
-        if ((spins & 255) == 0 && __hip_atomic_load(hang, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            break;
-        __builtin_amdgcn_s_sleep(2);
-    }
+// Bounded wait until *flag >= target (the watchdog of mf_sync.h).
+template <typename T>
+__device__ __forceinline__ void wait_ge(int32_t* flag, int32_t target, const DevFactor<T>& F) {
+    (void)bounded_wait(flag, target, F.hang, F.ticket, F.stall, 2);
 }
 
 __device__ __forceinline__ int ld_rel(const int32_t* p) {
@@ -280,19 +270,6 @@ __device__ __forceinline__ void wave_front(const DevPlan& P, const DevFactor<T>&
     }
 }
 
-// Bounded wait of one lane until *flag >= target (see wait_ge).
-__device__ __forceinline__ void lane_wait_ge(int32_t* flag, int32_t target, int32_t* hang) {
-    int spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++spins > (1 << 24)) {
-            atomicExch(hang, 1);
-            break;
-        }
-        if ((spins & 255) == 0 && __hip_atomic_load(hang, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        __builtin_amdgcn_s_sleep(2);
-    }
-}
-
 // Persistent walk of the small region.  Tickets are units of the planner's
 // children-first order: one front of nJ > TINY_NJ (the whole workgroup), or
 // up to four consecutive tiny fronts (one per wavefront, each waiting for
@@ -321,7 +298,7 @@ __global__ __launch_bounds__(256, 4) void k_small(DevPlan P, DevFactor<T> F, con
             if (tid == 0) {
                 s_fail = 0;
                 const Front f = P.fronts[J];
-                for (int q = f.chd0; q < f.chd1; ++q) wait_ge(F.flags + P.chd[q], 1, F.hang);
+                for (int q = f.chd0; q < f.chd1; ++q) wait_ge(F.flags + P.chd[q], 1, F);
             }
             __syncthreads();
             small_front<T>(P, F, J, M, &s_piv, &s_inv, &s_fail, tid);
@@ -334,7 +311,7 @@ __global__ __launch_bounds__(256, 4) void k_small(DevPlan P, DevFactor<T> F, con
                 const int J = list[start + wave];
                 const Front f = P.fronts[J];
                 if (lane == 0)
-                    for (int q = f.chd0; q < f.chd1; ++q) lane_wait_ge(F.flags + P.chd[q], 1, F.hang);
+                    for (int q = f.chd0; q < f.chd1; ++q) wait_ge(F.flags + P.chd[q], 1, F);
                 // the children's data is read after the wait (device-coherent loads)
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1160,6 +1137,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
         if (tid < 64) {
             const int slot = tid / WDEP, q = tid & (WDEP - 1);
             int sel = -1;
+            Watch watch;
             for (int spins = 0;; ++spins) {
                 const int t = s_win[slot];
                 if (t != my_t) {  // a new ticket in this slot
@@ -1196,11 +1174,13 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                         }
                     }
                     sel = any ? best : -2;
-                    if (sel == -1) {
-                        if (spins > (1 << 24)) {
-                            atomicExch(F.hang, 1);
+                    // the watchdog of mf_sync.h: progress is a satisfied
+                    // dependency more or a ticket taken anywhere in the launch
+                    if (sel == -1 && (spins & 255) == 255) {
+                        if (ld_rel(F.hang)) {
                             sel = -2;
-                        } else if ((spins & 255) == 255 && ld_rel(F.hang)) {
+                        } else if (watch.expired(__popcll(okm), F.ticket, F.stall)) {
+                            atomicExch(F.hang, 1);
                             sel = -2;
                         }
                     }
@@ -1228,6 +1208,11 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
         int32_t* cnt = D.cnt + g.toff;
         unsigned long long t_begin = 0;
         if (D.prof && tid == 0) t_begin = wall_clock64();
+        // critical-chain tasks may issue ahead of the co-resident workgroup's
+        // matrix-core waves (wave priority; uniform branch)
+        const bool raise = ((D.prio >> d.kind) & 1) ||
+                           (((D.prio >> DT_KINDS) & 1) && d.kind == DT_UPD && d.c == (d.d >> 16));
+        if (raise) __builtin_amdgcn_s_setprio(3);
         switch (d.kind) {
             case DT_EA: {
                 const int wave = tid >> 6, lane = tid & 63;
@@ -1247,6 +1232,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                 break;
             }
         }
+        if (raise) __builtin_amdgcn_s_setprio(0);
         drain_stores();
         __syncthreads();
         if (tid == 0) {

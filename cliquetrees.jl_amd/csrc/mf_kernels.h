@@ -54,6 +54,7 @@ struct DevFactor {
     int32_t* flags;          // per internal front, persistent kernels
     uint32_t* ticket;
     int32_t* hang;           // set when a bounded wait gives up
+    unsigned long long stall;  // wall-clock ticks without progress before a wait gives up
     int diag;                // 0 Cholesky, 1 LDLt
     int reg_kind;
     double delta, eps;
@@ -72,6 +73,8 @@ struct DevDag {
     unsigned long long* prof;  // NULL, or per task kind: wait, body (wall-clock ticks), count
     unsigned long long* trace; // NULL, or per ticket: start | workgroup << 48, end (wall clock)
     int32_t ntask;
+    int32_t prio;      // bit k: tasks of kind k run at raised wave priority (bit DT_KINDS:
+                       // the look-ahead trailing update of the panel chain)
 };
 
 // ---- factorization (mf_factor.hip) ----

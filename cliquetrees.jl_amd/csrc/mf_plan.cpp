@@ -13,6 +13,7 @@
 #include "mf_plan.h"
 
 #include <algorithm>
+#include <cmath>
 #include <deque>
 #include <functional>
 #include <map>
@@ -74,8 +75,12 @@ inline int64_t rup(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* resptr,
                        const int64_t* sepptr, const int64_t* septgt, const int64_t* reltgt,
                        const int64_t* chdptr, const int64_t* chdtgt, const int64_t* Dptr,
-                       const int64_t* Lptr, double zero_frac) {
+                       const int64_t* Lptr, const PlanTuning& tune) {
     (void)reltgt;
+    P.tune = tune;
+    const double zero_frac = tune.zero_frac;
+    const int32_t UPD_GROUP = tune.upd_group, EA_ROWS = tune.ea_rows, WIDE_K = tune.wide_k,
+                  SIM_WORKERS = tune.sim_workers;
     if (n < 0 || nfr < 0) return "negative size";
     if (n > (int64_t)INT32_MAX / 2) return "n too large";
     P.n = n;
@@ -1047,9 +1052,12 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     const int32_t rows[4] = {d.b, d.c, (d.d & 1) ? d.b + 1 : d.b, (d.d & 2) ? d.c + 1 : d.c};
                     for (int32_t I : rows)
                         if (!edge(fin[tile(J, I, p - 1)], t)) return "dataflow: syrk without its rows";
+                    // the front's update-matrix extend-add groups: one event per
+                    // front (all groups done) instead of an edge per (group, tile)
+                    // pair, which would be quadratic in the front size
                     for (int32_t e = P.gtask0[J]; e < P.gtask1[J]; ++e)
-                        if (P.dag[e].kind == DT_EA && !P.dag[e].c && !edge(e, t))
-                            return "dataflow: syrk without its extend-add";
+                        if (P.dag[e].kind == DT_EA && !P.dag[e].c && e >= t)
+                            return "dataflow: syrk before its extend-add";
                     break;
                 }
                 default: break;
@@ -1063,6 +1071,26 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             P.dep0[e.first]++;
             nsucc[e.second + 1]++;
         }
+        // the update-matrix extend-add event of each front and its waiters
+        P.n_eau.assign(NJ, 0);
+        P.eau_ptr.assign(NJ + 1, 0);
+        for (int32_t t = 0; t < NT; ++t) {
+            const DagTask& d = P.dag[t];
+            if (d.kind == DT_EA && !d.c) P.n_eau[d.front]++;
+        }
+        for (int32_t t = 0; t < NT; ++t)
+            if (P.dag[t].kind == DT_SYRK && P.n_eau[P.dag[t].front] > 0) P.eau_ptr[P.dag[t].front + 1]++;
+        for (int32_t J = 0; J < NJ; ++J) P.eau_ptr[J + 1] += P.eau_ptr[J];
+        P.eau_wait.assign(P.eau_ptr[NJ], 0);
+        {
+            std::vector<int32_t> pos(P.eau_ptr.begin(), P.eau_ptr.end() - 1);
+            for (int32_t t = 0; t < NT; ++t)
+                if (P.dag[t].kind == DT_SYRK && P.n_eau[P.dag[t].front] > 0) {
+                    P.eau_wait[pos[P.dag[t].front]++] = t;
+                    P.dep0[t]++;
+                }
+        }
+        if (edges.size() > (size_t)INT32_MAX) return "dataflow: too many edges";
         P.succ_ptr.assign(NT + 1, 0);
         for (int32_t t = 0; t < NT; ++t) P.succ_ptr[t + 1] = P.succ_ptr[t] + nsucc[t + 1];
         P.succ.assign(edges.size(), 0);
@@ -1111,7 +1139,8 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             if (ready) (P.hi[t] ? P.init_hi : P.init_lo).push_back(t);
         }
         // host run of the dataflow (the kernel's completion rules)
-        std::vector<int32_t> dep(P.dep0), gate(P.gate0), eacnt(NJ, 0), done(NJ, 0), seen(NT, 0);
+        std::vector<int32_t> dep(P.dep0), gate(P.gate0), eacnt(NJ, 0), done(NJ, 0), seen(NT, 0),
+            eaucnt(NJ, 0);
         std::deque<int32_t> qh(P.init_hi.begin(), P.init_hi.end()), ql(P.init_lo.begin(), P.init_lo.end());
         int64_t executed = 0;
         auto push = [&](int32_t s) { (P.hi[s] ? qh : ql).push_back(s); };
@@ -1141,6 +1170,9 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             const DagTask& d = P.dag[t];
             if (d.kind == DT_EA) {
                 if (++eacnt[d.front] == P.dagf[d.front].n_ea) consumed(d.front);
+                if (!d.c && ++eaucnt[d.front] == P.n_eau[d.front])
+                    for (int32_t q = P.eau_ptr[d.front]; q < P.eau_ptr[d.front + 1]; ++q)
+                        if (--dep[P.eau_wait[q]] == 0) push(P.eau_wait[q]);
             } else if (d.kind == DT_SMALL) {
                 consumed(d.front);
                 if (!front_done(d.front)) return "dataflow: parent of a small front has no gate";
@@ -1162,7 +1194,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     // (a task starts after its predecessors finish) and becomes the tickets.
     {
         const int32_t NT = (int32_t)P.dag.size();
-        std::vector<double> cost(NT), bl(NT, 0.0), bgate(NJ, 0.0);
+        std::vector<double> cost(NT), bl(NT, 0.0), bgate(NJ, 0.0), bsyrk(NJ, 0.0);
         for (int32_t t = 0; t < NT; ++t) {
             const DagTask& d = P.dag[t];
             // microseconds per task on one workgroup (measured on MI355X with
@@ -1192,6 +1224,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             double m = 0;
             for (int32_t k = P.succ_ptr[t]; k < P.succ_ptr[t + 1]; ++k) m = std::max(m, bl[P.succ[k]]);
             const int32_t J = d.front;
+            if (d.kind == DT_EA && !d.c) m = std::max(m, bsyrk[J]);  // the front's update-matrix tiles
             if (d.kind == DT_SYRK || d.kind == DT_SMALL) {
                 const int32_t par = P.fronts[J].parent;
                 if (par >= 0) m = std::max(m, bgate[par]);
@@ -1201,11 +1234,12 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     m = std::max(m, bgate[P.uwaiters[q]]);
             bl[t] = cost[t] + m;
             if (d.kind == DT_EA || d.kind == DT_SMALL) bgate[J] = std::max(bgate[J], bl[t]);
+            if (d.kind == DT_SYRK) bsyrk[J] = std::max(bsyrk[J], bl[t]);
         }
         typedef std::pair<double, int32_t> Item;
         std::priority_queue<Item> ready;                                     // (bl, -t)
         std::priority_queue<Item, std::vector<Item>, std::greater<Item>> run;  // (finish, t)
-        std::vector<int32_t> dep(P.dep0), gate(P.gate0), eacnt(NJ, 0), done(NJ, 0);
+        std::vector<int32_t> dep(P.dep0), gate(P.gate0), eacnt(NJ, 0), done(NJ, 0), eaucnt(NJ, 0);
         auto push = [&](int32_t s) { ready.push({bl[s], -s}); };
         for (int32_t t : P.init_hi) push(t);
         for (int32_t t : P.init_lo) push(t);
@@ -1236,6 +1270,9 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             const DagTask& d = P.dag[t];
             const int32_t J = d.front;
             bool consumed = false, fdone = false;
+            if (d.kind == DT_EA && !d.c && ++eaucnt[J] == P.n_eau[J])
+                for (int32_t q = P.eau_ptr[J]; q < P.eau_ptr[J + 1]; ++q)
+                    if (--dep[P.eau_wait[q]] == 0) push(P.eau_wait[q]);
             if (d.kind == DT_EA) consumed = ++eacnt[J] == P.dagf[J].n_ea;
             else if (d.kind == DT_SMALL) consumed = fdone = true;
             else if (d.kind == DT_SYRK) fdone = ++done[J] == P.dagf[J].n_done;
@@ -1351,6 +1388,26 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     return std::string();
 }
 
+std::string tuning_from(const int64_t* v, int64_t nv, PlanTuning* t) {
+    auto get = [&](int64_t i, int64_t def) { return (v && i < nv && v[i] > 0) ? v[i] : def; };
+    const int64_t ug = get(0, t->upd_group), er = get(1, t->ea_rows), wk = get(2, t->wide_k),
+                  sw = get(3, t->sim_workers), zf = get(4, (int64_t)llround(t->zero_frac * 1e6));
+    const int64_t pr = (v && nv > 5 && v[5] >= 0) ? v[5] : t->prio;
+    if (ug < 1 || ug > 64) return "upd_group out of range (1..64)";
+    if (er < EA_WIN || er > (1 << 24)) return "ea_rows out of range";
+    if (wk < 1 || wk > (1 << 20)) return "wide_k out of range";
+    if (sw < 1 || sw > 65536) return "sim_workers out of range";
+    if (zf > 1000000) return "zero fraction above 1";
+    if (pr < 0 || pr >= (1 << (DT_KINDS + 1))) return "priority mask out of range";
+    t->upd_group = (int32_t)ug;
+    t->ea_rows = (int32_t)er;
+    t->wide_k = (int32_t)wk;
+    t->sim_workers = (int32_t)sw;
+    t->zero_frac = (double)zf * 1e-6;
+    t->prio = (int32_t)pr;
+    return std::string();
+}
+
 }  // namespace mf
 
 // Host-only entry (built into libmfsym.so): runs the planner with its bounds
@@ -1361,7 +1418,7 @@ extern "C" int mf_plan_check(int64_t n, int64_t nfr, const int64_t* resptr, cons
                              int64_t* stats, int64_t nstats, char* msg, int64_t msglen) {
     mf::HostPlan P;
     std::string err = mf::build_plan(P, n, nfr, resptr, sepptr, septgt, reltgt, chdptr, chdtgt,
-                                     Dptr, Lptr, 0.05);
+                                     Dptr, Lptr, mf::PlanTuning());
     if (msg && msglen > 0) {
         size_t k = std::min<size_t>(err.size(), (size_t)msglen - 1);
         std::copy(err.begin(), err.begin() + k, msg);

@@ -10,6 +10,18 @@
 
 namespace mf {
 
+// Planner parameters that shape the tile-DAG (chosen per pattern and
+// precision by the autotuner; the defaults were tuned by hand on the 64^3
+// 7-point Laplacian in FP64).
+struct PlanTuning {
+    int32_t upd_group = UPD_GROUP;      // panel steps per trailing-update tile (1..64)
+    int32_t ea_rows = EA_ROWS;          // parent rows per extend-add task (>= EA_WIN)
+    int32_t wide_k = WIDE_K;            // 128 x 128 update-matrix tiles when nn >= wide_k
+    int32_t sim_workers = SIM_WORKERS;  // resident workgroups of the ticket simulation
+    double zero_frac = 0.05;            // explicit zeros allowed by a chain merge
+    int32_t prio = 0;                   // tile-DAG task kinds at raised wave priority (bit mask)
+};
+
 // Task of the persistent triangular solves (one launch per direction).
 enum : int32_t { PT_SMALL = 0, PT_DIAG = 1, PT_ROWS = 2, PT_TINY = 3 };
 struct PTask {
@@ -63,6 +75,8 @@ struct HostPlan {
     // by the front's gate (its children done, reused U slots consumed)
     std::vector<int32_t> dep0;              // per task: number of predecessor tasks
     std::vector<int32_t> succ_ptr, succ;    // successors (CSR over tasks)
+    std::vector<int32_t> n_eau;             // per front: update-matrix extend-add groups
+    std::vector<int32_t> eau_ptr, eau_wait; // per front: tiles waiting for all of them
     std::vector<int32_t> gate0;             // per front: children in the DAG + U-slot waits
     std::vector<int32_t> gtask0, gtask1;    // per front: the tasks its gate releases
     std::vector<int32_t> uwaiters_ptr, uwaiters;  // reverse of uwait
@@ -75,6 +89,7 @@ struct HostPlan {
     // copy-out
     std::vector<OutFront> out;
     std::vector<int32_t> septpos;
+    PlanTuning tune;
     // statistics
     double ref_flops = 0, engine_flops = 0, gemm_flops = 0;
     int64_t n_small_region = 0, n_big = 0, max_nj = 0, small_levels = 0;
@@ -84,6 +99,10 @@ struct HostPlan {
 std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* resptr,
                        const int64_t* sepptr, const int64_t* septgt, const int64_t* reltgt,
                        const int64_t* chdptr, const int64_t* chdtgt, const int64_t* Dptr,
-                       const int64_t* Lptr, double zero_frac);
+                       const int64_t* Lptr, const PlanTuning& tune);
+
+// Tuning array of the C-ABI (mf_plan_create_tuned): missing or non-positive
+// entries keep the defaults; returns an error message for out-of-range values.
+std::string tuning_from(const int64_t* v, int64_t nv, PlanTuning* t);
 
 }  // namespace mf

@@ -27,6 +27,7 @@
 #include <cstdint>
 
 #include "mf_kernels.h"
+#include "mf_sync.h"
 #include "mf_mfma.h"
 
 namespace mf {
@@ -58,38 +59,18 @@ __device__ __forceinline__ void stc(T* p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Bounded wait until *p >= target; gives up (and records it) after ~1 s.
-__device__ __forceinline__ void wait_geq(int32_t* p, int32_t target, int32_t* hang) {
-    int spins = 0;
-    // poll with relaxed loads (device-coherent; no cache-wide fence)
-    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++spins > (1 << 24)) {
-            atomicExch(hang, 1);
-            break;
-        }
-        // once any wait has given up, every other wait returns at once
-        if ((spins & 255) == 0 && __hip_atomic_load(hang, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            break;
-        __builtin_amdgcn_s_sleep(2);
-    }
+// Bounded waits (the watchdog of mf_sync.h; progress is a change of the
+// awaited counter or a ticket taken anywhere in the launch).
+__device__ __forceinline__ void wait_geq(int32_t* p, int32_t target, int32_t* hang, const uint32_t* ticket,
+                                         unsigned long long stall) {
+    (void)bounded_wait(p, target, hang, ticket, stall, 2);
 }
 
 // Same, returning the value seen (>= target unless the wait gave up), so a
 // caller that runs behind the counter does not poll again for every step.
-__device__ __forceinline__ int wait_val(int32_t* p, int32_t target, int32_t* hang) {
-    int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int spins = 0;
-    while (v < target) {
-        if (++spins > (1 << 24)) {
-            atomicExch(hang, 1);
-            break;
-        }
-        if ((spins & 255) == 0 && __hip_atomic_load(hang, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            break;
-        __builtin_amdgcn_s_sleep(2);
-        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return v;
+__device__ __forceinline__ int wait_val(int32_t* p, int32_t target, int32_t* hang, const uint32_t* ticket,
+                                        unsigned long long stall) {
+    return bounded_wait(p, target, hang, ticket, stall, 2);
 }
 
 struct Sync {
@@ -462,7 +443,7 @@ __device__ void diag_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
         T xr[RB / 4];
         for (int ib = 0; ib < kblk; ++ib) {
             if (tid == 0) {
-                if (ib >= avail) avail = wait_val(Y_.prog + J, ib + 1, F.hang);
+                if (ib >= avail) avail = wait_val(Y_.prog + J, ib + 1, F.hang, F.ticket, F.stall);
                 *s_av = avail;
             }
             __syncthreads();
@@ -512,7 +493,7 @@ __device__ void rows_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
         for (int ib = 0; ib < p; ++ib) {
             const int kbi = min(NB, nn - ib * NB);
             if (tid == 0) {
-                if (ib >= avail) avail = wait_val(Y_.prog + J, ib + 1, F.hang);
+                if (ib >= avail) avail = wait_val(Y_.prog + J, ib + 1, F.hang, F.ticket, F.stall);
                 *s_av = avail;
             }
             __syncthreads();
@@ -667,7 +648,7 @@ __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
         T xr[RB / 4];
         for (int t = 0; t < nch; ++t) {
             if (tid == 0) {
-                if (t + 1 > avail) avail = wait_val(Y_.prog + J, t + 1, F.hang);
+                if (t + 1 > avail) avail = wait_val(Y_.prog + J, t + 1, F.hang, F.ticket, F.stall);
                 *s_av = avail;
             }
             __syncthreads();
@@ -717,7 +698,7 @@ __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, De
                 if (lane == 0)
                     for (int q = f.chd0; q < f.chd1; ++q) {
                         const int K = P.chd[q];
-                        wait_geq(Y_.done + K, Y_.nt[K], F.hang);
+                        wait_geq(Y_.done + K, Y_.nt[K], F.hang, F.ticket, F.stall);
                     }
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -738,7 +719,7 @@ __global__ __launch_bounds__(256, 2) void k_fsolve(DevPlan P, DevFactor<T> F, De
         if (tid == 0) {
             for (int q = f.chd0; q < f.chd1; ++q) {
                 const int K = P.chd[q];
-                wait_geq(Y_.done + K, Y_.nt[K], F.hang);
+                wait_geq(Y_.done + K, Y_.nt[K], F.hang, F.ticket, F.stall);
             }
         }
         __syncthreads();
@@ -786,7 +767,7 @@ __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, De
             const int J = w == 0 ? tk.front : w == 1 ? tk.a : w == 2 ? tk.b : tk.c;
             if (J >= 0) {
                 const Front f = P.fronts[J];
-                if (lane == 0 && f.parent >= 0) wait_geq(Y_.done + f.parent, Y_.nt[f.parent], F.hang);
+                if (lane == 0 && f.parent >= 0) wait_geq(Y_.done + f.parent, Y_.nt[f.parent], F.hang, F.ticket, F.stall);
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 wave_bwd(P, F, S, J, Ps + w * TINY_NJ * LDT1, XY + w * TINY_NJ * RB);
@@ -804,7 +785,7 @@ __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, De
         const int J = tk.front;
         const Front f = P.fronts[J];
         if (tid == 0) {
-            if (f.parent >= 0) wait_geq(Y_.done + f.parent, Y_.nt[f.parent], F.hang);
+            if (f.parent >= 0) wait_geq(Y_.done + f.parent, Y_.nt[f.parent], F.hang, F.ticket, F.stall);
         }
         __syncthreads();
         if (tk.kind == PT_SMALL) small_bwd(P, F, S, f, Ps, X);

@@ -37,7 +37,7 @@ _UPLO = {"L": MF_UPLO_L, "U": MF_UPLO_U}
 STAT_NAMES = ["n", "nfr", "internal_fronts", "small_region", "big_fronts", "big_levels",
               "small_levels", "max_nj", "panel_elems", "u_elems", "w_elems", "gemm_tasks",
               "ea_tasks", "diag_tasks", "ref_flops", "engine_flops", "gemm_flops", "ndz", "nlz",
-              "dag_tasks", "ref_flops_dag", "ref_flops_small"]
+              "dag_tasks", "ref_flops_dag", "ref_flops_small", "mv_elems1"]
 KC_NAMES = ["small", "ea", "diag", "gemm", "copyout", "solve", "assemble", "dag"]
 
 
@@ -94,18 +94,35 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-class DevicePlan:
-    """A ChordalSymbolic uploaded once (mf_plan_create)."""
+TUNING_NAMES = ["upd_group", "ea_rows", "wide_k", "sim_workers", "zero_ppm", "prio"]
 
-    def __init__(self, S: ChordalSymbolic):
+
+class DevicePlan:
+    """A ChordalSymbolic uploaded once (mf_plan_create; with ``tuning``, a
+    sequence of the planner parameters TUNING_NAMES, mf_plan_create_tuned)."""
+
+    def __init__(self, S: ChordalSymbolic, tuning=None):
         lib = _native.hip()
         h = ctypes.c_void_p()
         arrs = [S.resptr, S.sepptr, S.septgt, S.reltgt, S.chdptr, S.chdtgt, S.Dptr, S.Lptr]
         self._keep = [np.ascontiguousarray(a, dtype=np.int64) for a in arrs]
-        _check(lib.mf_plan_create(S.n, S.nfr, *[a.ctypes.data_as(_native.i64p) for a in self._keep],
-                                  0, ctypes.byref(h)))
+        ptrs = [a.ctypes.data_as(_native.i64p) for a in self._keep]
+        if tuning is None:
+            _check(lib.mf_plan_create(S.n, S.nfr, *ptrs, 0, ctypes.byref(h)))
+        else:
+            t = np.ascontiguousarray(tuning, dtype=np.int64)
+            _check(lib.mf_plan_create_tuned(S.n, S.nfr, *ptrs, t.ctypes.data_as(_native.i64p), len(t),
+                                            ctypes.byref(h)))
         self.handle = h
         self.S = S
+
+    def tuning(self):
+        """The planner parameters in use and the scheduler's launch estimate (us)."""
+        out = np.zeros(len(TUNING_NAMES) + 1, dtype=np.int64)
+        _check(_native.hip().mf_plan_tuning(self.handle, out.ctypes.data_as(_native.i64p), len(out)))
+        d = dict(zip(TUNING_NAMES, out[:-1].tolist()))
+        d["sim_us"] = int(out[-1])
+        return d
 
     def stats(self):
         out = np.zeros(len(STAT_NAMES), dtype=np.int64)
@@ -121,11 +138,17 @@ class DevicePlan:
             self.handle = None
 
 
-def device_plan(S: ChordalSymbolic) -> DevicePlan:
-    dp = getattr(S, "_devplan", None)
+def device_plan(S: ChordalSymbolic, tuning=None) -> DevicePlan:
+    """The device plan of S (one per planner tuning, kept with S)."""
+    key = None if tuning is None else tuple(int(x) for x in tuning)
+    plans = getattr(S, "_devplans", None)
+    if plans is None:
+        plans = {}
+        S._devplans = plans
+    dp = plans.get(key)
     if dp is None:
-        dp = DevicePlan(S)
-        S._devplan = dp
+        dp = DevicePlan(S, key)
+        plans[key] = dp
     return dp
 
 
@@ -157,7 +180,7 @@ class ChordalFactorization:
     DIAG = MF_CHOLESKY
 
     def __init__(self, A, P: Permutation | None = None, S: ChordalSymbolic | None = None, *,
-                 alg=None, snd="maximal", precision=MF_FP64, uplo="L"):
+                 alg=None, snd="maximal", precision=MF_FP64, uplo="L", tuning=None):
         A = sp.csc_matrix(A, dtype=np.float64)
         A.sort_indices()
         if P is None or S is None:
@@ -166,7 +189,12 @@ class ChordalFactorization:
         self.perm, self.invp = P.perm, P.invp
         self.n = S.n
         self.info = 0
-        self._plan = device_plan(S)
+        if isinstance(tuning, str):
+            if tuning != "auto":
+                raise ValueError("tuning must be a sequence of planner parameters or 'auto'")
+            from .autotune import lookup
+            tuning = lookup(S, precision)
+        self._plan = device_plan(S, tuning)
         self._f = DeviceFactor(self._plan, precision)
         self._set_uplo(uplo)
         self._pattern = None

@@ -79,12 +79,19 @@ class _Pattern:
         return v
 
 
-def _slot_bytes(st, n, nnz, precision):
-    """Device bytes of one factor object of a plan (panels, U, W, reference
-    arrays, vectors, value buffers)."""
+def _slot_bytes(st, n, nnz, precision, nrhs=1):
+    """Device bytes of one resident factor object of a plan for ``nrhs``
+    right-hand sides: panels, U, W, reference arrays, per-column vectors and
+    counters, the pattern's scatter targets, the registered matrix, the
+    solve and refinement workspaces (update vectors, permuted right-hand
+    sides, staging, B copy / X / residual and norms) and the slot's own value
+    and right-hand-side buffers."""
     es = 4 if precision != MF_FP64 else 8
-    return int(es * (st["panel_elems"] + st["u_elems"] + st["w_elems"])
-               + 8 * (st["ndz"] + st["nlz"]) + 64 * n + 16 * nnz)
+    fixed = (es * (st["panel_elems"] + st["u_elems"] + st["w_elems"])
+             + 8 * (st["ndz"] + st["nlz"]) + 64 * n + 16 * st["internal_fronts"])
+    pattern = 16 * nnz + 8 * (n + 1) + 16 * nnz + 8 * nnz     # targets, CSC of A, values
+    per_rhs = es * st["mv_elems1"] + es * n + 8 * n + 3 * 8 * n + 2 * 8 * n + 32
+    return int(fixed + pattern + nrhs * per_rhs)
 
 
 class _Slot:
@@ -115,6 +122,8 @@ class _Slot:
         self.n = n
         self.nrhs = 0
         self.B_h = self.B_d = self.X_d = self.nrm_h = self.nrm_d = None
+        self.st = st
+        self.precision = precision
         self.bytes = _slot_bytes(st, n, self.nnz, precision)
         self.busy = False
         self.event = None
@@ -122,6 +131,9 @@ class _Slot:
         self.last_used = 0.0
 
     def ensure_rhs(self, nrhs, torch):
+        """Right-hand-side buffers for nrhs columns; returns the change of the
+        slot's device bytes (the workspaces grow with nrhs)."""
+        old = self.bytes
         if self.nrhs != nrhs:
             self.B_h = torch.empty((nrhs, self.n), dtype=torch.float64, pin_memory=True)
             self.B_d = torch.empty((nrhs, self.n), dtype=torch.float64, device="cuda")
@@ -129,6 +141,16 @@ class _Slot:
             self.nrm_h = torch.empty(2 * self.k * nrhs, dtype=torch.float64, pin_memory=True)
             self.nrm_d = torch.empty(2 * self.k * nrhs, dtype=torch.float64, device="cuda")
             self.nrhs = nrhs
+            self.bytes = max(self.bytes, _slot_bytes(self.st, self.n, self.nnz, self.precision, nrhs))
+        return self.bytes - old
+
+    def measured_bytes(self):
+        """What the factor object holds on the device (mf_factor_bytes) plus
+        the slot's own buffers."""
+        b = ctypes.c_int64(0)
+        _check(_native.hip().mf_factor_bytes(self.f.handle, ctypes.byref(b)))
+        own = 8 * self.nnz + (2 * 8 * self.n + 16 * self.k) * self.nrhs
+        return int(b.value) + own
 
     def release(self):
         self.f = None
@@ -186,13 +208,13 @@ class ServingLoop:
         return p
 
     # -- device memory admission --
-    def _slot_for(self, pat, k):
+    def _slot_for(self, pat, k, nrhs=1):
         for key, s in self.slots.items():
             if key[0] == pat.key and key[1] == k and not s.busy:
                 self.slots.move_to_end(key)
                 return s
         P, S, flat, plan, st, _ = pat.variant(k)
-        need = _slot_bytes(st, S.n, len(flat), self.precision)
+        need = _slot_bytes(st, S.n, len(flat), self.precision, nrhs)
         if need > self.budget:
             raise MemoryError(f"one factor of this pattern needs {need} bytes > budget {self.budget}")
         # evict idle slots (least recently used first) until the new one fits
@@ -215,7 +237,7 @@ class ServingLoop:
         nrhs = batch[0].B.shape[1]
         if any(r.B.shape != (slot.pat.n, nrhs) for r in batch):
             raise ValueError("requests of one pattern must have the same number of right-hand sides")
-        slot.ensure_rhs(nrhs, torch)
+        self.resident += slot.ensure_rhs(nrhs, torch)
         vals = np.concatenate([np.ascontiguousarray(r.K.data, dtype=np.float64) for r in batch])
         slot.vals_h.numpy()[:] = vals
         Bh = slot.B_h.numpy()
@@ -262,6 +284,11 @@ class ServingLoop:
             r.done = now
         slot.busy, slot.event, slot.batch = False, None, []
         slot.last_used = now
+        # account what the factor object really holds once its workspaces exist
+        real = slot.measured_bytes()
+        if real > slot.bytes:
+            self.resident += real - slot.bytes
+            slot.bytes = real
 
     def run(self, requests, realtime=True, batching=True):
         """Serve requests (sorted by arrival).  realtime: requests become visible
@@ -295,7 +322,7 @@ class ServingLoop:
                 while take * 2 <= min(len(q), maxb):
                     take *= 2
                 pat = self.patterns[key]
-                slot = self._slot_for(pat, take)
+                slot = self._slot_for(pat, take, q[0].B.shape[1])
                 if slot is None:
                     break
                 batch = [q.popleft() for _ in range(take)]

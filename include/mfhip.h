@@ -60,14 +60,36 @@ int mf_plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t*
                    int64_t flags, mf_plan** out);
 int mf_plan_destroy(mf_plan* plan);
 
+/*
+ * mf_plan_create with planner parameters (no counterpart in the reference:
+ * the autotuner's entry).  tuning[0..4]: panel steps per trailing-update
+ * tile (1..64, default 8), parent rows per extend-add task (default 16384),
+ * the front width from which update-matrix tiles are 128 x 128 (default
+ * 128), workgroups assumed resident when ordering the tile-DAG's tickets
+ * (default 512), explicit zeros allowed by a chain merge in parts per
+ * million (default 50000), and tuning[5] a bit mask of tile-DAG task kinds
+ * run at raised wave priority (bits 0..5: extend-add, small front, diagonal
+ * tile, panel solve, trailing update, update matrix; bit 6: the look-ahead
+ * trailing update of the panel chain; default 0; negative keeps it).
+ * Entries 0..4 <= 0 or beyond ntuning keep the default.  mf_plan_tuning
+ * reads back the six values and, as a seventh, the scheduler's estimate of
+ * the tile-DAG launch in microseconds.
+ */
+int mf_plan_create_tuned(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                         const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                         const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
+                         const int64_t* tuning, int64_t ntuning, mf_plan** out);
+int mf_plan_tuning(const mf_plan* plan, int64_t* tuning, int64_t ntuning);
+
 /* Plan statistics, in this order: n, nfr, internal fronts, fronts of the
  * small region, big fronts, big levels, small levels, max front size,
  * panel / U / W pool elements, GEMM tasks, extend-add tasks, diagonal-block
  * tasks, reference flops (sum nn^3/3 + na nn^2 + na^2 nn over the
  * reference's fronts), engine flops (the same over internal fronts), GEMM
  * flops, ndz, nlz, DAG tasks, reference flops of the fronts the tile-DAG
- * launch factors, reference flops of the small region.  nstats may be
- * smaller than the list. */
+ * launch factors, reference flops of the small region, update-vector
+ * elements per right-hand side of the solves.  nstats may be smaller than
+ * the list. */
 int mf_plan_stats(const mf_plan* plan, int64_t* stats, int64_t nstats);
 
 /*
@@ -78,6 +100,9 @@ int mf_plan_stats(const mf_plan* plan, int64_t* stats, int64_t nstats);
  */
 int mf_factor_create(mf_plan* plan, int precision, mf_factor** out);
 int mf_factor_destroy(mf_factor* f);
+/* Device bytes the factor object holds now (its storage plus the solve,
+ * refinement and staging workspaces allocated so far). */
+int mf_factor_bytes(const mf_factor* f, int64_t* bytes);
 
 /*
  * Storage triangle of the reference-layout arrays at this factor's boundary
