@@ -376,7 +376,11 @@ def run_single(args, ct, torch, name):
             tr = np.zeros(2 * nt, dtype=np.uint64)
             E.c(E.lib.mf_plan_dag(E.plan.handle, tasks.ctypes.data_as(ctypes.c_void_p), len(tasks)))
             E.c(E.lib.mf_dag_trace_read(h, tr.ctypes.data_as(ctypes.c_void_p), len(tr)))
-            np.savez_compressed(args.trace_out, tasks=tasks.reshape(-1, 6), trace=tr.reshape(-1, 2))
+            ng = int(E.stats["gemm_tasks"])
+            gm = np.zeros(4 * max(ng, 1), dtype=np.int32)
+            E.c(E.lib.mf_plan_gemm(E.plan.handle, gm.ctypes.data_as(ctypes.c_void_p), len(gm)))
+            np.savez_compressed(args.trace_out, tasks=tasks.reshape(-1, 6), trace=tr.reshape(-1, 2),
+                                gemm=gm.reshape(-1, 4))
         rel, acq, tick = prof[18:21] / args.steps
         print(f"dag sync: release {rel / 1e3:.2f} ms  acquire {acq / 1e3:.2f} ms  ticket {tick / 1e3:.2f} ms"
               f" (summed over workgroups, per step)", file=sys.stderr)

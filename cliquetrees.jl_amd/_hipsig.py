@@ -45,6 +45,7 @@ SIGNATURES = {
     "mf_dag_profile_read": ([vp, f64p, ci], ci),
     "mf_dag_trace_read": ([vp, vp, i64], ci),
     "mf_plan_dag": ([vp, vp, i64], ci),
+    "mf_plan_gemm": ([vp, vp, i64], ci),
     "mf_solve_trace_read": ([vp, vp, i64], ci),
     "mf_plan_solve_tasks": ([vp, vp, i64], ci),
     "mf_plan_fronts": ([vp, i64p, i64], ci),

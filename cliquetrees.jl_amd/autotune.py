@@ -1,11 +1,12 @@
 """Autotuner of the tile-DAG planner parameters, per pattern and precision.
 
-The shape of the big-front schedule is set by six planner parameters
+The shape of the big-front schedule is set by seven planner parameters
 (``TUNING_NAMES``: panel steps per trailing-update tile, parent rows per
 extend-add task, the front width from which update-matrix tiles are
 128 x 128, the workgroups the ticket order is simulated with, and the
 explicit zeros a chain merge may add, the task kinds run at raised wave
-priority).  Their best values depend on the
+priority, and whether the diagonal-tile task also runs the next panel solve
+and look-ahead update).  Their best values depend on the
 pattern (front sizes, tree shape) and on the precision of the fronts (FP32
 fronts run three workgroups per CU, FP64 two).
 
@@ -36,7 +37,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
 DEFAULTS = {"upd_group": 8, "ea_rows": 16384, "wide_k": 128, "sim_workers": 512, "zero_ppm": 50000,
-            "prio": 0}
+            "prio": 0, "fuse": 0, "wg_per_cu": 0}
 CANDIDATES = {
     "upd_group": [4, 8, 12, 16],
     "ea_rows": [8192, 16384, 32768],
@@ -44,6 +45,7 @@ CANDIDATES = {
     "sim_workers": [512, 768, 1024],
     "zero_ppm": [20000, 50000, 100000],
     "prio": [0, 4, 4 | 8 | 64],      # diagonal tiles; + panel solves and the look-ahead update
+    "fuse": [0, 1],
 }
 
 

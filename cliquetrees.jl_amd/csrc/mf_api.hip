@@ -285,7 +285,7 @@ void enqueue_factor(mf_factor* f) {
         launch_fill<uint32_t>(f->ticket, 1, 0u, s);
         DevDag D{pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->eacnt + nfr1,
                  f->prof, f->prof ? f->trace_buf : nullptr,
-                 (int32_t)H.dag.size(), H.tune.prio};
+                 (int32_t)H.dag.size(), H.tune.prio, H.tune.wg_per_cu};
         timed(f, MF_KC_DAG, pl->ref_flops_dag, 1, [&] { launch_dag<T>(P, F, D, s); });
     }
 }
@@ -476,7 +476,7 @@ int mf_plan_tuning(const mf_plan* pl, int64_t* tuning, int64_t ntuning) {
     if (!pl || !tuning) return fail(MF_EARG, "null argument");
     const PlanTuning& t = pl->H.tune;
     const int64_t v[] = {t.upd_group, t.ea_rows, t.wide_k, t.sim_workers,
-                         (int64_t)llround(t.zero_frac * 1e6), t.prio,
+                         (int64_t)llround(t.zero_frac * 1e6), t.prio, t.fuse, t.wg_per_cu,
                          (int64_t)llround(pl->H.sim_makespan_us)};
     for (int64_t i = 0; i < ntuning && i < (int64_t)(sizeof(v) / sizeof(v[0])); ++i) tuning[i] = v[i];
     return MF_OK;
@@ -1199,6 +1199,18 @@ int mf_plan_dag(const mf_plan* pl, int32_t* tasks, int64_t n) {
         const DagTask& d = H.dag[t];
         const int32_t v[6] = {d.kind, d.front, d.a, d.b, d.c, d.d};
         std::copy(v, v + 6, tasks + 6 * t);
+    }
+    return MF_OK;
+}
+
+int mf_plan_gemm(const mf_plan* pl, int32_t* out, int64_t n) {
+    if (!pl || !out) return fail(MF_EARG, "null argument");
+    const HostPlan& H = pl->H;
+    if (n < 4 * (int64_t)H.gemm.size()) return fail(MF_EARG, "buffer too small (4 x GEMM tasks)");
+    for (size_t t = 0; t < H.gemm.size(); ++t) {
+        const GemmTask& g = H.gemm[t];
+        const int32_t v[4] = {g.m, g.n, g.k, g.lower | (g.wide << 1) | ((int32_t)g.mode << 2)};
+        std::copy(v, v + 4, out + 4 * t);
     }
     return MF_OK;
 }

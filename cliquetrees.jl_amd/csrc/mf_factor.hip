@@ -567,7 +567,7 @@ __device__ void potrf_tile(const DevFactor<T>& F, const Front& f, int k0, bool i
 }
 
 constexpr int PB = 16;                          // block size inside the diagonal tile
-constexpr int PTILE = NB * LDS + 3 * PB * PB + 2 * NB;  // LDS elements of potrf_blocked
+constexpr int PTILE = NB * LDS + 3 * PB * PB + 2 * NB + PB;  // LDS elements of potrf_blocked
 
 // Value of v held by lane l of the wavefront (l wave-uniform).
 __device__ __forceinline__ double bcast(double v, int l) {
@@ -595,6 +595,79 @@ __device__ __forceinline__ void blk_mma(const T* X, int ra, int ca, const T* Y, 
     }
 }
 
+// Factors the 16 x 16 diagonal block held by lanes 0..15 of one wavefront
+// (lane i: row i in x[0..15]; the other lanes compute alongside and are
+// ignored), nk <= 16 columns.  Per pivot the pivot itself comes by readlane
+// (a short chain to the next pivot), and column k reaches every lane through
+// LDS (cb) for the rank-1 update.  pv/rd receive the pivots and the
+// reciprocals of L's diagonal (Cholesky) or of the pivots (LDLt).  RULE 0: the general pivot rule (signs,
+// regularization: regularize per pivot); RULE 1: Cholesky with the default
+// signs (fails unless the pivot is positive); RULE 2: LDLt with the default
+// signs (fails on a zero pivot only).
+template <typename T, int RULE>
+__device__ __forceinline__ void pivot_block(T (&x)[PB], const DevFactor<T>& F, const int gcol0, const int nk,
+                                            const bool chol, T* cb, T* pv, T* rd, const int lane) {
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+        if (k >= nk) continue;  // uniform
+        // the lane id is re-read per step so that the compiler does not keep
+        // sixteen steps' worth of lane masks live in scalar registers
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        // column k as it stands to every lane through LDS: one write, then
+        // uniform reads (a wavefront's LDS operations complete in order)
+        if (ln < PB) cb[ln] = x[k];
+        __builtin_amdgcn_wave_barrier();
+        const T akk = bcast(x[k], k);
+        double piv;
+        bool bad;
+        if (RULE == 0) {
+            piv = regularize(F, sign_of(F, gcol0 + k), (double)akk);
+            bad = (piv == 0.0);
+        } else {
+            piv = (double)akk;
+            bad = RULE == 1 ? !(piv > 0.0) : (piv == 0.0);
+        }
+        if (bad) piv = 1.0;  // keeps the rest finite after a failure
+        // 1/sqrt(piv) and 1/piv from the hardware estimates plus two Newton
+        // steps each (full double precision, a short dependency chain)
+        T ip, sq = (T)1, isq = (T)1;
+        if (chol) {
+            double y = __builtin_amdgcn_rsq(piv);
+            const double h = 0.5 * piv;
+            y = y * __builtin_fma(-h * y, y, 1.5);
+            y = y * __builtin_fma(-h * y, y, 1.5);
+            isq = (T)y;
+            sq = (T)(piv * y);
+            ip = (T)(y * y);
+        } else {
+            double y = __builtin_amdgcn_rcp(piv);
+            y = __builtin_fma(__builtin_fma(-piv, y, 1.0), y, y);
+            y = __builtin_fma(__builtin_fma(-piv, y, 1.0), y, y);
+            ip = (T)y;
+        }
+        T colv[PB];
+#pragma unroll
+        for (int j = k + 1; j < PB; ++j) colv[j] = cb[j];
+        const T ark = x[k], ar = ark * ip;
+        if (ln == 0) {
+            if (bad) atomicMin(F.info, (unsigned long long)(gcol0 + k + 1));
+            else if (!chol) stc(F.d + gcol0 + k, (T)piv);
+            pv[k] = (T)piv;
+            rd[k] = chol ? isq : ip;
+        }
+#pragma unroll
+        for (int j = k + 1; j < PB; ++j) x[j] -= ar * colv[j];
+        if (ln >= k) {
+            if (chol) {
+                x[k] = (ln == k) ? sq : ark * isq;
+            } else if (ln > k) {
+                x[k] = ar;  // LDLt keeps the unregularized pivot on the diagonal
+            }
+        }
+    }
+}
+
 // Diagonal tile, blocked: the 64 x 64 tile lives in LDS and is factored in
 // four 16-column block steps (potrf2!'s right-looking recursion cut at 16):
 // (1) one wavefront factors the 16 x 16 diagonal block in registers with the
@@ -614,6 +687,7 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
     T* Tm = lds + NB * LDS;           // 3 blocks of 16 x 16 (stage products, ld 16)
     T* pv = Tm + 3 * PB * PB;         // pivots of the tile (LDLt scaling)
     T* rd = pv + NB;                  // reciprocals of L's diagonal (Cholesky)
+    T* cb = rd + NB;                  // the pivot column of the 16 x 16 block (broadcast)
     const int nj = f.nn + f.na, kb = min(NB, f.nn - k0), gc0 = f.c0 + k0;
     T* blk = F.panel + f.poff + k0 + (int64_t)k0 * nj;
     const int r = tid & 63, cg = tid >> 6, lane = tid & 63, wave = tid >> 6;
@@ -648,55 +722,15 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
                 T x[PB];
 #pragma unroll
                 for (int j = 0; j < PB; ++j) x[j] = lane < PB ? A[(c0 + lane) + (c0 + j) * LDS] : (T)0;
-#pragma unroll
-                for (int k = 0; k < PB; ++k) {
-                    if (c0 + k >= kb) continue;  // uniform
-                    // the lane id is re-read per step so that the compiler does not
-                    // keep sixteen steps' worth of lane masks live in scalar registers
-                    int ln = lane;
-                    asm volatile("" : "+v"(ln));
-                    T colv[PB];                  // column k as it stands (lane j's x[k])
-#pragma unroll
-                    for (int j = k; j < PB; ++j) colv[j] = bcast(x[k], j);
-                    const T akk = colv[k];
-                    double piv = regularize(F, sign_of(F, gc0 + c0 + k), (double)akk);
-                    const bool bad = (piv == 0.0);
-                    if (bad) piv = 1.0;  // keeps the rest finite after a failure
-                    // 1/sqrt(piv) and 1/piv from the hardware estimates plus two
-                    // Newton steps each (full double precision, a short
-                    // dependency chain on the pivot-to-pivot path)
-                    T ip, sq = (T)1, isq = (T)1;
-                    if (chol) {
-                        double y = __builtin_amdgcn_rsq(piv);
-                        const double h = 0.5 * piv;
-                        y = y * __builtin_fma(-h * y, y, 1.5);
-                        y = y * __builtin_fma(-h * y, y, 1.5);
-                        isq = (T)y;
-                        sq = (T)(piv * y);
-                        ip = (T)(y * y);
-                    } else {
-                        double y = __builtin_amdgcn_rcp(piv);
-                        y = __builtin_fma(__builtin_fma(-piv, y, 1.0), y, y);
-                        y = __builtin_fma(__builtin_fma(-piv, y, 1.0), y, y);
-                        ip = (T)y;
-                    }
-                    const T ark = x[k], ar = ark * ip;
-                    if (ln == 0) {
-                        if (bad) atomicMin(F.info, (unsigned long long)(gc0 + c0 + k + 1));
-                        else if (!chol) stc(F.d + gc0 + c0 + k, (T)piv);
-                        pv[c0 + k] = (T)piv;
-                        rd[c0 + k] = isq;
-                    }
-#pragma unroll
-                    for (int j = k + 1; j < PB; ++j) x[j] -= ar * colv[j];
-                    if (ln >= k) {
-                        if (chol) {
-                            x[k] = (ln == k) ? sq : ark * isq;
-                        } else if (ln > k) {
-                            x[k] = ar;  // LDLt keeps the unregularized pivot on the diagonal
-                        }
-                    }
-                }
+                const int nk = min(PB, kb - c0);
+                // the default pivot rule (no signs, no regularization) without
+                // per-pivot branches; signs or a regularization take the general form
+                if (F.signs != nullptr || F.reg_kind != 0)
+                    pivot_block<T, 0>(x, F, gc0 + c0, nk, chol, cb, pv + c0, rd + c0, lane);
+                else if (chol)
+                    pivot_block<T, 1>(x, F, gc0 + c0, nk, true, cb, pv + c0, rd + c0, lane);
+                else
+                    pivot_block<T, 2>(x, F, gc0 + c0, nk, false, cb, pv + c0, rd + c0, lane);
                 if (lane < PB)
 #pragma unroll
                     for (int j = 0; j < PB; ++j)
@@ -704,20 +738,25 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
             }
             __syncthreads();
             // (2) rows below: y L_D' = a (unit L_D for LDLt), then l = y / d for LDLt
+            // (the reciprocals rd loaded unconditionally and applied by selects:
+            // a load under a branch would be waited for inside it)
             const int rb = c0 + PB;
             if (tid < 64 && rb + lane < NB) {
                 const int i = rb + lane;
-                T y[PB];
+                T y[PB], rv[PB];
 #pragma unroll
-                for (int j = 0; j < PB; ++j) y[j] = A[i + (c0 + j) * LDS];
+                for (int j = 0; j < PB; ++j) {
+                    y[j] = A[i + (c0 + j) * LDS];
+                    rv[j] = rd[c0 + j];
+                }
 #pragma unroll
                 for (int k = 0; k < PB; ++k) {
-                    if (chol) y[k] *= rd[c0 + k];
+                    y[k] *= chol ? rv[k] : (T)1;
 #pragma unroll
                     for (int j = k + 1; j < PB; ++j) y[j] -= y[k] * A[(c0 + j) + (c0 + k) * LDS];
                 }
 #pragma unroll
-                for (int k = 0; k < PB; ++k) A[i + (c0 + k) * LDS] = chol ? y[k] : y[k] / pv[c0 + k];
+                for (int k = 0; k < PB; ++k) A[i + (c0 + k) * LDS] = chol ? y[k] : y[k] * rv[k];
             }
             __syncthreads();
             // (3) trailing update of the lower triangle, A_ij -= sum_k L_ik s_k L_jk,
@@ -732,7 +771,8 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
                         for (int s4 = 0; s4 < PB / 4; ++s4) {
                             const int k = 4 * s4 + lk;
                             T av = A[(PB * ti + li) + (c0 + k) * LDS];
-                            if (!chol) av *= pv[c0 + k];
+                            const T sv = pv[c0 + k];
+                            av *= chol ? (T)1 : sv;
                             acc = Mfma<T>::op(av, A[(PB * tj + li) + (c0 + k) * LDS], acc);
                         }
 #pragma unroll
@@ -763,7 +803,8 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
             for (int i = 0; i < PB; ++i) w[i] = (i == lane) ? (T)1 : (T)0;
 #pragma unroll
             for (int k = 0; k < PB; ++k) {
-                if (chol) w[k] *= rd[c0 + k];
+                const T rk = rd[c0 + k];
+                w[k] *= chol ? rk : (T)1;
 #pragma unroll
                 for (int i = k + 1; i < PB; ++i) w[i] -= A[(c0 + i) + (c0 + k) * LDS] * w[k];
             }
@@ -928,8 +969,10 @@ constexpr int LDW = 128 + 4;  // leading dimension of its LDS stages
 // update matrices that carry most of the flops.  Wave w owns the 64 x 64
 // quarter (w & 1, w >> 1) as 4 x 4 MFMA blocks (the strictly upper quarter
 // of a diagonal tile does no MFMA work).  The k-loop stages KW columns of A
-// and B through LDS (As/Bs: KW x LDW) with the next chunk prefetched into
-// registers; C is updated after the k-loop in four row-block rounds.
+// and B through two LDS buffers (As/Bs: 2 x KW x LDW each): while the waves
+// multiply out of one buffer, the next chunk, prefetched into registers one
+// chunk ahead, is written to the other, so each chunk costs one barrier.  C
+// is updated after the k-loop in four row-block rounds.
 template <typename T>
 __device__ __forceinline__ void gemm_tile128(const DevFactor<T>& F, const GemmTask& g, T* As, T* Bs, const int tid) {
     typedef typename Mfma<T>::v4 v4;
@@ -964,23 +1007,32 @@ __device__ __forceinline__ void gemm_tile128(const DevFactor<T>& F, const GemmTa
             rs[i] = sv;
         }
     };
-    fetch(0);
-    for (int k0 = 0; k0 < g.k; k0 += KW) {
+    auto stage = [&](int buf) {
+        T* a = As + buf * KW * LDW;
+        T* b = Bs + buf * KW * LDW;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            As[(kq + 2 * i) * LDW + r] = s ? ra[i] * rs[i] : ra[i];
-            Bs[(kq + 2 * i) * LDW + r] = rb[i];
+            a[(kq + 2 * i) * LDW + r] = s ? ra[i] * rs[i] : ra[i];
+            b[(kq + 2 * i) * LDW + r] = rb[i];
         }
-        __syncthreads();
-        if (k0 + KW < g.k) fetch(k0 + KW);
+    };
+    fetch(0);
+    stage(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < g.k; k0 += KW, buf ^= 1) {
+        const bool more = k0 + KW < g.k;
+        if (more) fetch(k0 + KW);
         if (!idle) {
+            const T* a0 = As + buf * KW * LDW;
+            const T* b0 = Bs + buf * KW * LDW;
 #pragma unroll
             for (int kk = 0; kk < KW; kk += 4) {
                 T a[4], b[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    a[i] = As[(kk + lk) * LDW + wr + 16 * i + li];
-                    b[i] = Bs[(kk + lk) * LDW + wc + 16 * i + li];
+                    a[i] = a0[(kk + lk) * LDW + wr + 16 * i + li];
+                    b[i] = b0[(kk + lk) * LDW + wc + 16 * i + li];
                 }
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
@@ -988,6 +1040,8 @@ __device__ __forceinline__ void gemm_tile128(const DevFactor<T>& F, const GemmTa
                     for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
             }
         }
+        // the other buffer was last read in the previous chunk, before its barrier
+        if (more) stage(buf ^ 1);
         __syncthreads();
     }
     if (idle) return;
@@ -1040,6 +1094,8 @@ __device__ __forceinline__ bool dag_dep(const DevPlan& P, const DevFactor<T>& F,
             return true;
         }
         case DT_POTRF:
+            // the diagonal tile (the fused products wait for their own tiles
+            // inside the task, after the factorization)
             *nw = 1;
             if (q >= 1) return false;
             *w = d.a == 0 ? D.eapcnt + J : cnt + d.a * g.R + d.a;
@@ -1110,7 +1166,7 @@ constexpr unsigned long long WMASK = WDEP == 64 ? ~0ull : (1ull << (WDEP & 63)) 
 // task cost 3 ms of the 35 ms launch on the 64^3 Laplacian.)
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDag D) {
-    constexpr int S1 = PTILE, S2 = 2 * KC * LDT, S3 = 2 * KW * LDW;
+    constexpr int S1 = PTILE, S2 = 2 * KC * LDT, S3 = 4 * KW * LDW;
     constexpr int SMEM = S1 > S2 ? (S1 > S3 ? S1 : S3) : (S2 > S3 ? S2 : S3);
     __shared__ T smem[SMEM];
     __shared__ T s_piv, s_inv;
@@ -1224,10 +1280,43 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                 break;
             case DT_POTRF:
                 potrf_blocked<T>(F, P.fronts[J], d.a * NB, false, smem, tid);
+                if (d.b >= 0) {
+                    // publish the diagonal tile (the other panel solves of column a
+                    // need only it), then the fused panel solve of tile (a+1, a)
+                    drain_stores();
+                    __syncthreads();
+                    if (tid == 0) {
+                        __builtin_amdgcn_s_waitcnt(0);
+                        __hip_atomic_store(cnt + d.a * g.R + d.a, d.a + 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                        // tile (a+1, a) updated through column a - 1: set by a
+                        // task with a smaller ticket (the planner checks), so
+                        // the wait ends
+                        (void)bounded_wait(cnt + d.a * g.R + d.a + 1, d.a, F.hang, F.ticket, F.stall, 1);
+                    }
+                    __syncthreads();
+                    gemm_tile<T>(F, P.gemm[d.b], smem, smem + KC * LDT, tid);
+                    if (d.c >= 0) {
+                        // publish tile (a+1, a), then the fused look-ahead update of
+                        // the next diagonal tile, which reads it back
+                        drain_stores();
+                        __syncthreads();
+                        if (tid == 0) {
+                            __builtin_amdgcn_s_waitcnt(0);
+                            __hip_atomic_store(cnt + d.a * g.R + d.a + 1, d.a + 1, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                            // tile (a+1, a+1) updated through column a - 1 (as above)
+                            (void)bounded_wait(cnt + (d.a + 1) * g.R + d.a + 1, d.a, F.hang, F.ticket,
+                                               F.stall, 1);
+                        }
+                        __syncthreads();
+                        gemm_tile<T>(F, P.gemm[d.c], smem, smem + KC * LDT, tid);
+                    }
+                }
                 break;
             default: {
                 const GemmTask gt = P.gemm[d.a];
-                if (gt.wide) gemm_tile128<T>(F, gt, smem, smem + KW * LDW, tid);
+                if (gt.wide) gemm_tile128<T>(F, gt, smem, smem + 2 * KW * LDW, tid);
                 else gemm_tile<T>(F, gt, smem, smem + KC * LDT, tid);
                 break;
             }
@@ -1261,7 +1350,11 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                     __hip_atomic_store(D.eacnt + J, 1, R, S);
                     __hip_atomic_store(F.flags + J, 1, R, S);
                     break;
-                case DT_POTRF: __hip_atomic_store(cnt + d.a * g.R + d.a, d.a + 1, R, S); break;
+                case DT_POTRF:
+                    __hip_atomic_store(cnt + d.a * g.R + d.a, d.a + 1, R, S);
+                    if (d.b >= 0) __hip_atomic_store(cnt + d.a * g.R + d.a + 1, d.a + 1, R, S);
+                    if (d.c >= 0) __hip_atomic_store(cnt + (d.a + 1) * g.R + d.a + 1, d.a + 1, R, S);
+                    break;
                 case DT_TRSM: __hip_atomic_store(cnt + d.c * g.R + d.b, d.c + 1, R, S); break;
                 case DT_UPD: __hip_atomic_store(cnt + d.c * g.R + d.b, d.d >> 16, R, S); break;
                 default: __hip_atomic_fetch_add(F.flags + J, 1, R, S); break;
@@ -1414,6 +1507,7 @@ void launch_dag(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, hipStr
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_dag<T>, 256, 0);
+    if (D.wg_per_cu > 0 && D.wg_per_cu < per) per = D.wg_per_cu;
     int grid = std::max(1, cus) * std::max(1, per);
     if (grid > D.ntask) grid = D.ntask;
     hipLaunchKernelGGL(k_dag<T>, dim3(grid), dim3(256), 0, s, P, F, D);

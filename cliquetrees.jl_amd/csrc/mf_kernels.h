@@ -75,6 +75,7 @@ struct DevDag {
     int32_t ntask;
     int32_t prio;      // bit k: tasks of kind k run at raised wave priority (bit DT_KINDS:
                        // the look-ahead trailing update of the panel chain)
+    int32_t wg_per_cu; // resident workgroups per CU of the launch (0: as many as fit)
 };
 
 // ---- factorization (mf_factor.hip) ----

@@ -81,6 +81,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     const double zero_frac = tune.zero_frac;
     const int32_t UPD_GROUP = tune.upd_group, EA_ROWS = tune.ea_rows, WIDE_K = tune.wide_k,
                   SIM_WORKERS = tune.sim_workers;
+    const bool FUSE = tune.fuse != 0;
     if (n < 0 || nfr < 0) return "negative size";
     if (n > (int64_t)INT32_MAX / 2) return "n too large";
     P.n = n;
@@ -534,9 +535,19 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             }
             int32_t maxp = 0;
             for (int32_t J : L.big_list) maxp = std::max(maxp, P.dagf[J].p);
+            // With fusion (tune.fuse), the diagonal-tile task of step K also runs
+            // the panel solve of tile (K+1, K) and the look-ahead update of the
+            // next diagonal tile (K+1, K+1): the critical chain
+            // POTRF -> TRSM -> update -> POTRF then hands over between workgroups
+            // once per step instead of three times.  b / c of the task hold the
+            // two products' GemmTask indices (-1: none).
+            std::vector<int64_t> potrf_at(NJ, -1);
             for (int32_t K = 0; K < maxp; ++K) {
                 for (int32_t J : L.big_list)
-                    if (K < P.dagf[J].p) P.dag.push_back(DagTask{DT_POTRF, J, K, 0, 0, 0});
+                    if (K < P.dagf[J].p) {
+                        potrf_at[J] = (int64_t)P.dag.size();
+                        P.dag.push_back(DagTask{DT_POTRF, J, K, -1, -1, 0});
+                    }
                 for (int32_t J : L.big_list) {
                     const Front& F = P.fronts[J];
                     const DagFront& D = P.dagf[J];
@@ -544,7 +555,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     int32_t nj = F.nn + F.na, k0 = K * NB, kb = rlen(F, D.p, K);
                     for (int32_t I = K + 1; I < D.R; ++I) {
                         int64_t ao = F.poff + row0(F, D.p, I) + (int64_t)k0 * nj;
-                        P.dag.push_back(DagTask{DT_TRSM, J, (int32_t)P.gemm.size(), I, K, 0});
+                        if (FUSE && I == K + 1 && K + 1 < D.p)
+                            P.dag[potrf_at[J]].b = (int32_t)P.gemm.size();
+                        else
+                            P.dag.push_back(DagTask{DT_TRSM, J, (int32_t)P.gemm.size(), I, K, 0});
                         add_gemm(G_TRSM, ARR_PANEL, ao, nj, ARR_W, F.woff + k0, F.nn, ARR_PANEL, ao, nj,
                                  rlen(F, D.p, I), kb, kb, F.c0 + k0, 0);
                     }
@@ -574,7 +588,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                         const int32_t jc1 = pass == 2 ? D.p : std::min(jc0 + 1, D.p);
                         for (int32_t Jc = jc0; Jc < jc1; ++Jc)
                             for (int32_t I = Jc; I < D.R; ++I) {
-                                P.dag.push_back(DagTask{DT_UPD, J, (int32_t)P.gemm.size(), I, Jc, K0 | (K1 << 16)});
+                                if (FUSE && pass == 0 && I == Jc && Jc == K + 1 && Jc < D.p)
+                                    P.dag[potrf_at[J]].c = (int32_t)P.gemm.size();
+                                else
+                                    P.dag.push_back(DagTask{DT_UPD, J, (int32_t)P.gemm.size(), I, Jc, K0 | (K1 << 16)});
                                 add_gemm(G_SUB, ARR_PANEL, F.poff + row0(F, D.p, I) + (int64_t)k0 * nj, nj,
                                          ARR_PANEL, F.poff + row0(F, D.p, Jc) + (int64_t)k0 * nj, nj,
                                          ARR_PANEL, F.poff + row0(F, D.p, I) + (int64_t)(Jc * NB) * nj, nj,
@@ -1033,6 +1050,17 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                         return "dataflow: potrf without its update";
                     }
                     fin[tile(J, d.a, d.a)] = t;
+                    if (d.b >= 0) {  // fused panel solve of (a+1, a)
+                        if (d.a > 0 && !edge(lastw[tile(J, d.a + 1, d.a)], t))
+                            return "dataflow: fused trsm without its update";
+                        fin[tile(J, d.a + 1, d.a)] = t;
+                    }
+                    if (d.c >= 0) {  // fused look-ahead update of (a+1, a+1)
+                        if (d.b < 0) return "dataflow: fused update without the fused trsm";
+                        if (d.a > 0 && !edge(lastw[tile(J, d.a + 1, d.a + 1)], t))
+                            return "dataflow: fused update order";
+                        lastw[tile(J, d.a + 1, d.a + 1)] = t;
+                    }
                     break;
                 case DT_TRSM:
                     if (!edge(fin[tile(J, d.c, d.c)], t)) return "dataflow: trsm without potrf";
@@ -1209,7 +1237,14 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     break;
                 }
                 case DT_SMALL: c += 20.0; break;
-                case DT_POTRF: c += 26.0; break;
+                case DT_POTRF:
+                    c += 26.0;
+                    for (int32_t gi : {d.b, d.c})
+                        if (gi >= 0) {
+                            const GemmTask& g = P.gemm[gi];
+                            c += 2.0 * g.m * g.n * (double)g.k / 7.5e4;
+                        }
+                    break;
                 default: {
                     const GemmTask& g = P.gemm[d.a];
                     c += 2.0 * g.m * g.n * (double)g.k / (g.wide ? 1.3e5 : 7.5e4);
@@ -1309,7 +1344,12 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     }
                     break;
                 case DT_SMALL: last_done[d.front] = t; break;
-                case DT_POTRF: setter[cnt_id(d.front, d.a, d.a)] = t; first_panel[d.front] = std::min(first_panel[d.front], t); break;
+                case DT_POTRF:
+                    setter[cnt_id(d.front, d.a, d.a)] = t;
+                    first_panel[d.front] = std::min(first_panel[d.front], t);
+                    if (d.b >= 0) setter[cnt_id(d.front, d.a + 1, d.a)] = t;
+                    if (d.c >= 0) upd_at[cnt_id(d.front, d.a + 1, d.a + 1) * 4096 + d.a] = t;
+                    break;
                 case DT_TRSM: setter[cnt_id(d.front, d.b, d.c)] = t; break;
                 case DT_UPD: upd_at[cnt_id(d.front, d.b, d.c) * 4096 + (d.d >> 16) - 1] = t; break;
                 case DT_SYRK: last_done[d.front] = t; break;
@@ -1345,6 +1385,15 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             if (d.kind == DT_POTRF && d.a > 0) {
                 auto it = upd_at.find(cnt_id(d.front, d.a, d.a) * 4096 + d.a - 1);
                 if (it == upd_at.end() || it->second >= t) return "dag: potrf before its update";
+                // the fused products: their tiles carry the updates through a - 1
+                if (d.b >= 0) {
+                    auto it2 = upd_at.find(cnt_id(d.front, d.a + 1, d.a) * 4096 + d.a - 1);
+                    if (it2 == upd_at.end() || it2->second >= t) return "dag: fused trsm before its update";
+                }
+                if (d.c >= 0) {
+                    auto it3 = upd_at.find(cnt_id(d.front, d.a + 1, d.a + 1) * 4096 + d.a - 1);
+                    if (it3 == upd_at.end() || it3->second >= t) return "dag: fused update order";
+                }
             }
             if (d.kind == DT_TRSM) {
                 if (need(d.c, d.c) < 0 || need(d.c, d.c) >= t) return "dag: trsm before potrf";
@@ -1393,18 +1442,24 @@ std::string tuning_from(const int64_t* v, int64_t nv, PlanTuning* t) {
     const int64_t ug = get(0, t->upd_group), er = get(1, t->ea_rows), wk = get(2, t->wide_k),
                   sw = get(3, t->sim_workers), zf = get(4, (int64_t)llround(t->zero_frac * 1e6));
     const int64_t pr = (v && nv > 5 && v[5] >= 0) ? v[5] : t->prio;
+    const int64_t fu = (v && nv > 6 && v[6] >= 0) ? v[6] : t->fuse;
+    const int64_t wc = (v && nv > 7 && v[7] >= 0) ? v[7] : t->wg_per_cu;
     if (ug < 1 || ug > 64) return "upd_group out of range (1..64)";
     if (er < EA_WIN || er > (1 << 24)) return "ea_rows out of range";
     if (wk < 1 || wk > (1 << 20)) return "wide_k out of range";
     if (sw < 1 || sw > 65536) return "sim_workers out of range";
     if (zf > 1000000) return "zero fraction above 1";
     if (pr < 0 || pr >= (1 << (DT_KINDS + 1))) return "priority mask out of range";
+    if (fu != 0 && fu != 1) return "fuse must be 0 or 1";
+    if (wc < 0 || wc > 16) return "workgroups per CU out of range (0..16)";
     t->upd_group = (int32_t)ug;
     t->ea_rows = (int32_t)er;
     t->wide_k = (int32_t)wk;
     t->sim_workers = (int32_t)sw;
     t->zero_frac = (double)zf * 1e-6;
     t->prio = (int32_t)pr;
+    t->fuse = (int32_t)fu;
+    t->wg_per_cu = (int32_t)wc;
     return std::string();
 }
 

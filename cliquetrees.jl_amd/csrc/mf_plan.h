@@ -20,6 +20,8 @@ struct PlanTuning {
     int32_t sim_workers = SIM_WORKERS;  // resident workgroups of the ticket simulation
     double zero_frac = 0.05;            // explicit zeros allowed by a chain merge
     int32_t prio = 0;                   // tile-DAG task kinds at raised wave priority (bit mask)
+    int32_t fuse = 0;                   // diagonal tasks run the next panel solve and look-ahead update
+    int32_t wg_per_cu = 0;              // resident tile-DAG workgroups per CU (0: as many as fit)
 };
 
 // Task of the persistent triangular solves (one launch per direction).

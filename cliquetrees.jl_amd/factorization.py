@@ -94,7 +94,8 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-TUNING_NAMES = ["upd_group", "ea_rows", "wide_k", "sim_workers", "zero_ppm", "prio"]
+TUNING_NAMES = ["upd_group", "ea_rows", "wide_k", "sim_workers", "zero_ppm", "prio", "fuse",
+                "wg_per_cu"]
 
 
 class DevicePlan:

@@ -70,10 +70,14 @@ int mf_plan_destroy(mf_plan* plan);
  * million (default 50000), and tuning[5] a bit mask of tile-DAG task kinds
  * run at raised wave priority (bits 0..5: extend-add, small front, diagonal
  * tile, panel solve, trailing update, update matrix; bit 6: the look-ahead
- * trailing update of the panel chain; default 0; negative keeps it).
- * Entries 0..4 <= 0 or beyond ntuning keep the default.  mf_plan_tuning
- * reads back the six values and, as a seventh, the scheduler's estimate of
- * the tile-DAG launch in microseconds.
+ * trailing update of the panel chain; default 0; negative keeps it),
+ * tuning[6] 1 to let each diagonal-tile task also run the panel solve of
+ * the tile below it and the look-ahead update of the next diagonal tile, 0
+ * (default) for separate tasks (negative keeps it), tuning[7] the resident
+ * tile-DAG workgroups per CU (0, the default: as many as fit).  Entries
+ * 0..4 <= 0 or beyond ntuning keep the default.  mf_plan_tuning reads back
+ * the eight values and, as a ninth, the scheduler's estimate of the
+ * tile-DAG launch in microseconds.
  */
 int mf_plan_create_tuned(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
                          const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
@@ -274,6 +278,10 @@ int mf_dag_profile_read(mf_factor* f, double* out, int reset);
  * mf_plan_dag: the tickets' tasks, 6 int32 each (kind, front, a, b, c, d). */
 int mf_dag_trace_read(mf_factor* f, unsigned long long* out, int64_t n);
 int mf_plan_dag(const mf_plan* plan, int32_t* tasks, int64_t n);
+/* mf_plan_gemm: the tile products the DAG's panel-solve, trailing-update and
+ * update-matrix tasks index (their field a), 4 int32 each: m, n, k, and
+ * lower | wide << 1 | mode << 2 (mode 0: C -= A B', 1: C = A W'). */
+int mf_plan_gemm(const mf_plan* plan, int32_t* out, int64_t n);
 /* With the profile on, the last solve's per-ticket times (forward tickets,
  * then backward), as mf_dag_trace_read; n >= 2 x solve tasks.
  * mf_plan_solve_tasks: 5 int32 per ticket (kind 0 small front / 1 diagonal

@@ -244,3 +244,18 @@ def test_wide_front_fp32(ct, oracle, wide):
     B = np.random.default_rng(7).standard_normal((A.shape[0], 4))
     X = ct.refine(F, B, tol=1e-10)
     assert rel(X, oracle.solve(S, P, D0, L0, d0, B)) <= 1e-9
+
+
+def test_fused_diagonal_tasks(ct, oracle, wide):
+    """The tile schedule with each diagonal-tile task also running the next
+    panel solve and look-ahead update (planner tuning fuse = 1) gives the
+    oracle's factor on the wide-front problem, Cholesky and LDLt."""
+    A, P, S, (D0, L0, d0, _), (D1, L1, d1, _) = wide
+    tuning = [8, 16384, 128, 512, 50000, 0, 1]
+    F = ct.ChordalCholesky(A, P, S, tuning=tuning)
+    ct.cholesky_(F)
+    assert F._plan.tuning()["fuse"] == 1
+    assert rel(F.Dval, D0) <= 1e-12 and rel(F.Lval, L0) <= 1e-12
+    G = ct.ChordalLDLt(A, P, S, tuning=tuning)
+    ct.ldlt_(G)
+    assert rel(G.d, d1) <= 1e-12 and rel(G.Lval, L1) <= 1e-12
