@@ -19,6 +19,8 @@
  *                  div_fwd_update!   divide.jl:1206-1248
  *   or_div_bwd     div_impl! backward, div_bwd_loop_snd!/nod! divide.jl:864-1204,
  *                  div_bwd_update!   divide.jl:1250-1293
+ *   or_selinv      selinv_impl!      selinv.jl:72-424 (selected inversion of a
+ *                  Cholesky factor, root first)
  *
  * The dense kernels are the reference's own pure-Julia fallbacks restated in
  * C.  For Float64 the reference calls OpenBLAS instead (potrf.jl:87-103,
@@ -466,4 +468,136 @@ void or_div_bwd(i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt
                 for (i64 r = 0; r < ma; ++r) A_(M, ma, r, c) = A_(F, nj, inj[r] - 1, c);
         }
     }
+}
+
+/* ---- selected inversion -------------------------------------------------- */
+
+/*
+ * or_selinv: the entries of (P A P')^-1 on the pattern of the Cholesky factor,
+ * written over the factor in place (Dval: the lower triangle of each diagonal
+ * block, strict upper set to 0; Lval: the rows below).  Restates
+ *   selinv_impl!       selinv.jl:72-94   (root first: j = nfr, ..., 1)
+ *   selinv_loop_snd!   selinv.jl:122-260
+ *   selinv_loop_nod!   selinv.jl:262-379 (nn = 1)
+ *   selinv_send!       selinv.jl:381-424 (copygathertri!, utils.jl:529-553)
+ * with trsm!(:R, :L, :N), symm!(:L, :L), trrk! (= syr2k! with alpha / 2 for
+ * Float64, blas/trrk.jl:5-8), trtri! and syrk!(:L, :C) as plain loops.  The
+ * stack discipline is the reference's: the parent pushes one update matrix
+ * M_i = R_i' F R_i per child in increasing child order, and the next front
+ * in the reverse walk (its last child) pops the top.  Mptr is a 1-based
+ * stack index with 0-based offsets; Mval must hold the stack's peak.
+ * Returns 0.
+ */
+i64 or_selinv(i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt,
+              const i64* chdptr, const i64* chdtgt, const i64* Dptr, const i64* Lptr,
+              double* Dval, double* Lval, double* Mval, i64* Mptr, double* Fval, double* Xval) {
+    i64 ns = 0;
+    Mptr[1] = 0;
+    for (i64 j = nfr - 1; j >= 0; --j) {
+        i64 nn = resptr[j + 1] - resptr[j];
+        i64 na = sepptr[j + 1] - sepptr[j];
+        i64 nj = nn + na;
+        double* F = Fval;
+        double* D11 = Dval + Dptr[j] - 1;
+        double* L21 = Lval + Lptr[j] - 1;
+        const double* M = NULL;
+        if (na > 0) {
+            M = Mval + Mptr[ns];
+            ns -= 1;
+            /* F22 <- M22 (copytri!, lower) */
+            for (i64 c = 0; c < na; ++c)
+                for (i64 r = c; r < na; ++r) A_(F, nj, nn + r, nn + c) = A_(M, na, r, c);
+        }
+        if (nn == 1) {
+            /* selinv_loop_nod! */
+            double d11 = D11[0];
+            double f11 = 0.0;
+            if (na > 0) {
+                for (i64 r = 0; r < na; ++r) L21[r] /= d11;
+                /* f21 <- -M22 l21 (symv!, lower) */
+                for (i64 r = 0; r < na; ++r) {
+                    double s = 0.0;
+                    for (i64 k = 0; k < na; ++k)
+                        s += (r >= k ? A_(M, na, r, k) : A_(M, na, k, r)) * L21[k];
+                    A_(F, nj, 1 + r, 0) = -s;
+                }
+                double dot = 0.0;
+                for (i64 r = 0; r < na; ++r) dot += L21[r] * A_(F, nj, 1 + r, 0);
+                f11 -= dot;
+            }
+            d11 = 1.0 / d11;
+            f11 += d11 * d11;
+            D11[0] = f11;
+            F[0] = f11;
+            for (i64 r = 0; r < na; ++r) L21[r] = A_(F, nj, 1 + r, 0);
+        } else {
+            /* selinv_loop_snd! */
+            for (i64 c = 0; c < nn; ++c)
+                for (i64 r = 0; r < nn; ++r) A_(F, nj, r, c) = 0.0;
+            if (na > 0) {
+                /* L21 <- L21 D11^-1 (trsm!(:R, :L, :N, :N)): columns last to first */
+                for (i64 c = nn - 1; c >= 0; --c) {
+                    for (i64 k = c + 1; k < nn; ++k) {
+                        double a = A_(D11, nn, k, c);
+                        for (i64 r = 0; r < na; ++r) A_(L21, na, r, c) -= A_(L21, na, r, k) * a;
+                    }
+                    double inv = 1.0 / A_(D11, nn, c, c);
+                    for (i64 r = 0; r < na; ++r) A_(L21, na, r, c) *= inv;
+                }
+                /* F21 <- -M22 L21 (symm!(:L, :L)) */
+                for (i64 c = 0; c < nn; ++c)
+                    for (i64 r = 0; r < na; ++r) {
+                        double s = 0.0;
+                        for (i64 k = 0; k < na; ++k)
+                            s += (r >= k ? A_(M, na, r, k) : A_(M, na, k, r)) * A_(L21, na, k, c);
+                        A_(F, nj, nn + r, c) = -s;
+                    }
+                /* F11 <- F11 - (L21' F21 + F21' L21) / 2 on the lower triangle (trrk!) */
+                for (i64 c = 0; c < nn; ++c)
+                    for (i64 r = c; r < nn; ++r) {
+                        double s = 0.0;
+                        for (i64 k = 0; k < na; ++k)
+                            s += A_(L21, na, k, r) * A_(F, nj, nn + k, c) +
+                                 A_(F, nj, nn + k, r) * A_(L21, na, k, c);
+                        A_(F, nj, r, c) -= 0.5 * s;
+                    }
+            }
+            /* D11 <- D11^-1 (trtri!(:L, :N)) into X, column by column */
+            double* X = Xval;
+            for (i64 c = 0; c < nn; ++c) {
+                for (i64 r = 0; r < c; ++r) A_(X, nn, r, c) = 0.0;
+                A_(X, nn, c, c) = 1.0 / A_(D11, nn, c, c);
+                for (i64 r = c + 1; r < nn; ++r) {
+                    double s = 0.0;
+                    for (i64 k = c; k < r; ++k) s += A_(D11, nn, r, k) * A_(X, nn, k, c);
+                    A_(X, nn, r, c) = -s / A_(D11, nn, r, r);
+                }
+            }
+            /* F11 <- F11 + D11' D11 (syrk!(:L, :C) with D11 = tril(D11^-1)) */
+            for (i64 c = 0; c < nn; ++c)
+                for (i64 r = c; r < nn; ++r) {
+                    double s = 0.0;
+                    for (i64 k = r; k < nn; ++k) s += A_(X, nn, k, r) * A_(X, nn, k, c);
+                    A_(F, nj, r, c) += s;
+                }
+            /* D11 <- F11 (copyrec!: the strict upper triangle of F11 is 0), L21 <- F21 */
+            for (i64 c = 0; c < nn; ++c) {
+                for (i64 r = 0; r < nn; ++r) A_(D11, nn, r, c) = A_(F, nj, r, c);
+                for (i64 r = 0; r < na; ++r) A_(L21, na, r, c) = A_(F, nj, nn + r, c);
+            }
+        }
+        /* selinv_send!: M_i <- R_i' F R_i (lower) for each child, increasing order */
+        for (i64 q = chdptr[j]; q < chdptr[j + 1]; ++q) {
+            i64 i = chdtgt[q - 1] - 1;
+            i64 ma = sepptr[i + 1] - sepptr[i];
+            const i64* inj = reltgt + sepptr[i] - 1;
+            ns += 1;
+            i64 strt = Mptr[ns];
+            Mptr[ns + 1] = strt + ma * ma;
+            double* Mi = Mval + strt;
+            for (i64 c = 0; c < ma; ++c)
+                for (i64 r = c; r < ma; ++r) A_(Mi, ma, r, c) = A_(F, nj, inj[r] - 1, inj[c] - 1);
+        }
+    }
+    return 0;
 }

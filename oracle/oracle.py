@@ -8,6 +8,7 @@ Functions mirror the reference's host verbs on the flat arrays of a
   * ``assemble``  — copyto!(F, A) (src/Multifrontal.jl/src/chordal_triangular.jl:852-893)
   * ``chol_impl`` — chol_impl!  (src/Multifrontal.jl/src/cholesky.jl:361-392)
   * ``solve``     — ldiv!(F, B) (src/Multifrontal.jl/src/divide.jl:191-206,1762-1764)
+  * ``selinv``    — selinv!(F)  (src/Multifrontal.jl/src/selinv.jl:38-424)
 """
 from __future__ import annotations
 
@@ -49,6 +50,8 @@ def lib():
             f.argtypes = ([ctypes.c_int64] + [i64p] * 7 + [f64p] * 2 + [ctypes.c_int, f64p,
                           ctypes.c_int64, ctypes.c_int64, f64p, i64p, f64p])
             f.restype = None
+        L.or_selinv.argtypes = [ctypes.c_int64] + [i64p] * 7 + [f64p] * 3 + [i64p, f64p, f64p]
+        L.or_selinv.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -205,3 +208,34 @@ def solve(S, P, Dval, Lval, d, B, diag=0):
     X = div_natural(S, Dval, Lval, d, C, diag)
     X = P.apply_inv(X)
     return X[:, 0] if vec else X
+
+
+def selinv_stack_peak(S):
+    """Peak of selinv_impl!'s update-matrix stack (elements): the root-first
+    walk pops its own M and pushes one per child."""
+    na = np.diff(S.sepptr)
+    stack, top, peak = [], 0, 0
+    for j in range(S.nfr - 1, -1, -1):
+        if na[j] > 0:
+            top -= stack.pop()
+        for q in range(S.chdptr[j] - 1, S.chdptr[j + 1] - 1):
+            i = S.chdtgt[q] - 1
+            stack.append(int(na[i]) ** 2)
+            top += stack[-1]
+        peak = max(peak, top)
+    return peak
+
+
+def selinv(S, Dval, Lval):
+    """selinv! of a Cholesky factor (uplo = :L) on copies: returns (Dval,
+    Lval) holding (P A P')^-1 on the factor's pattern."""
+    Dval = np.array(Dval, dtype=np.float64, copy=True)
+    Lval = np.array(Lval, dtype=np.float64, copy=True)
+    nn = np.diff(S.resptr)
+    Mval = np.zeros(max(selinv_stack_peak(S), 1))
+    Mptr = np.zeros(S.nfr + 2, dtype=np.int64)
+    Fval = np.zeros(max(S.nFval * S.nFval, 2))
+    Xval = np.zeros(max(int(nn.max()) ** 2, 1))
+    lib().or_selinv(S.nfr, *_sym_args(S), _p(Dval, f64p), _p(Lval, f64p), _p(Mval, f64p),
+                    _p(Mptr, i64p), _p(Fval, f64p), _p(Xval, f64p))
+    return Dval, Lval

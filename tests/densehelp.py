@@ -110,6 +110,13 @@ README_LU = np.array([[1, 0, 0, 0, 0], [0, 1, 0, 0, 0], [0, .5, 1, 0, 0], [.5, 0
                       [0, -.5, -.5, .5, 1]])
 README_D = np.array([4.0, -4.0, -4.0, 4.0, 8.0])
 README_PERM = [3, 1, 2, 4, 5]
+# selinv! of the README matrix, as printed (src/Multifrontal.jl/src/selinv.jl:14-31):
+# the lower triangle of (PAP')^-1 on the factor's pattern, perm README_PERM
+README_SELINV = np.array([[0.328125, 0, 0, 0, 0],
+                          [0.0, 0.328125, 0, 0, 0],
+                          [0.0, -0.09375, 0.3125, 0, 0],
+                          [-0.15625, 0.0, 0.0, 0.3125, 0],
+                          [0.0, -0.0625, -0.125, -0.125, 0.25]])
 # elimination orders whose postordered supernodal tree gives README_PERM
 README_ORDERS = [[3, 4, 1, 2, 5], [3, 1, 4, 2, 5]]
 

@@ -13,7 +13,7 @@ import scipy.linalg as sla
 import scipy.sparse as sp
 
 from densehelp import (DYNREG_A, DYNREG_B, README_A, README_AI, README_D, README_L, README_LU,
-                       README_ORDERS, README_PERM, dense_lower, first_indefinite_minor,
+                       README_ORDERS, README_PERM, README_SELINV, dense_lower, first_indefinite_minor,
                        leading_pivots, nod_columns, outside_pattern_norm, permuted_dense, rel,
                        to_layout, weakly_indefinite)
 
@@ -304,3 +304,56 @@ def test_host_blas_mode_matches_loops(ct, oracle, diag):
         k = S.nn()[j]
         blk = D1[S.Dptr[j] - 1: S.Dptr[j] - 1 + k * k].reshape(k, k, order="F")
         assert not np.triu(blk, 1).any()
+
+
+# ---- selected inversion (selinv.jl:38-424) --------------------------------------
+
+def _pattern(S):
+    return dense_lower(S, np.ones(S.ndz), np.ones(S.nlz)) != 0
+
+
+@pytest.mark.parametrize("order", README_ORDERS)
+def test_selinv_docstring(ct, oracle, order):
+    """selinv.jl:14-31: the printed selected inverse of the README matrix
+    (10 stored entries, perm [3,1,2,4,5]); its entries are dyadic, so exact."""
+    P, S, D, L, d, info = _chol(ct, oracle, README_A, alg=order)
+    Ds, Ls = oracle.selinv(S, D, L)
+    assert int(_pattern(S).sum()) == 10
+    assert np.array_equal(dense_lower(S, Ds, Ls), README_SELINV)
+
+
+@pytest.mark.parametrize("name", SPD)
+def test_selinv_vs_dense_inverse(ct, oracle, spd, name):
+    """The selected inverse equals numpy.linalg.inv(PAP') on every entry of the
+    factor's pattern (lower triangle) to 1e-12, the strict upper triangle of
+    each diagonal block is 0 (copyrec! of F11, selinv.jl:250-255), and nothing
+    outside the pattern is written."""
+    A, order, snd = spd[name]
+    P, S, D, L, d, info = _chol(ct, oracle, A, alg=order, snd=snd)
+    Ds, Ls = oracle.selinv(S, D, L)
+    inv = np.linalg.inv(permuted_dense(A, P))
+    mask = _pattern(S)
+    got = dense_lower(S, Ds, Ls)
+    assert rel(got[mask], np.tril(inv)[mask]) <= 1e-12
+    for j in range(S.nfr):
+        k = int(S.nn()[j])
+        blk = Ds[S.Dptr[j] - 1: S.Dptr[j] - 1 + k * k].reshape(k, k, order="F")
+        assert not np.triu(blk, 1).any()
+
+
+def test_selinv_diagonal_and_chain(ct, oracle):
+    """Closed forms: a diagonal matrix inverts entrywise; the path graph's
+    tridiagonal Laplacian + I has a known inverse on its (tridiagonal) pattern."""
+    a = np.array([2.0, 4.0, 8.0, 0.5])
+    P, S, D, L, d, info = _chol(ct, oracle, sp.diags(a).tocsc())
+    Ds, Ls = oracle.selinv(S, D, L)
+    got = np.diag(dense_lower(S, Ds, Ls))
+    assert np.allclose(got, 1.0 / a[P.perm - 1], rtol=1e-15)
+    n = 40
+    T = sp.diags([-np.ones(n - 1), 3.0 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).tocsc()
+    P, S, D, L, d, info = _chol(ct, oracle, T, alg=list(range(1, n + 1)))
+    Ds, Ls = oracle.selinv(S, D, L)
+    inv = np.linalg.inv(T.toarray())
+    got = dense_lower(S, Ds, Ls)
+    mask = _pattern(S)
+    assert rel(got[mask], np.tril(inv)[mask]) <= 1e-14
