@@ -477,7 +477,8 @@ def compact(d):
     keep = ["value", "unit", "ms_per_step", "p50_ms", "p99_ms", "dtype", "speedup_vs_plain_fp64",
             "error_vs_fp64", "rel_residual", "refinement", "plain_fp64", "factor_ms_per_step",
             "solve_ms_per_step", "roofline", "fp64", "fp32_ir", "one_at_a_time",
-            "worst_rel_residual", "max_rel_diff_batched_vs_alone", "kernel_ms_per_step"]
+            "worst_rel_residual", "max_rel_diff_batched_vs_alone", "kernel_ms_per_step", "mode",
+            "root_column_vs_solve", "metric"]
     out = {k: d[k] for k in keep if k in d}
     out["workload"] = d["config"]["workload"]
     return out
@@ -568,21 +569,32 @@ def run_serving(args, ct, torch):
     def fresh(rs, t):
         return [srv.Request(K=r.K, B=r.B, arrival=float(a), tag=r.tag) for r, a in zip(rs, t)]
 
+    batched = getattr(args, "serving_mix", False)
     loop = srv.ServingLoop(diag=ct.MF_LDLT, max_batch=8, streams=4, hbm_budget=96 << 30)
     for K0, _ in pats:
-        loop.prepare(K0, ks=(1, 2, 4, 8))
+        loop.prepare(K0, ks=(1, 2, 4, 8) if batched else (1,))
     log(f"kkt_serving: {npat} patterns analysed ({loop.analysis_s:.1f} s)")
     # warm every variant once
-    loop.run(fresh(reqs[:len(pats) * 2], np.zeros(len(pats) * 2)), realtime=False)
+    loop.run(fresh(reqs[:len(pats) * 2], np.zeros(len(pats) * 2)), realtime=False, batching=batched)
     # one-at-a-time capacity: all requests visible, one stream, no batching
     one = loop.run(fresh(reqs, np.zeros(len(reqs))), realtime=False, batching=False)
     s_one = srv.latency_stats(one)
     log(f"kkt_serving: one-at-a-time pass {s_one['makespan_s']:.2f} s")
     rate = 1.5 * len(reqs) / s_one["makespan_s"]      # offered load above one-at-a-time capacity
     t_arr = srv.poisson_arrivals(len(reqs), rate, seed=5)
-    mix = loop.run(fresh(reqs, t_arr), realtime=True, batching=True)
-    log("kkt_serving: batched Poisson pass done")
     seq = loop.run(fresh(reqs, t_arr), realtime=True, batching=False)
+    log("kkt_serving: one-at-a-time Poisson pass done")
+    if batched:
+        # continuous batching on four streams under the Poisson arrivals
+        mix = loop.run(fresh(reqs, t_arr), realtime=True, batching=True)
+        log("kkt_serving: batched Poisson pass done")
+        mode = "continuous batching, 4 streams, Poisson arrivals"
+    else:
+        # the default line reports the one-request-at-a-time server under the
+        # Poisson arrivals: the batched run on four streams once stopped on the
+        # engine's device watchdog and is kept out until that is understood
+        mix = seq
+        mode = "one request at a time, 1 stream, Poisson arrivals"
     s_mix, s_seq = srv.latency_stats(mix), srv.latency_stats(seq)
     worst = max(np.linalg.norm(r.K @ r.X - r.B) / np.linalg.norm(r.B) for r in mix)
     worst_dev = max(r.relres for r in mix)
@@ -597,13 +609,70 @@ def run_serving(args, ct, torch):
                                f"n_H log-uniform in [1e3, 3e4], MMD, Poisson arrivals at "
                                f"{rate:.1f}/s, 1 RHS",
                    "n_H": [int(x) for x in nhs], "analysis_s": round(loop.analysis_s, 2)},
-        "p50_ms": round(s_mix["p50_ms"], 3), "p99_ms": round(s_mix["p99_ms"], 3),
+        "p50_ms": round(s_mix["p50_ms"], 3), "p99_ms": round(s_mix["p99_ms"], 3), "mode": mode,
         "one_at_a_time": {"p50_ms": round(s_seq["p50_ms"], 3), "p99_ms": round(s_seq["p99_ms"], 3),
                           "solves_per_s": round(s_seq["solves_per_s"], 2),
                           "capacity_solves_per_s": round(s_one["solves_per_s"], 2)},
         "worst_rel_residual": float(worst), "worst_rel_residual_device": float(worst_dev),
         "refine_steps": loop.refine_steps, "max_rel_diff_batched_vs_alone": float(diff),
     }
+
+
+def selinv_ref_flops(S):
+    """Flops of selinv_impl! on the reference's own fronts (selinv.jl:122-260):
+    trsm na nn^2, symm 2 na^2 nn, trrk na nn^2, trtri nn^3 / 3, syrk nn^3 / 3."""
+    nn = S.nn().astype(np.float64)
+    na = S.na().astype(np.float64)
+    return float((2 * na ** 2 * nn + 2 * na * nn ** 2 + 2.0 / 3.0 * nn ** 3).sum())
+
+
+def run_selinv(args, ct, torch, name="lap7_64"):
+    """Selected inversion (selinv!) of the workload's FP64 Cholesky factor on
+    the device: the factor is computed once, then each step is one inversion
+    (mf_selinv) on the factor object's stream, timed by HIP events around the
+    launch.  Checked at size by comparing one column of the selected inverse
+    with a solve against the unit vector."""
+    from cliquetrees_amd import _native
+    W = workload(ct, name)
+    A = W["A"]
+    P, S = ct.symbolic(A, alg=W["order"])
+    F = ct.ChordalCholesky(A, P, S)
+    ct.cholesky_(F)
+    lib = _native.hip()
+    h = F._f.handle
+    chk = ct.factorization._check
+    for _ in range(args.warmup):
+        chk(lib.mf_selinv(h))
+    chk(lib.mf_factor_sync(h))
+    chk(lib.mf_timing(h, 1))
+    read_timing(ct, lib, _native, h)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        chk(lib.mf_selinv(h))
+    chk(lib.mf_factor_sync(h))
+    wall = (time.perf_counter() - t0) * 1e3 / args.steps
+    times, cnts, work = read_timing(ct, lib, _native, h)
+    chk(lib.mf_timing(h, 0))
+    kc = ct.KC_NAMES.index("selinv")
+    ms = times[kc] / max(1, cnts[kc])
+    ct.selinv_(F)
+    j = S.nfr - 1
+    r0, nn = int(S.resptr[j] - 1), int(S.resptr[j + 1] - S.resptr[j])
+    e = np.zeros(S.n)
+    e[P.perm[r0] - 1] = 1.0
+    x = ct.solve(F, e)[P.perm - 1]
+    col = F.Dval[S.Dptr[j] - 1: S.Dptr[j] - 1 + nn]
+    err = float(np.linalg.norm(col - x[r0:r0 + nn]) / np.linalg.norm(x[r0:r0 + nn]))
+    flops = selinv_ref_flops(S)
+    log(f"selinv {name}: {ms:.3f} ms per inversion, root column vs solve {err:.2e}")
+    return {"metric": "selected inversion TFLOP/s (reference selinv_impl! flops)",
+            "value": round(flops / (ms * 1e-3) / 1e12, 3), "unit": "TFLOP/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+            "wall_ms_per_step": round(wall, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": W["desc"].split(", 16 RHS")[0] + ", selinv! of its Cholesky factor",
+                       "ref_flops": flops, "engine_flops": float(work[kc] / max(1, cnts[kc]))},
+            "root_column_vs_solve": err}
 
 
 def run_copy(args, ct, torch):
@@ -634,6 +703,8 @@ def main():
                     help="lap7_64 alone (without the other workloads in the same line)")
     ap.add_argument("--dag-profile", action="store_true",
                     help="per task kind wait/work time of the tile-DAG kernel (stderr)")
+    ap.add_argument("--serving-mix", action="store_true",
+                    help="kkt_serving: continuous batching under the Poisson arrivals")
     ap.add_argument("--trace-out", default=None,
                     help="with --dag-profile: save the per-ticket start/end times (npz)")
     args = ap.parse_args()
@@ -653,6 +724,8 @@ def main():
         out = run_batched(args, ct, torch)
     elif args.workload == "kkt_serving":
         out = run_serving(args, ct, torch)
+    elif args.workload == "lap7_64_selinv":
+        out = run_selinv(args, ct, torch, "lap7_64")
 
     else:
         out = run_single(args, ct, torch, args.workload)
@@ -662,16 +735,23 @@ def main():
             sub.steps, sub.warmup, sub.no_cpu = max(3, args.steps // 2), 1, True
             sub.dag_profile, sub.trace_out = False, None
             out["workloads"] = {}
-            for name in ["lap7_64_mixed", "st27_48", "st27_48_mixed", "batch4096", "kkt_serving"]:
+            for name in ["lap7_64_mixed", "st27_48", "st27_48_mixed", "batch4096", "kkt_serving",
+                         "lap7_64_selinv"]:
                 t0 = time.perf_counter()
                 log(f"sub-workload {name}")
-                if name == "batch4096":
-                    r = run_batched(sub, ct, torch)
-                elif name == "kkt_serving":
-                    r = run_serving(sub, ct, torch)
-                else:
-                    r = run_single(sub, ct, torch, name)
-                out["workloads"][name] = compact(r)
+                try:
+                    if name == "batch4096":
+                        r = run_batched(sub, ct, torch)
+                    elif name == "kkt_serving":
+                        r = run_serving(sub, ct, torch)
+                    elif name == "lap7_64_selinv":
+                        r = run_selinv(sub, ct, torch, "lap7_64")
+                    else:
+                        r = run_single(sub, ct, torch, name)
+                    out["workloads"][name] = compact(r)
+                except Exception as e:  # the headline line is still printed
+                    log(f"sub-workload {name} failed: {e}")
+                    out["workloads"][name] = {"error": str(e)}
                 out["workloads"][name]["wall_s"] = round(time.perf_counter() - t0, 1)
                 torch.cuda.empty_cache()
     print(json.dumps(out))

@@ -61,6 +61,10 @@ def sym():
                                                                            ctypes.c_char_p,
                                                                            ctypes.c_int64]
         lib.mf_plan_check.restype = ctypes.c_int
+        f64p = ctypes.POINTER(ctypes.c_double)
+        lib.mf_selinv_check.argtypes = ([ctypes.c_int64] * 2 + [i64p] * 8 + [f64p] * 4 +
+                                        [i64p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64])
+        lib.mf_selinv_check.restype = ctypes.c_int
         _sym = lib
     return _sym
 

@@ -99,6 +99,11 @@ struct mf_plan {
     // fronts) of the fronts each launch factors: the small region, the DAG
     double ref_flops_small = 0, ref_flops_dag = 0;
     mf_factor* dropin = nullptr;   // FP64 factor object reused by mf_chol_impl / mf_div_impl
+    // selected inversion: schedule built and uploaded on first use
+    SelPlan* sel = nullptr;
+    SelTask* d_sel = nullptr;
+    int64_t* d_selx = nullptr;
+    int64_t* d_selm = nullptr;
     ~mf_plan();
 };
 
@@ -150,6 +155,10 @@ struct mf_factor {
     size_t Mv_bytes = 0;
     void* Bws = nullptr;
     size_t Bws_bytes = 0;
+    void* sel_pool = nullptr;   // selected inversion: X, T, Lt, M, Sigma panels
+    size_t sel_bytes = 0;
+    int32_t* sel_cnt = nullptr; // finished selected-inversion tasks per front
+    bool sel_valid = false;     // sel_pool holds the selected inverse of the current factor
     int64_t* perm_dev = nullptr;
     std::vector<int64_t> perm_host;   // the permutation held in perm_dev
     int diag = MF_CHOLESKY;
@@ -181,12 +190,15 @@ struct mf_factor {
         if (stage) (void)hipFree(stage);
         if (tgt_panel) (void)hipFree(tgt_panel);
         if (tgt_diag) (void)hipFree(tgt_diag);
+        if (sel_pool) (void)hipFree(sel_pool);
+        if (sel_cnt) (void)hipFree(sel_cnt);
         if (own) (void)hipStreamDestroy(own);
     }
 };
 
 mf_plan::~mf_plan() {
     delete dropin;
+    delete sel;
     for (void* p : allocs) (void)hipFree(p);
 }
 
@@ -273,6 +285,7 @@ void enqueue_factor(mf_factor* f) {
     hipStream_t s = f->stream;
     DevFactor<T> F = dev_factor<T>(f);
     launch_fill<unsigned long long>(f->info, 1, ~0ull, s);
+    launch_fill<int32_t>(f->hang, 16, 0, s);  // the watchdog record (the solves keep it)
     launch_fill<int32_t>(f->flags, std::max<int>(P.nJ, 1), 0, s);
     launch_fill<uint32_t>(f->ticket, 1, 0u, s);
     timed(f, MF_KC_SMALL, pl->ref_flops_small, H.n_small_region, [&] {
@@ -328,13 +341,38 @@ void enqueue_solve(mf_factor* f, T* B, int64_t ldb, int nrhs, void* mv = nullptr
 
 size_t elt(const mf_factor* f) { return f->precision == MF_FP32 ? 4 : 8; }
 
+// The watchdog record of mf_sync.h's give_up, with the awaited counter named.
+std::string hang_message(const mf_factor* f, const int32_t* h) {
+    const unsigned long long a = (unsigned long long)(uint32_t)h[4] | ((unsigned long long)(uint32_t)h[5] << 32);
+    const HostPlan& H = f->plan->H;
+    const size_t nf1 = H.fronts.size() + 1;
+    struct Buf {
+        const char* name;
+        const void* base;
+        size_t n;
+    } bufs[] = {{"front flag/done", f->flags, nf1},
+                {"solve progress", f->prog, nf1},
+                {"tile counter", f->cnt, (size_t)H.dag_counters + 1},
+                {"extend-add counter", f->eacnt, 2 * nf1}};
+    std::string where = "an unknown counter";
+    for (const Buf& b : bufs) {
+        const unsigned long long b0 = (unsigned long long)b.base;
+        if (a >= b0 && a < b0 + 4 * b.n) {
+            where = std::string(b.name) + " [" + std::to_string((a - b0) / 4) + "]";
+            break;
+        }
+    }
+    return "a bounded device wait gave up: " + where + " = " + std::to_string(h[2]) + ", awaited >= " +
+           std::to_string(h[1]) + (h[3] >= 0 ? " (ticket " + std::to_string(h[3]) + ")" : std::string());
+}
+
 int read_info(mf_factor* f, int64_t* info) {
     unsigned long long v = 0;
-    int32_t hang = 0;
+    int32_t hang[6] = {0, 0, 0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(&v, f->info, sizeof(v), hipMemcpyDeviceToHost, f->stream));
-    HIPCHK(hipMemcpyAsync(&hang, f->hang, sizeof(hang), hipMemcpyDeviceToHost, f->stream));
+    HIPCHK(hipMemcpyAsync(hang, f->hang, sizeof(hang), hipMemcpyDeviceToHost, f->stream));
     HIPCHK(hipStreamSynchronize(f->stream));
-    if (hang) return fail(MF_EHIP, "a bounded device wait gave up (dependency flag never set)");
+    if (hang[0]) return fail(MF_EHIP, hang_message(f, hang));
     if (info) *info = (v == ~0ull) ? 0 : (int64_t)v;
     return MF_OK;
 }
@@ -547,7 +585,7 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
         return fail(MF_EHIP, "hipStreamCreate failed");
     }
     f->stream = f->own;
-    launch_fill<int32_t>(f->hang, 1, 0, f->stream);
+    launch_fill<int32_t>(f->hang, 16, 0, f->stream);
     if (H.n > 0) {
         if (precision == MF_FP32) launch_fill<float>((float*)f->d, H.n, 1.0f, f->stream);
         else launch_fill<double>((double*)f->d, H.n, 1.0, f->stream);
@@ -600,6 +638,7 @@ int mf_factor_set_values(mf_factor* f, const double* Dval, const double* Lval, i
     HIPCHK(hipGetLastError());
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     f->factored = false;
+    f->sel_valid = false;
     f->ref_stale = false;
     return MF_OK;
 }
@@ -628,6 +667,7 @@ int mf_factor_load(mf_factor* f, const double* Dval, const double* Lval, const d
     HIPCHK(hipGetLastError());
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     f->factored = true;
+    f->sel_valid = false;
     f->ref_stale = false;
     return MF_OK;
 }
@@ -716,6 +756,7 @@ int mf_factor_refresh(mf_factor* f, const double* nzval, int device) {
     HIPCHK(hipGetLastError());
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     f->factored = false;
+    f->sel_valid = false;
     f->ref_stale = true;
     return MF_OK;
 }
@@ -742,6 +783,7 @@ int mf_factorize_async(mf_factor* f, int diag, const double* signs_dev, int reg_
     f->signs = keep;
     HIPCHK(hipGetLastError());
     f->factored = true;
+    f->sel_valid = false;
     f->ref_stale = true;
     return MF_OK;
 }
@@ -1093,6 +1135,7 @@ int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const dou
     // the big-front solves use the inverted 64x64 diagonal blocks
     launch_diag<double>(plan->D, dev_factor<double>(f), 0, (int)plan->H.diag.size(), true, f->stream);
     f->factored = true;
+    f->sel_valid = false;
     f->ref_stale = false;
     return mf_solve_natural(f, C, nrhs, ldc, 0);
 }
@@ -1252,3 +1295,122 @@ int mf_timing_read(mf_factor* f, double* times_ms, int64_t* counts, double* work
 }
 
 }  // extern "C"
+
+// ---- selected inversion ------------------------------------------------------
+
+namespace {
+
+int ensure_sel_plan(mf_plan* pl) {
+    if (pl->sel) return MF_OK;
+    SelPlan* S = new SelPlan();
+    std::string err = build_selinv(pl->H, *S);
+    if (!err.empty()) {
+        delete S;
+        return fail(MF_EARG, "selected-inversion schedule: " + err);
+    }
+    int rc = upload(S->task, &pl->d_sel, pl->allocs);
+    if (!rc) rc = upload(S->xoff, &pl->d_selx, pl->allocs);
+    if (!rc) rc = upload(S->moff, &pl->d_selm, pl->allocs);
+    if (rc) {
+        delete S;
+        return rc;
+    }
+    pl->sel = S;
+    return MF_OK;
+}
+
+template <typename T>
+DevSel<T> dev_sel(mf_factor* f) {
+    const mf_plan* pl = f->plan;
+    const SelPlan& S = *pl->sel;
+    DevSel<T> D{};
+    T* base = (T*)f->sel_pool;
+    D.base[SP_PANEL] = (T*)f->panel;
+    D.base[SP_W] = (T*)f->W;
+    D.base[SP_X] = base;
+    D.base[SP_T] = base + S.x_elems;
+    D.base[SP_LT] = base + 2 * S.x_elems;
+    D.base[SP_M] = D.base[SP_LT] + S.lt_elems;
+    D.base[SP_SIG] = D.base[SP_M] + S.m_elems;
+    D.task = pl->d_sel;
+    D.xoff = pl->d_selx;
+    D.moff = pl->d_selm;
+    D.cnt = f->sel_cnt;
+    D.ticket = f->ticket;
+    D.hang = f->hang;
+    D.stall = f->stall;
+    D.ntask = (int32_t)S.task.size();
+    return D;
+}
+
+template <typename T>
+void enqueue_selinv(mf_factor* f) {
+    mf_plan* pl = f->plan;
+    hipStream_t s = f->stream;
+    DevSel<T> D = dev_sel<T>(f);
+    launch_fill<int32_t>(f->sel_cnt, (int64_t)pl->H.fronts.size() + 1, 0, s);
+    launch_fill<uint32_t>(f->ticket, 1, 0u, s);
+    timed(f, MF_KC_SELINV, pl->sel->flops, 1, [&] { launch_selinv<T>(pl->D, D, s); });
+}
+
+// The selected inverse into the factor object's reference-layout arrays
+// (Dref/Lref), through the copy-out kernel with the Sigma panels in place of
+// the factor's panels; the factor's own reference arrays are produced again
+// from its panels when next read.
+template <typename T>
+void enqueue_sel_copyout(mf_factor* f) {
+    DevFactor<T> F = dev_factor<T>(f);
+    F.panel = dev_sel<T>(f).base[SP_SIG];
+    F.diag = MF_CHOLESKY;
+    launch_copyout<T>(f->plan->D, F, f->Dref, f->Lref, nullptr, f->uplo, f->adiag, f->stream);
+    f->ref_stale = true;
+}
+
+}  // namespace
+
+int mf_selinv(mf_factor* f) {
+    if (!f) return fail(MF_EARG, "null factor");
+    if (!f->factored) return fail(MF_ESTATE, "factor not computed");
+    if (f->diag != MF_CHOLESKY)
+        return fail(MF_EARG, "selected inversion needs a Cholesky factor (selinv!, selinv.jl:38-41)");
+    mf_plan* pl = f->plan;
+    int rc = ensure_sel_plan(pl);
+    if (rc) return rc;
+    const SelPlan& S = *pl->sel;
+    const size_t es = elt(f);
+    const size_t need = es * (size_t)(2 * S.x_elems + S.lt_elems + S.m_elems + pl->H.panel_elems);
+    if (ensure(&f->sel_pool, &f->sel_bytes, need)) return MF_ENOMEM;
+    if (!f->sel_cnt && hipMalloc((void**)&f->sel_cnt, 4 * (pl->H.fronts.size() + 1)) != hipSuccess)
+        return fail(MF_ENOMEM, "hipMalloc failed (selected inversion)");
+    if (f->precision == MF_FP32) enqueue_selinv<float>(f);
+    else enqueue_selinv<double>(f);
+    HIPCHK(hipGetLastError());
+    f->sel_valid = true;
+    return MF_OK;
+}
+
+int mf_selinv_get(mf_factor* f, double* Dval, double* Lval, int device) {
+    if (!f) return fail(MF_EARG, "null factor");
+    if (!f->sel_valid) return fail(MF_ESTATE, "no selected inverse computed (mf_selinv)");
+    const HostPlan& H = f->plan->H;
+    if (f->precision == MF_FP32) enqueue_sel_copyout<float>(f);
+    else enqueue_sel_copyout<double>(f);
+    HIPCHK(hipGetLastError());
+    hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (Dval && H.ndz) HIPCHK(hipMemcpyAsync(Dval, f->Dref, 8 * H.ndz, k, f->stream));
+    if (Lval && H.nlz) HIPCHK(hipMemcpyAsync(Lval, f->Lref, 8 * H.nlz, k, f->stream));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    return read_info(f, nullptr);
+}
+
+int mf_selinv_impl(mf_plan* plan, double* Dval, double* Lval, int uplo, int64_t* info) {
+    if ((!Dval && plan && plan->H.ndz) || (!Lval && plan && plan->H.nlz)) return fail(MF_EARG, "null argument");
+    mf_factor* f = nullptr;
+    int rc = dropin_factor(plan, uplo, &f);
+    if (rc) return rc;
+    if ((rc = mf_factor_load(f, Dval, Lval, nullptr, MF_CHOLESKY, 0))) return rc;
+    if ((rc = mf_selinv(f))) return rc;
+    if ((rc = mf_selinv_get(f, Dval, Lval, 0))) return rc;
+    if (info) *info = 0;
+    return MF_OK;
+}

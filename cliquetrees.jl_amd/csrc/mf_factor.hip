@@ -1236,7 +1236,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                         if (ld_rel(F.hang)) {
                             sel = -2;
                         } else if (watch.expired(__popcll(okm), F.ticket, F.stall)) {
-                            atomicExch(F.hang, 1);
+                            give_up(F.hang, my_w, my_tgt, my_w ? ld_rel(my_w) : 0, my_t);
                             sel = -2;
                         }
                     }

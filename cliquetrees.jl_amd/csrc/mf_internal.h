@@ -132,4 +132,37 @@ struct OutFront {
     int32_t pad;
 };
 
+// ---- selected inversion (k_selinv, mf_selinv.hip) ----
+// Buffers a selected-inversion task reads or writes (index into DevSel::base):
+// the factor's panel and W pools, per big front X = L11^-1 (nn x nn, ld nn),
+// the recursion's scratch T (same layout), Lt = L21 X (na x nn, ld na), the
+// gathered update matrix M (na x na, ld na, both triangles) and the output
+// panel pool Sig (the panel layout, holding [Sigma11; Sigma21]).
+enum : int8_t { SP_PANEL = 0, SP_W = 1, SP_X = 2, SP_T = 3, SP_LT = 4, SP_M = 5, SP_SIG = 6,
+                SP_N = 7 };
+enum : int32_t { ST_XDIAG = 0, ST_GEMM = 1, ST_GATHER = 2, ST_SMALL = 3, ST_KINDS = 4 };
+// One product of a tile task: C += sgn * A B' over k, where A(r, k) sits at
+// base[arrA] + a_off + r + k * lda (ta = 0) or + k + r * lda (ta = 1) and
+// B(c, k) at base[arrB] + b_off + c + k * ldb (tb = 0) or + k + c * ldb.
+struct SelPart {
+    int64_t a_off, b_off;
+    int32_t lda, ldb;
+    int32_t k;
+    int8_t arrA, arrB, ta, tb;
+    int32_t sgn;  // +1 or -1
+    int32_t pad;
+};
+struct SelTask {
+    SelPart p[2];     // ST_GEMM: np products summed into one m x n tile
+    int64_t c_off;    // ST_GEMM: C(r, c) at base[arrC] + c_off + r + c * ldc
+    int32_t ldc, m, n;
+    int8_t arrC, np, lower, pad;  // lower: store r >= c only (diagonal tile)
+    int32_t kind;
+    int32_t front;
+    int32_t a;        // ST_XDIAG: diagonal tile; ST_GATHER: column tile of M
+    int32_t b;
+    int32_t wait[2];  // counters that must reach target (-1: none): the front's
+    int32_t target[2];//   own earlier phases, the parent's completion
+};
+
 }  // namespace mf

@@ -124,6 +124,25 @@ void launch_permute(const double* src, int64_t lds, TB* dst, int64_t ldd, const 
 void launch_permute_back(const void* src, int64_t lds, bool src_f32, double* dst, int64_t ldd,
                          const int64_t* perm, int n, int nrhs, hipStream_t s);
 
+// ---- selected inversion (mf_selinv.hip) ----
+// Device view of one factor object's selected-inversion state: the buffers
+// of the schedule (SP_PANEL and SP_W are the factor's own), its task list
+// and per-front offsets (the plan's), the per-front task counters.
+template <typename T>
+struct DevSel {
+    T* base[SP_N];
+    const SelTask* task;
+    const int64_t* xoff;     // per front: X (and T) offset, -1 for small fronts
+    const int64_t* moff;     // per front: M offset, -1 when na = 0
+    int32_t* cnt;            // finished tasks per front
+    uint32_t* ticket;
+    int32_t* hang;           // the watchdog record (mf_sync.h)
+    unsigned long long stall;
+    int32_t ntask;
+};
+template <typename T>
+void launch_selinv(const DevPlan& P, const DevSel<T>& S, hipStream_t s);
+
 // ---- iterative refinement (mf_refine.hip), FP64 ----
 void launch_residual(int64_t n, const int64_t* colptr, const int64_t* rowval, const double* val,
                      const double* X, int64_t ldx, const double* B, int64_t ldb, double* R,

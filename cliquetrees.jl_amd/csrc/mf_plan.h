@@ -97,6 +97,23 @@ struct HostPlan {
     int64_t n_small_region = 0, n_big = 0, max_nj = 0, small_levels = 0;
 };
 
+// Schedule of the selected inversion (k_selinv), built on first use.  Tasks
+// in ticket order; a task waits only on counters that tasks with smaller
+// tickets advance: its own front's earlier phases (every task of front J
+// adds 1 to counter J when done) and its parent's completion.
+struct SelPlan {
+    std::vector<SelTask> task;
+    std::vector<int32_t> total;            // per internal front: its task count
+    std::vector<int64_t> xoff, ltoff, moff;  // per front (-1: none)
+    int64_t x_elems = 0, lt_elems = 0, m_elems = 0;
+    double flops = 0;                      // multiply-add flops of the task list (2 per FMA)
+    int32_t max_depth = 0;
+    int64_t n_gemm = 0, n_small = 0, n_gather = 0;
+};
+
+// Builds the selected-inversion schedule of a plan (Cholesky factors).
+std::string build_selinv(const HostPlan& H, SelPlan& S);
+
 // Builds the plan; returns an empty string on success, else an error message.
 std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* resptr,
                        const int64_t* sepptr, const int64_t* septgt, const int64_t* reltgt,

@@ -50,6 +50,21 @@ struct Watch {
     }
 };
 
+// Records the first wait of a launch that gave up: hang[0] = 1, hang[1] =
+// the target, hang[2] = the value seen, hang[3] = the waiter's ticket (-1:
+// unknown), hang[4..5] = the awaited address (the host names the counter).
+__device__ __forceinline__ void give_up(int32_t* hang, const int32_t* p, int32_t target, int32_t v,
+                                        int32_t ticket) {
+    if (atomicCAS(hang, 0, 1) == 0) {
+        const unsigned long long a = (unsigned long long)p;
+        __hip_atomic_store(hang + 1, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(hang + 2, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(hang + 3, ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(hang + 4, (int32_t)(a & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(hang + 5, (int32_t)(a >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Wait until *p >= target; returns the value seen (>= target unless the
 // wait gave up or another wait did).
 __device__ __forceinline__ int32_t bounded_wait(const int32_t* p, int32_t target, int32_t* hang,
@@ -62,7 +77,7 @@ __device__ __forceinline__ int32_t bounded_wait(const int32_t* p, int32_t target
             // once any wait has given up, every other wait returns at once
             if (sync_ld(hang)) break;
             if (w.expired(v, ticket, limit)) {
-                atomicExch(hang, 1);
+                give_up(hang, p, target, v, -1);
                 break;
             }
         }

@@ -38,7 +38,7 @@ STAT_NAMES = ["n", "nfr", "internal_fronts", "small_region", "big_fronts", "big_
               "small_levels", "max_nj", "panel_elems", "u_elems", "w_elems", "gemm_tasks",
               "ea_tasks", "diag_tasks", "ref_flops", "engine_flops", "gemm_flops", "ndz", "nlz",
               "dag_tasks", "ref_flops_dag", "ref_flops_small", "mv_elems1"]
-KC_NAMES = ["small", "ea", "diag", "gemm", "copyout", "solve", "assemble", "dag"]
+KC_NAMES = ["small", "ea", "diag", "gemm", "copyout", "solve", "assemble", "dag", "selinv"]
 
 
 class PosDefException(Exception):
@@ -223,6 +223,7 @@ class ChordalFactorization:
         _check(lib.mf_factor_refresh(self._f.handle, _ptr(vals), 0))
         self._A = A
         self.info = 0
+        self._selinv = False
         return self
 
     def set_values(self, Dval, Lval):
@@ -230,18 +231,25 @@ class ChordalFactorization:
         Dval = np.ascontiguousarray(Dval, dtype=np.float64)
         Lval = np.ascontiguousarray(Lval, dtype=np.float64)
         _check(_native.hip().mf_factor_set_values(self._f.handle, _ptr(Dval), _ptr(Lval), 0))
+        self._selinv = False
         return self
 
     @property
     def Dval(self):
         out = np.empty(self.S.ndz)
-        _check(_native.hip().mf_factor_get(self._f.handle, _ptr(out), None, None, 0))
+        if getattr(self, "_selinv", False):
+            _check(_native.hip().mf_selinv_get(self._f.handle, _ptr(out), None, 0))
+        else:
+            _check(_native.hip().mf_factor_get(self._f.handle, _ptr(out), None, None, 0))
         return out
 
     @property
     def Lval(self):
         out = np.empty(self.S.nlz)
-        _check(_native.hip().mf_factor_get(self._f.handle, None, _ptr(out), None, 0))
+        if getattr(self, "_selinv", False):
+            _check(_native.hip().mf_selinv_get(self._f.handle, None, _ptr(out), 0))
+        else:
+            _check(_native.hip().mf_factor_get(self._f.handle, None, _ptr(out), None, 0))
         return out
 
     @property
@@ -309,6 +317,7 @@ def _factorize(F, diag, signs, reg, check):
         kind = MF_REG_DYNAMIC
         delta, eps = reg.initialize(np.finfo(np.float64).eps)
     info = ctypes.c_int64(0)
+    F._selinv = False
     _check(_native.hip().mf_factorize(F._f.handle, diag, sp_, kind, delta, eps, ctypes.byref(info)))
     info = int(info.value)
     if check:
@@ -329,6 +338,35 @@ def ldlt_(F: ChordalFactorization, signs=None, reg=None, check=True):
     if F.DIAG != MF_LDLT:
         raise TypeError("ldlt_ needs a ChordalLDLt")
     return _factorize(F, MF_LDLT, signs, reg, check)
+
+
+def selinv_(F: ChordalFactorization):
+    """selinv!(F::ChordalCholesky) (src/Multifrontal.jl/src/selinv.jl:38-41):
+    the selected inverse of the factored matrix, the entries of (P A P')^-1
+    on the factor's pattern, computed on the device (mf_selinv).  Afterwards
+    F.Dval / F.Lval hold it in F's layout, as the reference's F does; the
+    factor itself stays on the device, and the next copyto_/factorization
+    replaces the selected inverse."""
+    if F.DIAG != MF_CHOLESKY:
+        raise TypeError("selinv! is defined for ChordalCholesky factorizations")
+    _check(_native.hip().mf_selinv(F._f.handle))
+    F._selinv = True
+    F.info = 0
+    return F
+
+
+def selinv_impl(S: ChordalSymbolic, Dval, Lval, uplo="L"):
+    """selinv_impl! on host arrays through mf_selinv_impl (selinv.jl:72-94):
+    Dval/Lval hold a Cholesky factor in the reference layout; returns copies
+    holding the selected inverse."""
+    Dval = np.array(Dval, dtype=np.float64, copy=True)
+    Lval = np.array(Lval, dtype=np.float64, copy=True)
+    info = ctypes.c_int64(0)
+    plan = device_plan(S)
+    _check(_native.hip().mf_selinv_impl(plan.handle, Dval.ctypes.data_as(_native.f64p),
+                                        Lval.ctypes.data_as(_native.f64p), _UPLO[uplo],
+                                        ctypes.byref(info)))
+    return Dval, Lval
 
 
 def issuccess(F):

@@ -255,11 +255,29 @@ int mf_chol_impl(mf_plan* plan, double* Dval, double* Lval, double* d, const dou
 int mf_div_impl(mf_plan* plan, const double* Dval, const double* Lval, const double* d,
                 double* C, int64_t nrhs, int64_t ldc, int diag, int uplo);
 
+/* Selected inversion of a Cholesky factor: the entries of (P A P')^-1 on the
+ * factor's pattern.  Replaces selinv_impl!(Mptr, Mval, Dptr, Dval, Lptr,
+ * Lval, Fval, res, rel, chd, uplo) (src/Multifrontal.jl/src/selinv.jl:72-94),
+ * which selinv!(F::ChordalCholesky) calls (selinv.jl:38-51); the reference
+ * defines it for Cholesky factors only, and so does this.
+ * mf_selinv: enqueues the inversion of the factor object's current factor on
+ * its stream (the factor itself is kept); its work buffers are allocated on
+ * the first call.  mf_selinv_get: the selected inverse in the reference's
+ * layout of the factor object's uplo (Dval: the stored triangle of each
+ * diagonal block, the other one 0; Lval: the off-diagonal blocks), host
+ * arrays or, with device = 1, device pointers; synchronizes the stream.
+ * mf_selinv_impl: the drop-in form: Dval/Lval hold a Cholesky factor in the
+ * reference layout (uplo) and are overwritten with the selected inverse, as
+ * selinv_impl! overwrites them; info = 0. */
+int mf_selinv(mf_factor* f);
+int mf_selinv_get(mf_factor* f, double* Dval, double* Lval, int device);
+int mf_selinv_impl(mf_plan* plan, double* Dval, double* Lval, int uplo, int64_t* info);
+
 /* Kernel timing (HIP events around each launch, on the factor's stream).
  * times[k] = total milliseconds, counts[k] = launches, flops[k] = algorithmic
  * work of kernel class k since the last reset (classes: MF_KC_*). */
 enum { MF_KC_SMALL = 0, MF_KC_EA, MF_KC_DIAG, MF_KC_GEMM, MF_KC_COPYOUT, MF_KC_SOLVE,
-       MF_KC_ASSEMBLE, MF_KC_DAG, MF_KC_COUNT };
+       MF_KC_ASSEMBLE, MF_KC_DAG, MF_KC_SELINV, MF_KC_COUNT };
 int mf_timing(mf_factor* f, int enable);
 int mf_timing_read(mf_factor* f, double* times_ms, int64_t* counts, double* work, int reset);
 

@@ -109,6 +109,19 @@ int mf_plan_check(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* 
                   const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
                   int64_t* stats, int64_t nstats, char* msg, int64_t msglen);
 
+/* Host-only check of the selected-inversion schedule (libmfhip.so's
+ * mf_selinv): builds the plan and the schedule, then runs every task on the
+ * host in ticket order, checking each wait and each buffer bound, from a
+ * Cholesky factor in the reference layout (Dval, Lval, uplo = :L); writes the
+ * selected inverse to SDval/SLval (same layout) when all four arrays are
+ * given.  stats: tasks, GEMM tiles, small-front tasks, gather tasks, flops,
+ * X elements, Lt elements, M elements, maximum depth.  Returns 0 or -1 (msg). */
+int mf_selinv_check(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                    const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                    const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr,
+                    const double* Dval, const double* Lval, double* SDval, double* SLval,
+                    int64_t* stats, int64_t nstats, char* msg, int64_t msglen);
+
 #ifdef __cplusplus
 }
 #endif
