@@ -235,7 +235,9 @@ class Engine:
         from cliquetrees_amd import _native
         self.ct, self.torch, self.lib, self._native = ct, torch, _native.hip(), _native
         self.A, self.n, self.nrhs = A, A.shape[0], nrhs
-        self.plan = ct.device_plan(S)
+        # planner parameters tuned for this pattern and precision (autotune.py's
+        # table), else the defaults
+        self.plan = ct.device_plan(S, ct.autotune.lookup(S, precision))
         self.stats = self.plan.stats()
         self.fac = ct.DeviceFactor(self.plan, precision)
         self.h = self.fac.handle
