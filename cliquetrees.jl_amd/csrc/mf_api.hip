@@ -461,6 +461,7 @@ static int plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int6
         (rc = upload(H.out, (OutFront**)&D.out, pl->allocs)) ||
         (rc = upload(H.septpos, (int32_t**)&D.septpos, pl->allocs)) ||
         (rc = upload(H.mvoff1, (int64_t**)&D.mvoff1, pl->allocs)) ||
+        (rc = upload(H.sboff1, (int64_t**)&D.sboff1, pl->allocs)) ||
         (rc = upload(pl->in_small, (uint8_t**)&D.in_small, pl->allocs)) ||
         (rc = upload(pl->col2ref, (int32_t**)&D.col2ref, pl->allocs))) {
         delete pl;

@@ -29,6 +29,8 @@ constexpr int EA_ROWS = 16384;  // parent rows per extend-add task (about)
 constexpr int WIDE_K = 128;     // update-matrix tiles are 128 x 128 when nn >= WIDE_K
 constexpr int UPD_GROUP = 8;    // panel steps applied by one trailing-update tile
 constexpr int SIM_WORKERS = 512;  // resident workgroups assumed when ordering the DAG tickets
+constexpr int SEP_TILES = 8;    // separator tiles per backward partial-product task (fronts whose
+                                // separator has more are split over several tasks)
 
 struct Front {
     int64_t poff;    // panel offset (elements) in the panel pool

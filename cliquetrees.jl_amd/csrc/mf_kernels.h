@@ -34,6 +34,7 @@ struct DevPlan {
     const OutFront* out;
     const int32_t* septpos;
     const int64_t* mvoff1;
+    const int64_t* sboff1;       // per front: backward separator partials (per right-hand side), -1: none
     const uint8_t* in_small;     // 1: front of the small region
     const int32_t* col2ref;      // global column -> reference front
     int32_t nJ;

@@ -381,6 +381,17 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         P.mvoff1[J] = mv;
         mv += P.fronts[J].na;
     }
+    // backward separator partials of big fronts with long separators: one
+    // 64-row slot per (diagonal block, chunk of SEP_TILES separator tiles)
+    P.sboff1.assign(NJ, -1);
+    for (int32_t J = 0; J < NJ; ++J) {
+        const Front& F = P.fronts[J];
+        const int32_t nsep = (F.na + NB - 1) / NB;
+        if (P.is_small[J] || nsep <= SEP_TILES) continue;
+        const int32_t pk = (F.nn + NB - 1) / NB, nch = (nsep + SEP_TILES - 1) / SEP_TILES;
+        P.sboff1[J] = mv;
+        mv += (int64_t)pk * nch * NB;
+    }
     P.mv_elems1 = mv + 1;
 
     // ---- 4. task lists ---------------------------------------------------------
@@ -802,37 +813,59 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 P.ntf[x.J]++;
             }
         }
-        // backward: level order (levels from the root down; chains of a level
-        // interleaved by block, last block first; then the small region).  A
-        // list schedule like the forward one was measured slower here: the
-        // backward tasks of a level all become ready together when the level
-        // above finishes, and a level-ordered ticket list matches that.
-        int32_t nlev_b = (int32_t)P.levels.size();
-        for (int32_t l = nlev_b - 1; l >= 0; --l) {
-            const Level& L = P.levels[l];
-            for (int32_t J : L.small_list) {
-                P.btask.push_back(PTask{PT_SMALL, J, 0, 0});
-                P.ntb[J] = 1;
+        // backward: the big region by depth (distance from the root): a
+        // front's backward tasks all become ready when its parent finishes,
+        // so the fronts of one depth are ready together.  Per depth: the
+        // small fronts living in the big region, then the split separator
+        // products of the depth's big fronts (their diagonal tasks wait for
+        // them), then the diagonal chains interleaved by block, last block
+        // first.  (By height instead, a shallow front with a short subtree
+        // waits behind deeper fronts of larger height; a list schedule like
+        // the forward one was measured slower: 7.4 ms against 4.5.)
+        std::vector<int32_t> depth(NJ, 0);
+        for (int32_t J = NJ - 1; J >= 0; --J)
+            if (P.fronts[J].parent >= 0) depth[J] = depth[P.fronts[J].parent] + 1;
+        {
+            int32_t maxd = 0;
+            for (int32_t J = 0; J < NJ; ++J)
+                if (!in_small_region[J]) maxd = std::max(maxd, depth[J]);
+            std::vector<std::vector<int32_t>> small_at(maxd + 1), big_at(maxd + 1);
+            for (int32_t J = NJ - 1; J >= 0; --J) {
+                if (in_small_region[J]) continue;
+                (P.is_small[J] ? small_at : big_at)[depth[J]].push_back(J);
             }
-            int32_t maxp = 0;
-            for (int32_t J : L.big_list) maxp = std::max(maxp, nblk(J));
-            for (int32_t t = 0; t < maxp; ++t)
-                for (int32_t J : L.big_list) {
-                    int32_t pk = nblk(J), k = pk - 1 - t;
-                    if (k >= 0) {
-                        const Front& F = P.fronts[J];
-                        P.btask.push_back(PTask{PT_DIAG, J, k * NB, std::min(NB, F.nn - k * NB)});
-                        P.ntb[J]++;
-                    }
+            for (int32_t d = 0; d <= maxd; ++d) {
+                for (int32_t J : small_at[d]) {
+                    P.btask.push_back(PTask{PT_SMALL, J, 0, 0});
+                    P.ntb[J] = 1;
                 }
+                for (int32_t J : big_at[d]) {
+                    if (P.sboff1[J] < 0) continue;
+                    const Front& F = P.fronts[J];
+                    const int32_t nsep = (F.na + NB - 1) / NB;
+                    for (int32_t k = nblk(J) - 1; k >= 0; --k)
+                        for (int32_t t0 = 0; t0 < nsep; t0 += SEP_TILES) {
+                            P.btask.push_back(PTask{PT_SEPB, J, k * NB, t0, std::min(nsep, t0 + SEP_TILES)});
+                            P.ntb[J]++;
+                        }
+                }
+                int32_t maxp = 0;
+                for (int32_t J : big_at[d]) maxp = std::max(maxp, nblk(J));
+                for (int32_t t = 0; t < maxp; ++t)
+                    for (int32_t J : big_at[d]) {
+                        int32_t pk = nblk(J), k = pk - 1 - t;
+                        if (k >= 0) {
+                            const Front& F = P.fronts[J];
+                            P.btask.push_back(PTask{PT_DIAG, J, k * NB, std::min(NB, F.nn - k * NB)});
+                            P.ntb[J]++;
+                        }
+                    }
+            }
         }
         // the small region by depth (distance from the root): a front's
         // children follow right after its own depth, however low their
         // subtrees are (height order would put a short subtree's fronts at
         // the end of the list and make them wait for unrelated tickets)
-        std::vector<int32_t> depth(NJ, 0);
-        for (int32_t J = NJ - 1; J >= 0; --J)
-            if (P.fronts[J].parent >= 0) depth[J] = depth[P.fronts[J].parent] + 1;
         std::vector<int32_t> border(P.small_order.rbegin(), P.small_order.rend());
         std::stable_sort(border.begin(), border.end(), [&](int32_t a, int32_t b) { return depth[a] < depth[b]; });
         for (int32_t J : border) {
@@ -854,7 +887,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         for (size_t i = 0; i < tl->size();) {
             if (!tiny((*tl)[i])) {
                 out.push_back((*tl)[i]);
-                out.back().c = 0;
+                if (out.back().kind != PT_SEPB) out.back().c = 0;
                 ++i;
                 continue;
             }
@@ -885,6 +918,16 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 return "gemm task out of bounds";
             if (t.s_col < 0 || t.s_col + t.k > n || (t.mode == G_TRSM && t.s_col + t.n > n))
                 return "gemm scaling out of range";
+        }
+        for (const PTask& t : P.btask) {
+            if (t.kind != PT_SEPB) continue;
+            const Front& F = P.fronts[t.front];
+            const int32_t nsep = (F.na + NB - 1) / NB, pk = (F.nn + NB - 1) / NB;
+            const int32_t nch = (nsep + SEP_TILES - 1) / SEP_TILES;
+            if (P.sboff1[t.front] < 0 || t.a < 0 || t.a % NB != 0 || t.a >= F.nn || t.b < 0 ||
+                t.b % SEP_TILES != 0 || t.c <= t.b || t.c > nsep || t.c - t.b > SEP_TILES)
+                return "backward separator task";
+            if (P.sboff1[t.front] + (int64_t)pk * nch * NB > P.mv_elems1) return "separator partials out of bounds";
         }
         for (const DiagTask& t : P.diag) {
             const Front& F = P.fronts[t.front];
@@ -954,6 +997,8 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     if (t.a < F.nn || t.b > nj || t.b <= t.a || t.b - t.a > NB || (t.a - F.nn) % NB ||
                         (t.b != nj && t.b - t.a != NB))
                         return "rows solve task";
+                } else if (t.kind == PT_SEPB) {
+                    if (tl != &P.btask) return "separator task outside the backward solve";
                 } else {
                     return "solve task kind";
                 }
@@ -1003,6 +1048,23 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     if (t.a / NB != nextk[t.front]) return "backward chain order";
                     nextk[t.front]--;
                 }
+            // a front's separator tasks come before its diagonal tasks (which
+            // wait for all of them)
+            std::vector<uint8_t> diag_seen(NJ, 0);
+            std::vector<int32_t> nsepb(NJ, 0);
+            for (const PTask& t : P.btask) {
+                if (t.kind == PT_DIAG) diag_seen[t.front] = 1;
+                if (t.kind == PT_SEPB) {
+                    if (diag_seen[t.front]) return "backward separator task after a diagonal task";
+                    nsepb[t.front]++;
+                }
+            }
+            for (int32_t J = 0; J < NJ; ++J) {
+                const Front& F = P.fronts[J];
+                const int32_t nsep = (F.na + NB - 1) / NB;
+                const int32_t want = P.sboff1[J] < 0 ? 0 : ((F.nn + NB - 1) / NB) * ((nsep + SEP_TILES - 1) / SEP_TILES);
+                if (nsepb[J] != want) return "backward separator task count";
+            }
         }
     }
 

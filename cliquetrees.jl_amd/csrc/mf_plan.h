@@ -25,12 +25,14 @@ struct PlanTuning {
 };
 
 // Task of the persistent triangular solves (one launch per direction).
-enum : int32_t { PT_SMALL = 0, PT_DIAG = 1, PT_ROWS = 2, PT_TINY = 3 };
+enum : int32_t { PT_SMALL = 0, PT_DIAG = 1, PT_ROWS = 2, PT_TINY = 3, PT_SEPB = 4 };
 struct PTask {
     int32_t kind;    // PT_SMALL: a whole small front; PT_DIAG: diagonal block rows
                      // [a, a+b) of a big front; PT_ROWS: update rows [a, b);
                      // PT_TINY: up to four fronts of nJ <= TINY_NJ (front, a, b,
-                     // c; -1 when absent), one wavefront each
+                     // c; -1 when absent), one wavefront each; PT_SEPB (backward):
+                     // the separator product of diagonal block a over the
+                     // separator tiles [b, c), into the block's partial slot
     int32_t front;
     int32_t a, b;
     int32_t c;
@@ -88,6 +90,9 @@ struct HostPlan {
     double sim_makespan_us = 0;             // list-scheduled length of the DAG (estimate)
     std::vector<int32_t> ntf, ntb;     // tasks per front (completion counts)
     std::vector<int64_t> mvoff1;       // per internal front: offset of Mv (na x 1)
+    std::vector<int64_t> sboff1;       // per internal front: offset (per right-hand side) of its
+                                       // backward separator partials (p x nch slots of 64 rows),
+                                       // -1: the diagonal tasks compute the product themselves
     // copy-out
     std::vector<OutFront> out;
     std::vector<int32_t> septpos;

@@ -527,16 +527,19 @@ __device__ void rows_fwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
 // (li, lk) of chunk u holds rows nn + 64q + 8c + 4s + lk (c = u % 8, s < 2).
 constexpr int SEP_S = 2;  // 4-row MFMA steps per chunk
 
+// r0: the first separator row of the tiles taken (a chunk of the
+// separator); sidx holds the chunk's row indices (sidx[0] is row r0).
 template <typename T>
 __device__ __forceinline__ void sep_operands(const T* Pj, int64_t nj, int nn, int k0, int kb, const T* B,
                                              int64_t ldb, int cb, int nb, int wv, int u, const int32_t* sidx,
-                                             T (&a)[4 * SEP_S], T (&b)[SEP_S]) {
+                                             int r0, T (&a)[4 * SEP_S], T (&b)[SEP_S]) {
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-    const int base = 64 * (wv + 4 * (u >> 3)) + 8 * (u & 7) + lk;  // separator row (local - nn)
+    const int rel = 64 * (wv + 4 * (u >> 3)) + 8 * (u & 7) + lk;  // row in the chunk
+    const int base = r0 + rel;                                      // separator row (local - nn)
     const int na = (int)nj - nn;
 #pragma unroll
     for (int s = 0; s < SEP_S; ++s)
-        b[s] = (base + 4 * s < na && li < nb) ? ldc(B + sidx[base + 4 * s] + (int64_t)(cb + li) * ldb) : (T)0;
+        b[s] = (base + 4 * s < na && li < nb) ? ldc(B + sidx[rel + 4 * s] + (int64_t)(cb + li) * ldb) : (T)0;
     // loads unconditional (the block's first separator entry stands in), the
     // predicate and sign afterwards, so that none is waited for on its own
     const T* anchor = Pj + (int64_t)k0 * nj + nn;
@@ -556,7 +559,7 @@ __device__ __forceinline__ void sep_operands(const T* Pj, int64_t nj, int nn, in
 template <typename T>
 __device__ __forceinline__ void sep_partial(const T* Pj, int nj, int nn, int nsep, int k0, int kb,
                                             const int32_t* sidx, const T* B, int64_t ldb, int cb, int nb,
-                                            typename Mfma<T>::v4 (&part)[4]) {
+                                            typename Mfma<T>::v4 (&part)[4], int r0 = 0) {
     const int wv = threadIdx.x >> 6;
     const int nu = nsep > wv ? 8 * ((nsep - wv + 3) / 4) : 0;
     if (nu == 0) return;
@@ -567,12 +570,12 @@ __device__ __forceinline__ void sep_partial(const T* Pj, int nj, int nn, int nse
 #pragma unroll
             for (int i = 0; i < 4; ++i) part[i] = Mfma<T>::op(a[SEP_S * i + s], b[s], part[i]);
     };
-    sep_operands(Pj, nj, nn, k0, kb, B, ldb, cb, nb, wv, 0, sidx, a0, b0);
+    sep_operands(Pj, nj, nn, k0, kb, B, ldb, cb, nb, wv, 0, sidx, r0, a0, b0);
     // nu is a multiple of 8: chunks alternate between the two operand sets
     for (int u = 0; u < nu; u += 2) {
-        sep_operands(Pj, nj, nn, k0, kb, B, ldb, cb, nb, wv, u + 1, sidx, a1, b1);
+        sep_operands(Pj, nj, nn, k0, kb, B, ldb, cb, nb, wv, u + 1, sidx, r0, a1, b1);
         mma(a0, b0);
-        if (u + 2 < nu) sep_operands(Pj, nj, nn, k0, kb, B, ldb, cb, nb, wv, u + 2, sidx, a0, b0);
+        if (u + 2 < nu) sep_operands(Pj, nj, nn, k0, kb, B, ldb, cb, nb, wv, u + 2, sidx, r0, a0, b0);
         mma(a1, b1);
     }
 }
@@ -580,6 +583,48 @@ __device__ __forceinline__ void sep_partial(const T* Pj, int nj, int nn, int nse
 // LDS elements of a solve workgroup's panel buffer (a small front, four tiny
 // fronts, or the separator indices and partial sums of diag_bwd)
 constexpr int NPS = SMALL_NJ * LDS > 4 * TINY_NJ * LDT1 ? SMALL_NJ * LDS : 4 * TINY_NJ * LDT1;
+
+// Backward separator product of diagonal block [k0, k0 + kb) over the
+// separator tiles [t0, t1) (PT_SEPB): part = -L21(chunk rows, block)'
+// x_sep(chunk rows), the four wavefronts' sums added in wave order, stored to
+// the block's partial slot for the chunk (64 x nrhs, column stride 64).  The
+// front's diagonal tasks add the slots in chunk order, so the result does
+// not depend on timing.
+template <typename T>
+__device__ __attribute__((noinline)) void sep_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve<T>& S, const Front& f,
+                        int J, int k0, int t0, int t1, T* Xbuf, T* Red) {
+    typedef typename Mfma<T>::v4 v4;
+    const int tid = threadIdx.x, li = tid & 15, lk = (tid & 63) >> 4, wv = tid >> 6;
+    const int nn = f.nn, nj = nn + f.na, kb = min(NB, nn - k0);
+    const T* Pj = F.panel + f.poff;
+    const int nsep = (f.na + NB - 1) / NB, nchk = (nsep + SEP_TILES - 1) / SEP_TILES;
+    T* slot = S.Mv + (P.sboff1[J] + (int64_t)((k0 / NB) * nchk + t0 / SEP_TILES) * NB) * S.nrhs;
+    // the chunk's separator row indices in LDS (the X/Y buffer is unused here)
+    const int r0 = NB * t0, nr = min(f.na, NB * t1) - r0;
+    int32_t* si = reinterpret_cast<int32_t*>(Xbuf);
+    const int32_t* rows = P.rows + f.rowoff + nn + r0;
+    for (int e = tid; e < nr; e += blockDim.x) si[e] = rows[e];
+    __syncthreads();
+    for (int cb = 0; cb < S.nrhs; cb += RB) {
+        const int nb = min(RB, S.nrhs - cb);
+        v4 part[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part[i] = v4{0, 0, 0, 0};
+        sep_partial(Pj, nj, nn, t1 - t0, k0, kb, si, S.B, S.ldb, cb, nb, part, r0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Red[wv * 1024 + 16 * i + Mfma<T>::row(lk, v) + li * 64] = part[i][v];
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = acc_row<T>(v), e = row + li * 64;
+            const T x = ((Red[e] + Red[1024 + e]) + Red[2048 + e]) + Red[3072 + e];
+            if (row < kb && li < nb) stc(slot + row + (int64_t)(cb + li) * NB, x);
+        }
+        __syncthreads();
+    }
+}
 
 // Backward diagonal block [k0, k0+kb): acc = C_k - L21' x_sep - sum_{ib>k}
 // L_{ib,k}' X_ib (x_sep final: the ancestors are complete; X_ib as the
@@ -599,6 +644,14 @@ __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
     const int32_t* rows = P.rows + f.rowoff;
     const int p = (nn + NB - 1) / NB, kblk = k0 / NB;
     const int nsep = (nj - nn + NB - 1) / NB, nch = p - 1 - kblk;
+    const bool split = P.sboff1[J] >= 0;
+    const int nchk = (nsep + SEP_TILES - 1) / SEP_TILES;
+    if (split) {
+        // every PT_SEPB task of the front is done: no diagonal task of the
+        // front can finish before that, so done[J] counts only them so far
+        if (tid == 0) wait_geq(Y_.done + J, p * nchk, F.hang, F.ticket, F.stall);
+        __syncthreads();
+    }
     // A fragment of chain step t: L_{ib,k}' for ib = p - 1 - t
     auto cfrag = [&](int t, T (&a)[NB / 4]) {
         const int ib = p - 1 - t;
@@ -612,7 +665,20 @@ __device__ void diag_bwd(const DevPlan& P, const DevFactor<T>& F, const DevSolve
             const int row = acc_row<T>(v);
             acc[v] = (row < kb && li < nb) ? S.B[f.c0 + k0 + row + (int64_t)(cb + li) * S.ldb] : (T)0;
         }
-        if (nsep > 0) {
+        if (split) {
+            // the separator product computed by the front's PT_SEPB tasks:
+            // their partials (64 x nrhs per chunk), summed in chunk order
+            const T* slot0 = S.Mv + (P.sboff1[J] + (int64_t)kblk * nchk * NB) * S.nrhs;
+            for (int c = 0; c < nchk; ++c) {
+                const T* sl = slot0 + (int64_t)c * NB * S.nrhs;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = acc_row<T>(v);
+                    const T x = ldc(sl + min(row, kb - 1) + (int64_t)min(cb + li, S.nrhs - 1) * NB);
+                    acc[v] += (row < kb && li < nb) ? x : (T)0;
+                }
+            }
+        } else if (nsep > 0) {
             v4 part[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) part[i] = v4{0, 0, 0, 0};
@@ -789,6 +855,7 @@ __global__ __launch_bounds__(256, 2) void k_bsolve(DevPlan P, DevFactor<T> F, De
         }
         __syncthreads();
         if (tk.kind == PT_SMALL) small_bwd(P, F, S, f, Ps, X);
+        else if (tk.kind == PT_SEPB) sep_bwd(P, F, S, f, J, tk.a, tk.b, tk.c, XY, Ps);
         else diag_bwd(P, F, S, f, J, tk.a, tk.b, Y_, X, Y, Ps, &s_av);
         drain_stores();
         __syncthreads();

@@ -259,3 +259,27 @@ def test_fused_diagonal_tasks(ct, oracle, wide):
     G = ct.ChordalLDLt(A, P, S, tuning=tuning)
     ct.ldlt_(G)
     assert rel(G.d, d1) <= 1e-12 and rel(G.Lval, L1) <= 1e-12
+
+
+def test_tuned_parameter_sets(ct, oracle, wide):
+    """Every parameter set of the autotuner's table (the planner tunings the
+    benchmark runs with) gives the oracle's factor on the wide-front problem,
+    FP64 Cholesky and LDLt, and FP32 to single precision."""
+    import json
+    import os
+    A, P, S, (D0, L0, d0, _), (D1, L1, d1, _) = wide
+    table = os.path.join(os.path.dirname(ct.__file__), "tuning.json")
+    with open(table) as f:
+        sets = sorted({tuple(v["tuning"]) for v in json.load(f).values()})
+    assert sets
+    for tuning in sets:
+        print(f"tuning {tuning}", flush=True)
+        F = ct.ChordalCholesky(A, P, S, tuning=list(tuning))
+        ct.cholesky_(F)
+        assert rel(F.Dval, D0) <= 1e-12 and rel(F.Lval, L0) <= 1e-12
+        G = ct.ChordalLDLt(A, P, S, tuning=list(tuning))
+        ct.ldlt_(G)
+        assert rel(G.d, d1) <= 1e-12 and rel(G.Lval, L1) <= 1e-12
+        H = ct.ChordalCholesky(A, P, S, tuning=list(tuning), precision=ct.MF_FP32)
+        ct.cholesky_(H)
+        assert rel(H.Dval, D0) <= 1e-5 and rel(H.Lval, L0) <= 1e-5
