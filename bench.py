@@ -100,10 +100,14 @@ def workload(ct, name):
     if name in ("lap7_64", "lap7_64_mixed"):
         A = ct.problems.laplacian3d_7pt(64, 64, 64, seed=2)
         mp = name == "lap7_64_mixed"
+        # with FP32 fronts, one correction meets 1e-10 when the solves run in
+        # FP64 arithmetic (worst column 8.8e-11; 1.5e-10 with FP32 solves,
+        # which then need a second correction)
         return dict(A=A, order=ct.grid_nd(64, 64, 64), nrhs=16, seed=2, tag=name,
-                    precision=ct.MF_FP32 if mp else ct.MF_FP64, refine=mp,
+                    precision=ct.MF_FP32 if mp else ct.MF_FP64, refine=mp, solve_fp64=mp,
                     desc="3D 7-point Laplacian 64^3, w~U[0.5,1.5] + 1e-3 I, geometric ND, 16 RHS, "
-                         + ("FP32 fronts + FP64 iterative refinement to 1e-10" if mp
+                         + ("FP32 fronts + FP64 iterative refinement to 1e-10 (FP64-arithmetic "
+                            "solves on the widened FP32 factor)" if mp
                             else "plain FP64 Cholesky"))
     if name in ("st27_48", "st27_48_mixed"):
         A = ct.problems.stencil3d_27pt(48, 48, 48, seed=3)
@@ -231,7 +235,7 @@ def run_cpu_reference(args, ct):
 class Engine:
     """One factor object of a workload, its stream and device buffers."""
 
-    def __init__(self, ct, torch, A, P, S, precision, nrhs, B0, stream):
+    def __init__(self, ct, torch, A, P, S, precision, nrhs, B0, stream, solve_fp64=False):
         from cliquetrees_amd import _native
         self.ct, self.torch, self.lib, self._native = ct, torch, _native.hip(), _native
         self.A, self.n, self.nrhs = A, A.shape[0], nrhs
@@ -241,6 +245,10 @@ class Engine:
         self.stats = self.plan.stats()
         self.fac = ct.DeviceFactor(self.plan, precision)
         self.h = self.fac.handle
+        if precision == ct.MF_FP32 and solve_fp64:
+            # FP32 factorization, FP64-arithmetic solves on the widened factor:
+            # the refinement's corrections contract faster than with FP32 solves
+            self.c(self.lib.mf_factor_set_solve_fp64(self.h, 1))
         self.precision = precision
         flat = ct.flatindices(S, A)
         self.c(self.lib.mf_factor_set_pattern(self.h, len(flat), flat.ctypes.data_as(_native.i64p)))
@@ -342,7 +350,7 @@ def run_single(args, ct, torch, name):
     torch.cuda.set_stream(stream)
     rng = np.random.default_rng(W["seed"])
     B0 = rng.standard_normal((n, W["nrhs"]))
-    E = Engine(ct, torch, A, P, S, W["precision"], W["nrhs"], B0, stream)
+    E = Engine(ct, torch, A, P, S, W["precision"], W["nrhs"], B0, stream, W.get("solve_fp64", False))
     lib, h = E.lib, E.h
     step = E.step_refine if W["refine"] else E.step_plain
     if W["refine"]:

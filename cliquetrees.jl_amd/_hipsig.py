@@ -20,6 +20,7 @@ SIGNATURES = {
     "mf_factor_bytes": ([vp, i64p], ci),
     "mf_factor_set_stream": ([vp, vp], ci),
     "mf_factor_sync": ([vp], ci),
+    "mf_factor_set_solve_fp64": ([vp, ci], ci),
     "mf_factor_set_values": ([vp, vp, vp, ci], ci),
     "mf_factor_load": ([vp, vp, vp, vp, ci, ci], ci),
     "mf_factor_set_pattern": ([vp, i64, i64p], ci),

@@ -155,6 +155,11 @@ struct mf_factor {
     size_t Mv_bytes = 0;
     void* Bws = nullptr;
     size_t Bws_bytes = 0;
+    bool solve64 = false;       // FP32 factor: solves in FP64 arithmetic on a widened copy
+    void* sh_panel = nullptr;   // that copy: panels, diagonal-tile inverses, pivots (FP64)
+    void* sh_W = nullptr;
+    void* sh_d = nullptr;
+    bool sh_valid = false;
     void* sel_pool = nullptr;   // selected inversion: X, T, Lt, M, Sigma panels
     size_t sel_bytes = 0;
     int32_t* sel_cnt = nullptr; // finished selected-inversion tasks per front
@@ -191,6 +196,9 @@ struct mf_factor {
         if (tgt_panel) (void)hipFree(tgt_panel);
         if (tgt_diag) (void)hipFree(tgt_diag);
         if (sel_pool) (void)hipFree(sel_pool);
+        if (sh_panel) (void)hipFree(sh_panel);
+        if (sh_W) (void)hipFree(sh_W);
+        if (sh_d) (void)hipFree(sh_d);
         if (sel_cnt) (void)hipFree(sel_cnt);
         if (own) (void)hipStreamDestroy(own);
     }
@@ -266,6 +274,17 @@ DevFactor<T> dev_factor(mf_factor* f) {
     return F;
 }
 
+// An FP32 factor object with FP64 solves: the solves read the widened copy.
+bool solves64(const mf_factor* f) { return f->precision == MF_FP32 && f->solve64; }
+
+DevFactor<double> dev_factor_solve64(mf_factor* f) {
+    DevFactor<double> F = dev_factor<double>(f);
+    F.panel = (double*)f->sh_panel;
+    F.W = (double*)f->sh_W;
+    F.d = (double*)f->sh_d;
+    return F;
+}
+
 int ensure(void** p, size_t* have, size_t need) {
     if (*have >= need && *p) return MF_OK;
     if (*p) (void)hipFree(*p);
@@ -274,6 +293,18 @@ int ensure(void** p, size_t* have, size_t need) {
     if (hipMalloc(p, std::max<size_t>(need, 16)) != hipSuccess) return fail(MF_ENOMEM, "hipMalloc failed (workspace)");
     *have = std::max<size_t>(need, 16);
     return MF_OK;
+}
+
+// The FP64 copy of an FP32 factor for FP64-arithmetic solves: panels and
+// pivots widened, the 64 x 64 diagonal-tile inverses formed again in FP64
+// from the widened panels (buffers allocated by mf_factor_set_solve_fp64).
+void enqueue_widen(mf_factor* f) {
+    const HostPlan& H = f->plan->H;
+    hipStream_t s = f->stream;
+    launch_widen((const float*)f->panel, (double*)f->sh_panel, H.panel_elems, s);
+    launch_widen((const float*)f->d, (double*)f->sh_d, H.n, s);
+    launch_diag<double>(f->plan->D, dev_factor_solve64(f), 0, (int)H.diag.size(), true, s);
+    f->sh_valid = true;
 }
 
 // Launch sequence of one numeric factorization (values already in the panels).
@@ -300,6 +331,9 @@ void enqueue_factor(mf_factor* f) {
                  f->prof, f->prof ? f->trace_buf : nullptr,
                  (int32_t)H.dag.size(), H.tune.prio, H.tune.wg_per_cu};
         timed(f, MF_KC_DAG, pl->ref_flops_dag, 1, [&] { launch_dag<T>(P, F, D, s); });
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (f->solve64) enqueue_widen(f);
     }
 }
 
@@ -330,6 +364,9 @@ void enqueue_solve(mf_factor* f, T* B, int64_t ldb, int nrhs, void* mv = nullptr
     const HostPlan& H = pl->H;
     const DevPlan& P = pl->D;
     DevFactor<T> F = dev_factor<T>(f);
+    if constexpr (sizeof(T) == 8) {
+        if (solves64(f)) F = dev_factor_solve64(f);
+    }
     DevSolve<T> S{B, ldb, (T*)(mv ? mv : f->Mv), nrhs};
     double bytes = (double)sizeof(T) * (2.0 * (double)(H.ndz + H.nlz) + 4.0 * H.n * nrhs);
     timed(f, MF_KC_SOLVE, bytes, 1, [&] {
@@ -340,6 +377,8 @@ void enqueue_solve(mf_factor* f, T* B, int64_t ldb, int nrhs, void* mv = nullptr
 }
 
 size_t elt(const mf_factor* f) { return f->precision == MF_FP32 ? 4 : 8; }
+// element size of the solves' workspaces
+size_t solve_elt(const mf_factor* f) { return f->precision == MF_FP32 && !solves64(f) ? 4 : 8; }
 
 // The watchdog record of mf_sync.h's give_up, with the awaited counter named.
 std::string hang_message(const mf_factor* f, const int32_t* h) {
@@ -612,6 +651,24 @@ int mf_factor_set_stream(mf_factor* f, void* stream) {
     return MF_OK;
 }
 
+int mf_factor_set_solve_fp64(mf_factor* f, int on) {
+    if (!f) return fail(MF_EARG, "null factor");
+    if (f->precision != MF_FP32) return on ? fail(MF_EARG, "FP64 solves are for FP32 factor objects") : MF_OK;
+    const HostPlan& H = f->plan->H;
+    if (on && !f->sh_panel) {
+        auto grab = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)) == hipSuccess; };
+        if (!grab(&f->sh_panel, 8 * (size_t)H.panel_elems) || !grab(&f->sh_W, 8 * (size_t)H.w_elems) ||
+            !grab(&f->sh_d, 8 * (size_t)(H.n + 1)))
+            return fail(MF_ENOMEM, "hipMalloc failed (FP64 solve copy)");
+        f->alloc_bytes += 8 * (size_t)(H.panel_elems + H.w_elems + H.n + 1);
+    }
+    f->solve64 = on != 0;
+    f->sh_valid = false;
+    if (f->solve64 && f->factored) enqueue_widen(f);
+    HIPCHK(hipGetLastError());
+    return MF_OK;
+}
+
 int mf_factor_sync(mf_factor* f) {
     if (!f) return fail(MF_EARG, "null factor");
     HIPCHK(hipStreamSynchronize(f->stream));
@@ -640,6 +697,7 @@ int mf_factor_set_values(mf_factor* f, const double* Dval, const double* Lval, i
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     f->factored = false;
     f->sel_valid = false;
+    f->sh_valid = false;
     f->ref_stale = false;
     return MF_OK;
 }
@@ -758,6 +816,7 @@ int mf_factor_refresh(mf_factor* f, const double* nzval, int device) {
     if (!device) HIPCHK(hipStreamSynchronize(f->stream));
     f->factored = false;
     f->sel_valid = false;
+    f->sh_valid = false;
     f->ref_stale = true;
     return MF_OK;
 }
@@ -826,9 +885,10 @@ int mf_solve_natural(mf_factor* f, double* C, int64_t nrhs, int64_t ldc, int dev
     const HostPlan& H = f->plan->H;
     if (ldc < H.n) return fail(MF_EARG, "ldc < n");
     if (nrhs == 0 || H.n == 0) return MF_OK;
-    size_t es = elt(f);
+    if (solves64(f) && !f->sh_valid) return fail(MF_ESTATE, "FP64 solves of an FP32 factor: factorize after enabling them");
+    size_t es = solve_elt(f);
     if (ensure(&f->Mv, &f->Mv_bytes, es * (size_t)H.mv_elems1 * nrhs)) return MF_ENOMEM;
-    if (f->precision == MF_FP64 && device) {
+    if ((f->precision == MF_FP64 || solves64(f)) && device) {
         enqueue_solve<double>(f, C, ldc, (int)nrhs);
     } else {
         if (ensure(&f->Bws, &f->Bws_bytes, es * (size_t)H.n * nrhs)) return MF_ENOMEM;
@@ -845,7 +905,7 @@ int mf_solve_natural(mf_factor* f, double* C, int64_t nrhs, int64_t ldc, int dev
             stage = f->stage;
         }
         int64_t lds = device ? ldc : H.n;
-        if (f->precision == MF_FP32) {
+        if (f->precision == MF_FP32 && !solves64(f)) {
             // permutation by the identity = conversion
             id.resize(H.n);
             for (int64_t i = 0; i < H.n; ++i) id[i] = i + 1;
@@ -879,7 +939,8 @@ int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t
     const HostPlan& H = f->plan->H;
     if (ldb < H.n) return fail(MF_EARG, "ldb < n");
     if (nrhs == 0 || H.n == 0) return MF_OK;
-    size_t es = elt(f);
+    if (solves64(f) && !f->sh_valid) return fail(MF_ESTATE, "FP64 solves of an FP32 factor: factorize after enabling them");
+    size_t es = solve_elt(f);
     if (ensure(&f->Mv, &f->Mv_bytes, es * (size_t)H.mv_elems1 * nrhs)) return MF_ENOMEM;
     if (ensure(&f->Bws, &f->Bws_bytes, es * (size_t)H.n * nrhs)) return MF_ENOMEM;
     if (f->perm_host.size() != (size_t)H.n ||
@@ -898,7 +959,7 @@ int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t
         src = f->stage;
         lds = H.n;
     }
-    if (f->precision == MF_FP32) {
+    if (f->precision == MF_FP32 && !solves64(f)) {
         launch_permute<float>(src, lds, (float*)f->Bws, H.n, f->perm_dev, (int)H.n, (int)nrhs, false,
                               f->stream);
         enqueue_solve<float>(f, (float*)f->Bws, H.n, (int)nrhs);
@@ -925,7 +986,8 @@ int mf_solve_noalloc(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs,
     const HostPlan& H = f->plan->H;
     if (ldb < H.n) return fail(MF_EARG, "ldb < n");
     if (nrhs == 0 || H.n == 0) return MF_OK;
-    const size_t es = elt(f);
+    if (solves64(f) && !f->sh_valid) return fail(MF_ESTATE, "FP64 solves of an FP32 factor: factorize after enabling them");
+    const size_t es = solve_elt(f);
     const size_t mv_bytes = (es * (size_t)H.mv_elems1 * (size_t)nrhs + 255) & ~(size_t)255;
     const size_t need = mv_bytes + es * (size_t)H.n * (size_t)nrhs;
     if (need > MF_NOALLOC_SCRATCH_BYTES)
@@ -944,7 +1006,7 @@ int mf_solve_noalloc(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs,
         HIPCHK(hipMemcpyAsync(f->perm_dev, f->perm_host.data(), 8 * H.n, hipMemcpyHostToDevice,
                               f->stream));
     }
-    if (f->precision == MF_FP32) {
+    if (f->precision == MF_FP32 && !solves64(f)) {
         launch_permute<float>(B, ldb, (float*)bws, H.n, f->perm_dev, (int)H.n, (int)nrhs, false,
                               f->stream);
         enqueue_solve<float>(f, (float*)bws, H.n, (int)nrhs, mv);

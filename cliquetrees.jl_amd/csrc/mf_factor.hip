@@ -1455,6 +1455,14 @@ __global__ __launch_bounds__(256) void k_copyout(DevPlan P, DevFactor<T> F, doub
     if (dout && F.diag != 0 && threadIdx.x == 0) dout[g] = (double)F.d[g];
 }
 
+// dst[i] = (double)src[i]: the FP64 copy of an FP32 factor's panels that
+// the FP64-arithmetic solves read.
+__global__ void k_widen(const float* src, double* dst, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) dst[i] = (double)src[i];
+}
+
 inline int grid_for(int64_t count, int block) {
     int64_t g = (count + block - 1) / block;
     if (g > 65536) g = 65536;
@@ -1511,6 +1519,11 @@ void launch_dag(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, hipStr
     int grid = std::max(1, cus) * std::max(1, per);
     if (grid > D.ntask) grid = D.ntask;
     hipLaunchKernelGGL(k_dag<T>, dim3(grid), dim3(256), 0, s, P, F, D);
+}
+
+void launch_widen(const float* src, double* dst, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_widen, dim3(grid_for(count, 256)), dim3(256), 0, s, src, dst, count);
 }
 
 template <typename T>

@@ -102,6 +102,7 @@ void launch_dag(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, hipStr
 template <typename T>
 void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, double* Lval,
                     double* dout, int uplo, const double* adiag, hipStream_t s);
+void launch_widen(const float* src, double* dst, int64_t count, hipStream_t s);
 
 // ---- solves (mf_solve.hip) ----
 template <typename T>

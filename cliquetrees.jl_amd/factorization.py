@@ -181,7 +181,8 @@ class ChordalFactorization:
     DIAG = MF_CHOLESKY
 
     def __init__(self, A, P: Permutation | None = None, S: ChordalSymbolic | None = None, *,
-                 alg=None, snd="maximal", precision=MF_FP64, uplo="L", tuning=None):
+                 alg=None, snd="maximal", precision=MF_FP64, uplo="L", tuning=None,
+                 solve_fp64=False):
         A = sp.csc_matrix(A, dtype=np.float64)
         A.sort_indices()
         if P is None or S is None:
@@ -198,6 +199,10 @@ class ChordalFactorization:
         self._plan = device_plan(S, tuning)
         self._f = DeviceFactor(self._plan, precision)
         self._set_uplo(uplo)
+        if solve_fp64:
+            # FP32 fronts, solves in FP64 arithmetic on a widened copy of the
+            # factor (mf_factor_set_solve_fp64)
+            _check(_native.hip().mf_factor_set_solve_fp64(self._f.handle, 1))
         self._pattern = None
         self._matrix_key = None
         self.copyto_(A)

@@ -124,6 +124,13 @@ int mf_factor_set_uplo(mf_factor* f, int uplo);
  * NULL restores the factor's own stream). */
 int mf_factor_set_stream(mf_factor* f, void* stream);
 int mf_factor_sync(mf_factor* f);
+/* FP32 factor objects: solve in FP64 arithmetic (on = 1).  Each
+ * factorization then also writes an FP64 copy of the factor (panels and
+ * pivots widened, the 64 x 64 diagonal-tile inverses formed again in FP64),
+ * which every later solve reads; the refinement's corrections then contract
+ * faster than with FP32 solves.  No counterpart in the reference (its
+ * factors have one element type).  Needs 8 bytes per factor entry more. */
+int mf_factor_set_solve_fp64(mf_factor* f, int on);
 
 /*
  * Values of P A P' in the reference layout, as left by copyto!(F, A)

@@ -275,6 +275,42 @@ def test_mixed_precision_refinement(ct, oracle, cases, name):
     assert res2.max() <= 1e-10
 
 
+@pytest.mark.parametrize("name", ["lap3d_16", "st27_10"])
+def test_fp32_factor_fp64_solves(ct, oracle, cases, name):
+    """FP32 fronts with FP64-arithmetic solves on the widened factor: the
+    refinement reaches 1e-10 in no more corrections than with FP32 solves,
+    agrees with the FP64 oracle, and a new factorization refreshes the copy."""
+    A, order, snd = cases[name]
+    P, S = ct.symbolic(A, alg=order, snd=snd)
+    Dv, Lv = oracle.assemble(S, P, A)
+    D0, L0, d0, _ = oracle.chol_impl(S, Dv, Lv)
+    B = np.random.default_rng(6).standard_normal((A.shape[0], 16))
+    X0 = oracle.solve(S, P, D0, L0, d0, B)
+    F32 = ct.ChordalCholesky(A, P, S, precision=ct.MF_FP32)
+    ct.cholesky_(F32)
+    ct.refine(F32, B, tol=1e-10)
+    F = ct.ChordalCholesky(A, P, S, precision=ct.MF_FP32, solve_fp64=True)
+    ct.cholesky_(F)
+    # the FP32 factor itself is unchanged by the FP64 solves
+    assert np.array_equal(F.Dval, F32.Dval) and np.array_equal(F.Lval, F32.Lval)
+    X = ct.refine(F, B, tol=1e-10)
+    res = np.linalg.norm(A @ X - B, axis=0) / np.linalg.norm(B, axis=0)
+    assert res.max() <= 1e-10 and F.ir_iters <= F32.ir_iters
+    assert rel(X, X0) <= 1e-9
+    # one solve runs the FP64 kernels (a result no FP32 solve gives) and is
+    # as close to the FP64 solution as one with FP32 arithmetic, or closer
+    Y64, Y32 = ct.solve(F, B), ct.solve(F32, B)
+    assert not np.array_equal(Y64, Y32)
+    assert not np.array_equal(Y64, Y64.astype(np.float32).astype(np.float64))
+    assert rel(Y64, X0) <= 1.5 * rel(Y32, X0)
+    A2 = (A + sp.diags(np.full(A.shape[0], 0.5))).tocsc()
+    F.copyto_(A2)
+    ct.cholesky_(F)
+    X2 = ct.refine(F, B[:, :3], tol=1e-10)
+    res2 = np.linalg.norm(A2 @ X2 - B[:, :3], axis=0) / np.linalg.norm(B[:, :3], axis=0)
+    assert res2.max() <= 1e-10
+
+
 def test_refinement_fp64_and_vector(ct, cases):
     """With FP64 fronts one solve already meets the tolerance (0 corrections);
     a vector right-hand side keeps its shape."""
