@@ -601,3 +601,154 @@ i64 or_selinv(i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt,
     }
     return 0;
 }
+
+/* ---- completion ---------------------------------------------------------- */
+
+/* In-place lower Cholesky of the n x n lower triangle of A (ld lda), the
+ * unblocked column algorithm; returns 0 or the 1-based failing column. */
+static i64 chol_lower(double* A, i64 lda, i64 n) {
+    for (i64 j = 0; j < n; ++j) {
+        double d = A_(A, lda, j, j);
+        if (!(d > 0.0)) return j + 1;
+        d = sqrt(d);
+        A_(A, lda, j, j) = d;
+        for (i64 i = j + 1; i < n; ++i) A_(A, lda, i, j) /= d;
+        for (i64 k = j + 1; k < n; ++k) {
+            double l = A_(A, lda, k, j);
+            for (i64 i = k; i < n; ++i) A_(A, lda, i, k) -= A_(A, lda, i, j) * l;
+        }
+    }
+    return 0;
+}
+
+/*
+ * or_complete: complete_impl! (complete.jl:46-72) with complete_loop_snd!
+ * (:108-234), complete_loop_nod! (:237-402) and complete_send! (:404-451):
+ * Dval/Lval hold a matrix Sigma on the factor's pattern (lower triangles);
+ * on return they hold the Cholesky factor of the inverse of Sigma's
+ * maximum-determinant positive definite completion.  Per front, root first:
+ * F <- [Sigma11; Sigma21] and M22 from the stack (the parent's frontal
+ * matrix gathered through rel), C = chol(M22), L21 <- C^-1 L21,
+ * D11 <- D11 - L21' L21, L21 <- -C^-T L21, D11 <- R with D11 = R' R
+ * (R lower: revtri!, potrf!(:U), revtri!), L21 <- L21 R^-1, D11 <- R^-1
+ * (trtri!); children gather their M from F, which keeps the unmodified
+ * Sigma.  Returns 0, the failing local column of a potrf!, or 1 for a
+ * non-positive nn = 1 pivot (complete_loop_nod!'s convention).
+ */
+i64 or_complete(i64 nfr, const i64* resptr, const i64* sepptr, const i64* reltgt,
+                const i64* chdptr, const i64* chdtgt, const i64* Dptr, const i64* Lptr,
+                double* Dval, double* Lval, double* Mval, i64* Mptr, double* Fval) {
+    i64 ns = 0;
+    Mptr[1] = 0;
+    for (i64 j = nfr - 1; j >= 0; --j) {
+        i64 nn = resptr[j + 1] - resptr[j];
+        i64 na = sepptr[j + 1] - sepptr[j];
+        i64 nj = nn + na;
+        double* F = Fval;
+        double* D11 = Dval + Dptr[j] - 1;
+        double* L21 = Lval + Lptr[j] - 1;
+        /* F11 <- D11 (lower), F21 <- L21 */
+        for (i64 c = 0; c < nn; ++c) {
+            for (i64 r = c; r < nn; ++r) A_(F, nj, r, c) = A_(D11, nn, r, c);
+            for (i64 r = 0; r < na; ++r) A_(F, nj, nn + r, c) = A_(L21, na, r, c);
+        }
+        double* M = NULL;
+        if (na > 0) {
+            M = Mval + Mptr[ns];
+            ns -= 1;
+            for (i64 c = 0; c < na; ++c)
+                for (i64 r = c; r < na; ++r) A_(F, nj, nn + r, nn + c) = A_(M, na, r, c);
+            i64 info = chol_lower(M, na, na);
+            if (info) return info;
+            /* L21 <- C^-1 L21 (forward substitution, column by column) */
+            for (i64 c = 0; c < nn; ++c)
+                for (i64 r = 0; r < na; ++r) {
+                    double s = A_(L21, na, r, c);
+                    for (i64 k = 0; k < r; ++k) s -= A_(M, na, r, k) * A_(L21, na, k, c);
+                    A_(L21, na, r, c) = s / A_(M, na, r, r);
+                }
+            if (nn == 1) {
+                double dot = 0.0;
+                for (i64 r = 0; r < na; ++r) dot += L21[r] * L21[r];
+                D11[0] -= dot;
+            } else {
+                /* D11 <- D11 - L21' L21 (lower) */
+                for (i64 c = 0; c < nn; ++c)
+                    for (i64 r = c; r < nn; ++r) {
+                        double s = 0.0;
+                        for (i64 k = 0; k < na; ++k) s += A_(L21, na, k, r) * A_(L21, na, k, c);
+                        A_(D11, nn, r, c) -= s;
+                    }
+            }
+            /* L21 <- -C^-T L21 (back substitution) */
+            for (i64 c = 0; c < nn; ++c)
+                for (i64 r = na - 1; r >= 0; --r) {
+                    double s = A_(L21, na, r, c);
+                    for (i64 k = r + 1; k < na; ++k) s -= A_(M, na, k, r) * A_(L21, na, k, c);
+                    A_(L21, na, r, c) = s / A_(M, na, r, r);
+                }
+            for (i64 c = 0; c < nn; ++c)
+                for (i64 r = 0; r < na; ++r) A_(L21, na, r, c) = -A_(L21, na, r, c);
+        }
+        if (nn == 1) {
+            double d11 = D11[0];
+            if (!(d11 > 0.0)) return 1;
+            d11 = sqrt(d11);
+            for (i64 r = 0; r < na; ++r) L21[r] /= d11;
+            D11[0] = 1.0 / d11;
+        } else {
+            /* D11 = R' R with R lower: the lower Cholesky of the reversed
+             * matrix J D11 J is J R' J (revtri!, potrf!(:U), revtri!) */
+            double* Rv = (double*)malloc((size_t)(nn * nn) * sizeof(double));
+            for (i64 c = 0; c < nn; ++c)
+                for (i64 r = c; r < nn; ++r) A_(Rv, nn, nn - 1 - c, nn - 1 - r) = A_(D11, nn, r, c);
+            /* Rv now holds J D11 J in its lower triangle (entry (n-1-c, n-1-r) with n-1-c >= n-1-r) */
+            i64 info = chol_lower(Rv, nn, nn);
+            if (info) {
+                free(Rv);
+                return info;
+            }
+            /* R = J Rv' J: R(r, c) = Rv(n-1-c, n-1-r), lower */
+            for (i64 c = 0; c < nn; ++c) {
+                for (i64 r = 0; r < c; ++r) A_(D11, nn, r, c) = 0.0;
+                for (i64 r = c; r < nn; ++r) A_(D11, nn, r, c) = A_(Rv, nn, nn - 1 - c, nn - 1 - r);
+            }
+            free(Rv);
+            /* L21 <- L21 R^-1 (right solve, R lower: columns last to first) */
+            for (i64 c = nn - 1; c >= 0; --c) {
+                for (i64 k = c + 1; k < nn; ++k) {
+                    double a = A_(D11, nn, k, c);
+                    for (i64 r = 0; r < na; ++r) A_(L21, na, r, c) -= A_(L21, na, r, k) * a;
+                }
+                double inv = 1.0 / A_(D11, nn, c, c);
+                for (i64 r = 0; r < na; ++r) A_(L21, na, r, c) *= inv;
+            }
+            /* D11 <- R^-1 (lower) */
+            double* X = (double*)malloc((size_t)(nn * nn) * sizeof(double));
+            for (i64 c = 0; c < nn; ++c) {
+                for (i64 r = 0; r < c; ++r) A_(X, nn, r, c) = 0.0;
+                A_(X, nn, c, c) = 1.0 / A_(D11, nn, c, c);
+                for (i64 r = c + 1; r < nn; ++r) {
+                    double s = 0.0;
+                    for (i64 k = c; k < r; ++k) s += A_(D11, nn, r, k) * A_(X, nn, k, c);
+                    A_(X, nn, r, c) = -s / A_(D11, nn, r, r);
+                }
+            }
+            memcpy(D11, X, (size_t)(nn * nn) * sizeof(double));
+            free(X);
+        }
+        /* complete_send!: M_i <- R_i' F R_i (lower), increasing child order */
+        for (i64 q = chdptr[j]; q < chdptr[j + 1]; ++q) {
+            i64 i = chdtgt[q - 1] - 1;
+            i64 ma = sepptr[i + 1] - sepptr[i];
+            const i64* inj = reltgt + sepptr[i] - 1;
+            ns += 1;
+            i64 strt = Mptr[ns];
+            Mptr[ns + 1] = strt + ma * ma;
+            double* Mi = Mval + strt;
+            for (i64 c = 0; c < ma; ++c)
+                for (i64 r = c; r < ma; ++r) A_(Mi, ma, r, c) = A_(F, nj, inj[r] - 1, inj[c] - 1);
+        }
+    }
+    return 0;
+}

@@ -9,6 +9,7 @@ Functions mirror the reference's host verbs on the flat arrays of a
   * ``chol_impl`` — chol_impl!  (src/Multifrontal.jl/src/cholesky.jl:361-392)
   * ``solve``     — ldiv!(F, B) (src/Multifrontal.jl/src/divide.jl:191-206,1762-1764)
   * ``selinv``    — selinv!(F)  (src/Multifrontal.jl/src/selinv.jl:38-424)
+  * ``complete``  — complete!(F) (src/Multifrontal.jl/src/complete.jl:1-451)
 """
 from __future__ import annotations
 
@@ -52,6 +53,8 @@ def lib():
             f.restype = None
         L.or_selinv.argtypes = [ctypes.c_int64] + [i64p] * 7 + [f64p] * 3 + [i64p, f64p, f64p]
         L.or_selinv.restype = ctypes.c_int64
+        L.or_complete.argtypes = [ctypes.c_int64] + [i64p] * 7 + [f64p] * 3 + [i64p, f64p]
+        L.or_complete.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -239,3 +242,18 @@ def selinv(S, Dval, Lval):
     lib().or_selinv(S.nfr, *_sym_args(S), _p(Dval, f64p), _p(Lval, f64p), _p(Mval, f64p),
                     _p(Mptr, i64p), _p(Fval, f64p), _p(Xval, f64p))
     return Dval, Lval
+
+
+def complete(S, Dval, Lval):
+    """complete! on copies: Dval/Lval hold a matrix on the factor's pattern
+    (lower triangles); returns (Dval, Lval, info) with the Cholesky factor of
+    the inverse of its maximum-determinant completion (strict upper triangles
+    of the diagonal blocks set to 0)."""
+    Dval = np.array(Dval, dtype=np.float64, copy=True)
+    Lval = np.array(Lval, dtype=np.float64, copy=True)
+    Mval = np.zeros(max(selinv_stack_peak(S), 1))
+    Mptr = np.zeros(S.nfr + 2, dtype=np.int64)
+    Fval = np.zeros(max(S.nFval * S.nFval, 2))
+    info = lib().or_complete(S.nfr, *_sym_args(S), _p(Dval, f64p), _p(Lval, f64p), _p(Mval, f64p),
+                             _p(Mptr, i64p), _p(Fval, f64p))
+    return Dval, Lval, int(info)

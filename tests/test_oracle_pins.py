@@ -357,3 +357,43 @@ def test_selinv_diagonal_and_chain(ct, oracle):
     got = dense_lower(S, Ds, Ls)
     mask = _pattern(S)
     assert rel(got[mask], np.tril(inv)[mask]) <= 1e-14
+
+
+# ---- completion (complete.jl:1-451) ------------------------------------------
+
+@pytest.mark.parametrize("name", SPD)
+def test_complete_inverts_selinv(ct, oracle, spd, name):
+    """The inverse of PAP' is the maximum-determinant completion of its own
+    entries on the chordal pattern (its inverse PAP' vanishes off the
+    pattern), so complete!(selinv!(cholesky!(A))) is cholesky!(A) again."""
+    A, order, snd = spd[name]
+    P, S, D, L, d, info = _chol(ct, oracle, A, alg=order, snd=snd)
+    SD, SL = oracle.selinv(S, D, L)
+    CD, CL, info = oracle.complete(S, SD, SL)
+    assert info == 0
+    assert rel(dense_lower(S, CD, CL), dense_lower(S, D, L)) <= 1e-11
+
+
+def test_complete_of_a_dense_pattern(ct, oracle):
+    """On a dense 6 x 6 pattern (one front) the completion is the matrix
+    itself: the result is the Cholesky factor of its inverse."""
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((6, 6))
+    Sig = G @ G.T + 6 * np.eye(6)
+    P, S = ct.symbolic(sp.csc_matrix(np.ones((6, 6))))
+    Sp = permuted_dense(Sig, P)
+    Dv, Lv = to_layout(S, np.tril(Sp))
+    CD, CL, info = oracle.complete(S, Dv, Lv)
+    assert info == 0
+    want = np.linalg.cholesky(np.linalg.inv(Sp))
+    assert rel(dense_lower(S, CD, CL), want) <= 1e-12
+
+
+def test_complete_failure_info(ct, oracle):
+    """An indefinite separator block stops the completion with its potrf!
+    failure (a positive info)."""
+    A = ct.problems.laplacian2d_dirichlet(8, 8)
+    P, S, D, L, d, info = _chol(ct, oracle, A, alg=ct.grid_nd(8, 8, 1))
+    SD, SL = oracle.selinv(S, D, L)
+    CD, CL, info = oracle.complete(S, -SD, -SL)
+    assert info > 0
