@@ -607,6 +607,12 @@ __device__ __forceinline__ void blk_mma(const T* X, int ra, int ca, const T* Y, 
 template <typename T, int RULE>
 __device__ __forceinline__ void pivot_block(T (&x)[PB], const DevFactor<T>& F, const int gcol0, const int nk,
                                             const bool chol, T* cb, T* pv, T* rd, const int lane) {
+    // the pivot of the next step, as it will stand after this step's update:
+    // lane k+1 forms it from its own entries (x[k+1] - ar * ark is the same
+    // fused multiply-add as the rank-1 update below applies to its x[k+1],
+    // since the broadcast column entry of row k+1 is its own x[k]), so the
+    // chain from pivot to pivot skips the LDS round trip of the column
+    T dnext = bcast(x[0], 0);
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
         if (k >= nk) continue;  // uniform
@@ -618,7 +624,7 @@ __device__ __forceinline__ void pivot_block(T (&x)[PB], const DevFactor<T>& F, c
         // uniform reads (a wavefront's LDS operations complete in order)
         if (ln < PB) cb[ln] = x[k];
         __builtin_amdgcn_wave_barrier();
-        const T akk = bcast(x[k], k);
+        const T akk = dnext;
         double piv;
         bool bad;
         if (RULE == 0) {
@@ -650,6 +656,7 @@ __device__ __forceinline__ void pivot_block(T (&x)[PB], const DevFactor<T>& F, c
 #pragma unroll
         for (int j = k + 1; j < PB; ++j) colv[j] = cb[j];
         const T ark = x[k], ar = ark * ip;
+        if (k + 1 < PB) dnext = bcast(x[k + 1] - ar * ark, k + 1);
         if (ln == 0) {
             if (bad) atomicMin(F.info, (unsigned long long)(gcol0 + k + 1));
             else if (!chol) stc(F.d + gcol0 + k, (T)piv);
