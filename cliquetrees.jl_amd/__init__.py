@@ -6,7 +6,7 @@ The directory name is not a Python identifier; load it with
 """
 from .symbolic import ChordalSymbolic, Permutation, symbolic, flatindices, grid_nd, mmd  # noqa: F401
 from .factorization import (  # noqa: F401
-    ChordalCholesky, ChordalLDLt, ChordalFactorization, DynamicRegularization,
+    ChordalCholesky, ChordalLDLt, ChordalFactorization, DynamicRegularization, GMW81,
     PosDefException, SingularException, MFError, cholesky_, ldlt_, ldiv_, solve, rdiv_, rsolve, issuccess, refine,
     chol_impl, div_impl, selinv_, selinv_impl, device_plan, DevicePlan, DeviceFactor, checkinfo, TUNING_NAMES,
     save_factor, load_factor, FACTOR_FORMAT,

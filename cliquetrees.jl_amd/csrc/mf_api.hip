@@ -91,6 +91,7 @@ struct mf_plan {
     int32_t* d_ntf = nullptr;
     int32_t* d_ntb = nullptr;
     DagTask* d_dag = nullptr;
+    DagTask* d_dag_gmw = nullptr;  // the same tasks in creation order (GMW81 factorizations)
     DagFront* d_dagf = nullptr;
     int32_t* d_uwait_ptr = nullptr;
     int32_t* d_uwait = nullptr;
@@ -133,6 +134,7 @@ struct mf_factor {
     int32_t* prog = nullptr;
     uint32_t* ticket = nullptr;
     int32_t* hang = nullptr;
+    double* gmw = nullptr;      // GMW81: reduction scratch, then beta (gmw[2])
     int32_t* cnt = nullptr;     // tile-DAG counters
     int32_t* eacnt = nullptr;   // 2 (fronts + 1): all extend-add tasks, then panel ones
     unsigned long long* prof = nullptr;  // task-kind profile of the tile-DAG (NULL: off)
@@ -271,6 +273,7 @@ DevFactor<T> dev_factor(mf_factor* f) {
     F.reg_kind = f->reg_kind;
     F.delta = f->delta;
     F.eps = f->eps;
+    F.gmw = f->gmw;
     return F;
 }
 
@@ -319,6 +322,11 @@ void enqueue_factor(mf_factor* f) {
     launch_fill<int32_t>(f->hang, 16, 0, s);  // the watchdog record (the solves keep it)
     launch_fill<int32_t>(f->flags, std::max<int>(P.nJ, 1), 0, s);
     launch_fill<uint32_t>(f->ticket, 1, 0u, s);
+    if (f->reg_kind == MF_REG_GMW81) {
+        // initialize(L, signs, GMW81()) (gmw81.jl:50-63): beta of the assembled matrix
+        if (f->eps < 0) launch_gmw_beta<T>(P, F, f->gmw, H.n, s);
+        else launch_fill<double>(f->gmw + 2, 1, f->eps, s);
+    }
     timed(f, MF_KC_SMALL, pl->ref_flops_small, H.n_small_region, [&] {
         launch_small<T>(P, F, P.small_order, P.small_units, (int)H.small_units.size(), s);
     });
@@ -327,7 +335,8 @@ void enqueue_factor(mf_factor* f) {
         const size_t nfr1 = H.fronts.size() + 1;
         launch_fill<int32_t>(f->eacnt, 2 * nfr1, 0, s);
         launch_fill<uint32_t>(f->ticket, 1, 0u, s);
-        DevDag D{pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->eacnt + nfr1,
+        // GMW81: creation order, where a diagonal tile follows every update of its block column
+        DevDag D{f->reg_kind == MF_REG_GMW81 ? pl->d_dag_gmw : pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->eacnt + nfr1,
                  f->prof, f->prof ? f->trace_buf : nullptr,
                  (int32_t)H.dag.size(), H.tune.prio, H.tune.wg_per_cu};
         timed(f, MF_KC_DAG, pl->ref_flops_dag, 1, [&] { launch_dag<T>(P, F, D, s); });
@@ -494,6 +503,7 @@ static int plan_create(int64_t n, int64_t nfr, const int64_t* resptr, const int6
         (rc = upload(H.ntf, &pl->d_ntf, pl->allocs)) ||
         (rc = upload(H.ntb, &pl->d_ntb, pl->allocs)) ||
         (rc = upload(H.dag, &pl->d_dag, pl->allocs)) ||
+        (rc = upload(H.dag_gmw, &pl->d_dag_gmw, pl->allocs)) ||
         (rc = upload(H.dagf, &pl->d_dagf, pl->allocs)) ||
         (rc = upload(H.uwait_ptr, &pl->d_uwait_ptr, pl->allocs)) ||
         (rc = upload(H.uwait, &pl->d_uwait, pl->allocs)) ||
@@ -612,7 +622,7 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
               alloc((void**)&f->signs, 8 * (size_t)(H.n + 1)) &&
               alloc((void**)&f->info, 64) && alloc((void**)&f->flags, 4 * (H.fronts.size() + 1)) &&
               alloc((void**)&f->prog, 4 * (H.fronts.size() + 1)) &&
-              alloc((void**)&f->ticket, 64) && alloc((void**)&f->hang, 64) &&
+              alloc((void**)&f->ticket, 64) && alloc((void**)&f->hang, 64) && alloc((void**)&f->gmw, 64) &&
               alloc((void**)&f->cnt, 4 * (size_t)(H.dag_counters + 1)) &&
               alloc((void**)&f->eacnt, 8 * (H.fronts.size() + 1)) &&
               alloc((void**)&f->perm_dev, 8 * (size_t)(H.n + 1));
@@ -824,13 +834,20 @@ int mf_factor_refresh(mf_factor* f, const double* nzval, int device) {
 int mf_factorize_async(mf_factor* f, int diag, const double* signs_dev, int reg_kind,
                        double delta, double eps) {
     if (!f || (diag != MF_CHOLESKY && diag != MF_LDLT) ||
-        (reg_kind != MF_REG_NONE && reg_kind != MF_REG_DYNAMIC))
+        (reg_kind != MF_REG_NONE && reg_kind != MF_REG_DYNAMIC && reg_kind != MF_REG_GMW81))
         return fail(MF_EARG, "bad argument");
+    // checksigns (no_regularization.jl:17-25): GMW81 needs signs +-1, which
+    // LDLt's default (Zeros) is not
+    if (reg_kind == MF_REG_GMW81 && diag == MF_LDLT && !signs_dev)
+        return fail(MF_EARG, "GMW81 with LDLt needs signs");
     const HostPlan& H = f->plan->H;
     f->diag = diag;
     f->reg_kind = reg_kind;
     f->delta = delta;
     f->eps = eps;
+    if (reg_kind == MF_REG_GMW81 && delta < 0)  // gmw81_delta: cbrt(eps(T))
+        f->delta = f->precision == MF_FP32 ? (double)std::cbrt(1.1920928955078125e-07f)
+                                           : std::cbrt(2.220446049250313e-16);
     double* keep = f->signs;
     if (signs_dev) {
         if (signs_dev != f->signs)

@@ -46,6 +46,26 @@ __device__ __forceinline__ double regularize(const DevFactor<T>& F, double s, do
     return Djj;
 }
 
+// Julia's max of floats: NaN when either argument is NaN.
+__device__ __forceinline__ double jmax(double a, double b) {
+    if (a != a || b != b) return a + b;
+    return a > b ? a : b;
+}
+
+// GMW81 regularize (gmw81.jl:109-155): s * max(|Djj|, amax^2 / beta, delta),
+// amax the largest magnitude below the pivot in its front column as the
+// column stands after the previous pivots (GMW81 never fails a pivot).
+template <typename T>
+__device__ __forceinline__ double regularize_gmw(const DevFactor<T>& F, double s, double Djj, double amax) {
+    return s * jmax(jmax(fabs(Djj), amax * amax / F.gmw[2]), F.delta);
+}
+
+// Largest value over the wavefront (NaN-propagating), in every lane.
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = jmax(v, __shfl_xor(v, o));
+    return v;
+}
+
 // Waits until this wave's global stores (and loads) have completed.  Task
 // outputs are device-coherent stores, so once every wave has drained before
 // the closing barrier, the counter update that thread 0 makes afterwards
@@ -85,9 +105,13 @@ template <typename T>
 __device__ __forceinline__ void factor_lds(T* M, int nn, int nj, int gc0, const DevFactor<T>& F,
                            T* s_piv, T* s_inv, int* s_fail, const int tid) {
     for (int k = 0; k < nn; ++k) {
+        double amax = 0.0;
+        if (F.reg_kind == 2 && tid < 64)  // GMW81: the column below the pivot (nj <= 64 rows)
+            amax = wave_max((tid > k && tid < nj) ? fabs((double)M[tid + k * LDS]) : 0.0);
         if (tid == 0) {
             double akk = (double)M[k + k * LDS];
-            double piv = regularize(F, sign_of(F, gc0 + k), akk);
+            double piv = F.reg_kind == 2 ? regularize_gmw(F, sign_of(F, gc0 + k), akk, amax)
+                                         : regularize(F, sign_of(F, gc0 + k), akk);
             if (piv == 0.0) {
                 *s_fail = 1;
                 atomicMin(F.info, (unsigned long long)(gc0 + k + 1));
@@ -240,7 +264,13 @@ __device__ __forceinline__ void wave_front(const DevPlan& P, const DevFactor<T>&
     }
     for (int k = 0; k < nn; ++k) {
         const T akk = Mw[k + k * LDT1];
-        double piv = regularize(F, sign_of(F, gc0 + k), (double)akk);
+        double piv;
+        if (F.reg_kind == 2) {  // GMW81: the column below the pivot, one row per lane
+            const double amax = wave_max((r > k && r < nj) ? fabs((double)Mw[r + k * LDT1]) : 0.0);
+            piv = regularize_gmw(F, sign_of(F, gc0 + k), (double)akk, amax);
+        } else {
+            piv = regularize(F, sign_of(F, gc0 + k), (double)akk);
+        }
         if (piv == 0.0) {
             if (lane == 0) atomicMin(F.info, (unsigned long long)(gc0 + k + 1));
             return;  // the front stays unfinished; info reports the column
@@ -467,102 +497,6 @@ __device__ __forceinline__ void ea_columns(const DevPlan& P, const DevFactor<T>&
                 if (p >= e.row0 && p < e.row1) stc(dst + p, to_u ? acc[lane + 64 * j] : pre[j] + acc[lane + 64 * j]);
             }
         }
-    }
-}
-
-// Diagonal tile of a big front: factors the kb x kb (kb <= 64) block at local
-// column k0 (potrf2!'s right-looking loop, blas/potrf.jl:3-41, unless
-// invert_only) and writes W = L^-1 (unit L for LDLt), ld = nn, for the panel
-// solve and the triangular solves.  256 threads; thread (r, g) = (tid % 64,
-// tid / 64) keeps row r, columns g + 4j of the tile in registers and, for the
-// inverse, rows g + 4j of column r of W.  Step k publishes column k through
-// a double-buffered LDS vector, so each step costs one barrier.  The update
-// of step k uses the unscaled column: a_rc -= a_rk a_ck / piv_k, which is
-// l_rk l_ck for Cholesky and l_rk d_k l_ck for LDLt.
-template <typename T>
-__device__ void potrf_tile(const DevFactor<T>& F, const Front& f, int k0, bool invert_only,
-                           T* Ms, T* cb, const int tid) {
-    const int nj = f.nn + f.na, kb = min(NB, f.nn - k0);
-    const int r = tid & 63, cg = tid >> 6, gc0 = f.c0 + k0;
-    T* blk = F.panel + f.poff + k0 + (int64_t)k0 * nj;
-    T m[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int c = cg + 4 * j;
-        m[j] = (r < kb && c < kb && c <= r) ? blk[r + (int64_t)c * nj] : (r == c ? (T)1 : (T)0);
-    }
-    if (!invert_only) {
-#pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            if (k >= kb) continue;  // uniform; a constant trip count keeps m[] in registers
-            T* col = cb + (k & 1) * NB;
-            if (cg == (k & 3)) col[r] = m[k >> 2];
-            __syncthreads();
-            const T akk = col[k];
-            double piv = regularize(F, sign_of(F, gc0 + k), (double)akk);
-            const bool bad = (piv == 0.0);
-            if (bad) piv = 1.0;  // keeps the rest finite after a failure
-            if (tid == 0) {
-                if (bad) atomicMin(F.info, (unsigned long long)(gc0 + k + 1));
-                else if (F.diag != 0) stc(F.d + gc0 + k, (T)piv);
-            }
-            const T ip = (T)(1.0 / piv);
-            const T ark = col[r];
-            const T ar = ark * ip;
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int c = cg + 4 * j;
-                if (c > k) m[j] -= ar * col[c];
-            }
-            if (cg == (k & 3) && r >= k) {
-                if (F.diag == 0) {
-                    const T sq = (T)sqrt(piv);
-                    m[k >> 2] = (r == k) ? sq : ark * ((T)1 / sq);
-                } else if (r > k) {
-                    m[k >> 2] = ar;  // LDLt keeps the unregularized pivot on the diagonal
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int c = cg + 4 * j;
-            if (r < kb && c < kb && c <= r) blk[r + (int64_t)c * nj] = m[j];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) Ms[r + (cg + 4 * j) * LDS] = m[j];
-    __syncthreads();
-    const bool unit = F.diag != 0;
-    T w[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) w[j] = (cg + 4 * j == r) ? (T)1 : (T)0;
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        if (k >= kb) continue;
-        T* col = cb + (k & 1) * NB;
-        if (cg == (k & 3)) {
-            T x = w[k >> 2];
-            if (!unit) x = x / Ms[k + k * LDS];
-            w[k >> 2] = x;
-            col[r] = x;
-        }
-        __syncthreads();
-        const T x = col[r];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int i = cg + 4 * j;
-            if (i > k) w[j] -= Ms[i + k * LDS] * x;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 16; ++j) Ms[(cg + 4 * j) + r * LDS] = w[j];
-    __syncthreads();
-    T* W = F.W + f.woff + k0;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int c = cg + 4 * j;
-        if (r < kb && c < kb) W[r + (int64_t)c * f.nn] = Ms[r + c * LDS];
     }
 }
 
@@ -851,6 +785,95 @@ __device__ __forceinline__ void potrf_blocked(const DevFactor<T>& F, const Front
     }
 }
 
+// Diagonal tile under GMW81 (chol_factor! GMW81 specialization,
+// cholesky.jl:739-875): every pivot takes the largest magnitude below it in
+// the whole front column, so the task waits for its whole block column (each
+// tile final through the previous block column) and reads the rows below the
+// tile.  In the tile, right-looking column by column in LDS.  Below it,
+// column j as it stands after the tile's pivots 0..j-1 is
+//   a_ij - sum_{q<j} a_iq w_q,  w_q = sum_{p=q}^{j-1} l_jp W_pq,
+// with a the block column's values (the panel solve has not run yet), l row j
+// of the tile's L (unit for LDLt) and W = L^-1, built row by row alongside:
+// W_jq = -w_q / l_jj (l_jj = 1 for LDLt), W_jj = 1 / l_jj.  L goes to the
+// panel and W to the W pool, as the blocked path writes them.  LDS:
+// 2 NB LDS + NB elements and 6 doubles.  Only the GMW81 instance of the
+// persistent kernel holds it.
+template <typename T>
+__device__ __forceinline__ void potrf_gmw(const DevFactor<T>& F, const Front& f, int k0, T* lds,
+                                          const int tid) {
+    T* A = lds;                // the tile, 64 x LDS, lower triangle
+    T* Wt = lds + NB * LDS;    // W = L^-1 of the tile, row by row
+    T* w = Wt + NB * LDS;      // w of the current pivot
+    double* s_red = (double*)(w + NB);  // per-wave maxima, then the pivot's l and 1 / l
+    double& s_l = s_red[4];
+    double& s_inv = s_red[5];
+    const int nj = f.nn + f.na, kb = min(NB, f.nn - k0), gc0 = f.c0 + k0, nbelow = nj - k0;
+    T* blk = F.panel + f.poff + k0 + (int64_t)k0 * nj;
+    const int r = tid & 63, cg = tid >> 6, lane = tid & 63, wave = tid >> 6;
+    const bool chol = F.diag == 0;
+    for (int c = cg; c < NB; c += 4) {
+        A[r + c * LDS] = (r < kb && c < kb && c <= r) ? ldc(blk + r + (int64_t)c * nj) : (r == c ? (T)1 : (T)0);
+        Wt[r + c * LDS] = (T)0;
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        // (1) w_q = sum_{p=q}^{j-1} l_jp W_pq, one thread per q < j
+        if (tid < j) {
+            T acc = (T)0;
+            for (int p = tid; p < j; ++p) acc += A[j + p * LDS] * Wt[p + tid * LDS];
+            w[tid] = acc;
+        }
+        __syncthreads();
+        // (2) the largest magnitude below the pivot: tile rows from LDS, rows
+        // below the tile from the block column
+        double m = (r > j && r < kb && cg == 0) ? fabs((double)A[r + j * LDS]) : 0.0;
+        for (int i = kb + tid; i < nbelow; i += 256) {
+            const T* row = blk + i;
+            T v = ldc(row + (int64_t)j * nj);
+            for (int q = 0; q < j; ++q) v -= ldc(row + (int64_t)q * nj) * w[q];
+            m = jmax(m, fabs((double)v));
+        }
+        m = wave_max(m);
+        if (lane == 0) s_red[wave] = m;
+        __syncthreads();
+        if (tid == 0) {
+            const double amax = jmax(jmax(s_red[0], s_red[1]), jmax(s_red[2], s_red[3]));
+            const double piv = regularize_gmw(F, sign_of(F, gc0 + j), (double)A[j + j * LDS], amax);
+            if (chol) {
+                const double l = sqrt(piv);
+                s_l = l;
+                s_inv = 1.0 / l;
+            } else {
+                stc(F.d + gc0 + j, (T)piv);
+                s_l = piv;
+                s_inv = 1.0 / piv;
+            }
+        }
+        __syncthreads();
+        // (3) column j of L (the LDLt diagonal keeps the unregularized value)
+        // and row j of W
+        const T inv = (T)s_inv, pj = (T)s_l;
+        if (cg == 0) {
+            if (r > j && r < kb) A[r + j * LDS] *= inv;
+            if (r == j && chol) A[j + j * LDS] = pj;
+        } else if (cg == 1 && r <= j) {
+            Wt[j + r * LDS] = r == j ? (chol ? inv : (T)1) : -w[r] * (chol ? inv : (T)1);
+        }
+        __syncthreads();
+        // (4) rank-1 update of the trailing tile: a_rc -= l_rj (d_j) l_cj
+        if (r > j && r < kb) {
+            const T lr = A[r + j * LDS] * (chol ? (T)1 : pj);
+            for (int c = j + 1 + cg; c <= r; c += 4) A[r + c * LDS] -= lr * A[c + j * LDS];
+        }
+        __syncthreads();
+    }
+    for (int c = cg; c < kb; c += 4)
+        if (r < kb && c <= r) stc(blk + r + (int64_t)c * nj, A[r + c * LDS]);
+    T* W = F.W + f.woff + k0;
+    for (int c = cg; c < kb; c += 4)
+        if (r < kb) stc(W + r + (int64_t)c * f.nn, (c <= r) ? Wt[r + c * LDS] : (T)0);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag(DevPlan P, DevFactor<T> F, int t0, int invert_only) {
     __shared__ T lds[PTILE];
@@ -1076,7 +1099,7 @@ __device__ __forceinline__ void gemm_tile128(const DevFactor<T>& F, const GemmTa
 
 // Dependency q of DAG task d: the counter w that must reach target.  Returns
 // false when d has no dependency q; *nw is the number of dependencies.
-template <typename T>
+template <typename T, bool GMW>
 __device__ __forceinline__ bool dag_dep(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, const DagTask& d,
                         int q, int32_t** w, int* target, int* nw) {
     const int J = d.front;
@@ -1102,10 +1125,11 @@ __device__ __forceinline__ bool dag_dep(const DevPlan& P, const DevFactor<T>& F,
         }
         case DT_POTRF:
             // the diagonal tile (the fused products wait for their own tiles
-            // inside the task, after the factorization)
-            *nw = 1;
-            if (q >= 1) return false;
-            *w = d.a == 0 ? D.eapcnt + J : cnt + d.a * g.R + d.a;
+            // inside the task, after the factorization); under GMW81 every
+            // tile of the block column, updated through the previous column
+            *nw = (GMW && d.a > 0) ? g.R - d.a : 1;
+            if (q >= *nw) return false;
+            *w = d.a == 0 ? D.eapcnt + J : cnt + d.a * g.R + d.a + q;
             *target = d.a == 0 ? g.n_eap : d.a;
             return true;
         case DT_TRSM:
@@ -1171,11 +1195,14 @@ constexpr unsigned long long WMASK = WDEP == 64 ? ~0ull : (1ull << (WDEP & 63)) 
 // end every wave drains its stores, the workgroup meets at a barrier, and
 // thread 0 publishes with relaxed counter updates.  (Whole-L2 write-backs per
 // task cost 3 ms of the 35 ms launch on the 64^3 Laplacian.)
-template <typename T>
+// GMW: the GMW81 instance (diagonal tiles by potrf_gmw, block-column waits);
+// the other regularizations run the instance without it.
+template <typename T, bool GMW>
 __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDag D) {
     constexpr int S1 = PTILE, S2 = 2 * KC * LDT, S3 = 4 * KW * LDW;
     constexpr int SMEM = S1 > S2 ? (S1 > S3 ? S1 : S3) : (S2 > S3 ? S2 : S3);
-    __shared__ T smem[SMEM];
+    static_assert(SMEM >= 2 * NB * LDS + NB + 6 * (int)(sizeof(double) / sizeof(T)), "potrf_gmw LDS");
+    __shared__ __align__(16) T smem[SMEM];
     __shared__ T s_piv, s_inv;
     __shared__ int s_fail, s_sel;
     __shared__ int s_win[WIN];
@@ -1210,7 +1237,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                     if (t >= 0) {
                         const DagTask dd = D.task[t];
                         int nw = 0;
-                        my_has = dag_dep<T>(P, F, D, dd, q, &my_w, &my_tgt, &nw);
+                        my_has = dag_dep<T, GMW>(P, F, D, dd, q, &my_w, &my_tgt, &nw);
                         if (q == 0 && nw > WDEP) my_extra = nw;
                     }
                 }
@@ -1220,7 +1247,7 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                     for (int k = WDEP; k < my_extra && ok; ++k) {
                         int32_t* w2;
                         int tg2, nw2;
-                        if (dag_dep<T>(P, F, D, dd, k, &w2, &tg2, &nw2)) ok = ld_rel(w2) >= tg2;
+                        if (dag_dep<T, GMW>(P, F, D, dd, k, &w2, &tg2, &nw2)) ok = ld_rel(w2) >= tg2;
                     }
                 }
                 const unsigned long long okm = __ballot(ok);
@@ -1286,7 +1313,8 @@ __global__ __launch_bounds__(256, 2) void k_dag(DevPlan P, DevFactor<T> F, DevDa
                 small_front<T>(P, F, J, smem, &s_piv, &s_inv, &s_fail, tid);
                 break;
             case DT_POTRF:
-                potrf_blocked<T>(F, P.fronts[J], d.a * NB, false, smem, tid);
+                if constexpr (GMW) potrf_gmw<T>(F, P.fronts[J], d.a * NB, smem, tid);
+                else potrf_blocked<T>(F, P.fronts[J], d.a * NB, false, smem, tid);
                 if (d.b >= 0) {
                     // publish the diagonal tile (the other panel solves of column a
                     // need only it), then the fused panel solve of tile (a+1, a)
@@ -1521,11 +1549,58 @@ void launch_dag(const DevPlan& P, const DevFactor<T>& F, const DevDag& D, hipStr
     int dev = 0, cus = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_dag<T>, 256, 0);
+    const bool gmw = F.reg_kind == 2;
+    if (gmw) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_dag<T, true>, 256, 0);
+    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_dag<T, false>, 256, 0);
     if (D.wg_per_cu > 0 && D.wg_per_cu < per) per = D.wg_per_cu;
     int grid = std::max(1, cus) * std::max(1, per);
     if (grid > D.ntask) grid = D.ntask;
-    hipLaunchKernelGGL(k_dag<T>, dim3(grid), dim3(256), 0, s, P, F, D);
+    if (gmw) hipLaunchKernelGGL((k_dag<T, true>), dim3(grid), dim3(256), 0, s, P, F, D);
+    else hipLaunchKernelGGL((k_dag<T, false>), dim3(grid), dim3(256), 0, s, P, F, D);
+}
+
+// gmw81_beta (gmw81.jl:74-101): the largest diagonal and off-diagonal
+// magnitudes of the assembled matrix (the panels hold A before the
+// factorization), one workgroup per internal front, combined with atomicMax
+// on the bit patterns (non-negative doubles order as their bits; a NaN
+// orders above every number and propagates).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gmw_reduce(DevPlan P, DevFactor<T> F, unsigned long long* acc) {
+    const Front f = P.fronts[blockIdx.x];
+    const int nj = f.nn + f.na;
+    const T* pn = F.panel + f.poff;
+    double g = 0.0, x = 0.0;
+    const int64_t total = (int64_t)nj * f.nn;
+    for (int64_t e = threadIdx.x; e < total; e += 256) {
+        const int c = (int)(e / nj), r = (int)(e - (int64_t)c * nj);
+        if (r < c) continue;
+        const double v = fabs((double)pn[e]);
+        if (r == c) g = jmax(g, v);
+        else x = jmax(x, v);
+    }
+    g = wave_max(g);
+    x = wave_max(x);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(acc, (unsigned long long)__builtin_bit_cast(unsigned long long, g));
+        atomicMax(acc + 1, (unsigned long long)__builtin_bit_cast(unsigned long long, x));
+    }
+}
+
+// beta = max(gamma, xi / sqrt(n^2 - 1), eps(T)), rounded to T
+__global__ void k_gmw_final(double* gmw, double n, double eps_t, int f32) {
+    const unsigned long long* acc = (const unsigned long long*)gmw;
+    const double gamma = __builtin_bit_cast(double, acc[0]), xi = __builtin_bit_cast(double, acc[1]);
+    const double b = jmax(jmax(gamma, xi / sqrt(n * n - 1.0)), eps_t);
+    gmw[2] = f32 ? (double)(float)b : b;
+}
+
+template <typename T>
+void launch_gmw_beta(const DevPlan& P, const DevFactor<T>& F, double* gmw, int64_t n, hipStream_t s) {
+    launch_fill<unsigned long long>((unsigned long long*)gmw, 2, 0ull, s);
+    if (P.nJ > 0)
+        hipLaunchKernelGGL(k_gmw_reduce<T>, dim3(P.nJ), dim3(256), 0, s, P, F, (unsigned long long*)gmw);
+    const double eps_t = sizeof(T) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
+    hipLaunchKernelGGL(k_gmw_final, dim3(1), dim3(1), 0, s, gmw, (double)n, eps_t, sizeof(T) == 4 ? 1 : 0);
 }
 
 void launch_widen(const float* src, double* dst, int64_t count, hipStream_t s) {
@@ -1553,7 +1628,8 @@ void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, doubl
                                  hipStream_t);                                                 \
     template void launch_copyout<T>(const DevPlan&, const DevFactor<T>&, double*, double*,      \
                                     double*, int, const double*, hipStream_t);                  \
-    template void launch_dag<T>(const DevPlan&, const DevFactor<T>&, const DevDag&, hipStream_t);
+    template void launch_dag<T>(const DevPlan&, const DevFactor<T>&, const DevDag&, hipStream_t); \
+    template void launch_gmw_beta<T>(const DevPlan&, const DevFactor<T>&, double*, int64_t, hipStream_t);
 MF_INST(double)
 MF_INST(float)
 template void launch_fill<int32_t>(int32_t*, int64_t, int32_t, hipStream_t);

@@ -57,8 +57,9 @@ struct DevFactor {
     int32_t* hang;           // set when a bounded wait gives up
     unsigned long long stall;  // wall-clock ticks without progress before a wait gives up
     int diag;                // 0 Cholesky, 1 LDLt
-    int reg_kind;
-    double delta, eps;
+    int reg_kind;            // 0 none, 1 DynamicRegularization, 2 GMW81
+    double delta, eps;       // GMW81: delta = minimum pivot (eps unused)
+    const double* gmw;       // GMW81: gmw[2] = beta (written on the device before the factorization)
 };
 
 // Device view of the tile-DAG schedule of the big region (one factor object's
@@ -103,6 +104,10 @@ template <typename T>
 void launch_copyout(const DevPlan& P, const DevFactor<T>& F, double* Dval, double* Lval,
                     double* dout, int uplo, const double* adiag, hipStream_t s);
 void launch_widen(const float* src, double* dst, int64_t count, hipStream_t s);
+// GMW81's beta (gmw81.jl:74-101) from the assembled panels into gmw[2]
+// (gmw[0..1]: scratch of the reduction); n = matrix order, eps_t = eps(T).
+template <typename T>
+void launch_gmw_beta(const DevPlan& P, const DevFactor<T>& F, double* gmw, int64_t n, hipStream_t s);
 
 // ---- solves (mf_solve.hip) ----
 template <typename T>

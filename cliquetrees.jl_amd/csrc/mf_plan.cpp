@@ -1380,12 +1380,16 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         P.sim_makespan_us = now;
         std::vector<DagTask> dag2(NT);
         for (int32_t i = 0; i < NT; ++i) dag2[i] = P.dag[order[i]];
+        P.dag_gmw = P.dag;  // creation order: each block column complete before its diagonal tile
         P.dag.swap(dag2);
     }
 
     // ---- DAG order validation: every wait targets an earlier ticket ------------
-    {
-        const int32_t NT = (int32_t)P.dag.size();
+    // Run on the list-scheduled tickets and on the creation order that GMW81
+    // factorizations use (gmw: a diagonal-tile task also waits until every
+    // tile of its block column is updated through the previous column).
+    auto validate_dag = [&](const std::vector<DagTask>& dg, bool gmw) -> const char* {
+        const int32_t NT = (int32_t)dg.size();
         std::vector<int32_t> last_ea(NJ, -1), last_pea(NJ, -1), first_panel(NJ, INT32_MAX), last_done(NJ, -1);
         std::vector<int32_t> n_ea(NJ, 0), n_eap(NJ, 0);
         std::vector<int32_t> setter(P.dag_counters + 1, -1);  // ticket that finalizes (I, K) at value K+1
@@ -1394,7 +1398,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             return (int64_t)P.dagf[J].toff + (int64_t)K * P.dagf[J].R + I;
         };
         for (int32_t t = 0; t < NT; ++t) {
-            const DagTask& d = P.dag[t];
+            const DagTask& d = dg[t];
             const DagFront& D = P.dagf[d.front];
             switch (d.kind) {
                 case DT_EA:
@@ -1424,9 +1428,9 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                 return "dag: extend-add counts";
         std::vector<int32_t> last_consume(NJ, -1);
         for (int32_t t = 0; t < NT; ++t)
-            if (P.dag[t].kind == DT_EA || P.dag[t].kind == DT_SMALL) last_consume[P.dag[t].front] = t;
+            if (dg[t].kind == DT_EA || dg[t].kind == DT_SMALL) last_consume[dg[t].front] = t;
         for (int32_t t = 0; t < NT; ++t) {
-            const DagTask& d = P.dag[t];
+            const DagTask& d = dg[t];
             const Front& F = P.fronts[d.front];
             if (d.kind == DT_EA || d.kind == DT_SMALL) {
                 for (int32_t q = P.uwait_ptr[d.front]; q < P.uwait_ptr[d.front + 1]; ++q) {
@@ -1447,6 +1451,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             if (d.kind == DT_POTRF && d.a > 0) {
                 auto it = upd_at.find(cnt_id(d.front, d.a, d.a) * 4096 + d.a - 1);
                 if (it == upd_at.end() || it->second >= t) return "dag: potrf before its update";
+                for (int32_t I = d.a + 1; gmw && I < P.dagf[d.front].R; ++I) {
+                    auto ig = upd_at.find(cnt_id(d.front, I, d.a) * 4096 + d.a - 1);
+                    if (ig == upd_at.end() || ig->second >= t) return "dag: potrf before its block column";
+                }
                 // the fused products: their tiles carry the updates through a - 1
                 if (d.b >= 0) {
                     auto it2 = upd_at.find(cnt_id(d.front, d.a + 1, d.a) * 4096 + d.a - 1);
@@ -1485,7 +1493,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
                     return "dag: syrk tile and its row blocks disagree";
             }
         }
-    }
+        return nullptr;
+    };
+    if (const char* e = validate_dag(P.dag, false)) return e;
+    if (const char* e = validate_dag(P.dag_gmw, true)) return std::string("gmw81 order: ") + e;
 
     // ---- statistics ------------------------------------------------------------
     for (int64_t j = 0; j < nfr; ++j) {

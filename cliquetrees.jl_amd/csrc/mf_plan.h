@@ -70,6 +70,8 @@ struct HostPlan {
     std::vector<DiagTask> diag;
     std::vector<PTask> ftask, btask;   // forward / backward solve tasks, ticket order
     std::vector<DagTask> dag;          // tile-DAG factorization of the big region
+    std::vector<DagTask> dag_gmw;      // the same tasks in creation order (GMW81: a diagonal
+                                       // tile waits for its whole block column)
     std::vector<DagFront> dagf;        // per internal front (small region: n_done = 1)
     int64_t dag_counters = 0;          // tile counters
     std::vector<int32_t> uwait_ptr, uwait;  // per front: fronts whose extend-add must end

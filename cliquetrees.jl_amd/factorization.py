@@ -30,7 +30,7 @@ from .symbolic import ChordalSymbolic, Permutation, flatindices, symbolic
 
 MF_FP64, MF_FP32 = 0, 1
 MF_CHOLESKY, MF_LDLT = 0, 1
-MF_REG_NONE, MF_REG_DYNAMIC = 0, 1
+MF_REG_NONE, MF_REG_DYNAMIC, MF_REG_GMW81 = 0, 1, 2
 MF_UPLO_L, MF_UPLO_U = 0, 1
 _UPLO = {"L": MF_UPLO_L, "U": MF_UPLO_U}
 
@@ -88,6 +88,28 @@ class DynamicRegularization:
         d = self.delta if self.delta >= 0 else eps ** (1.0 / 3.0)
         e = self.epsilon if self.epsilon >= 0 else math.sqrt(eps)
         return d, e
+
+
+class GMW81:
+    """GMW81 (regularization/gmw81.jl:1-155), the Gill-Murray-Wright 1981
+    modified Cholesky: each pivot becomes s * max(|Djj|, Aimax^2 / beta, delta)
+    with Aimax the largest magnitude below it in its column.  Negative
+    parameters take the defaults: beta = gmw81_beta of the matrix (computed
+    on the device), delta = cbrt(eps(T))."""
+
+    def __init__(self, beta=-1.0, delta=-1.0):
+        self.beta = beta
+        self.delta = delta
+
+
+def _reg_args(reg):
+    """(kind, delta, eps) of mf_factorize for a regularization object."""
+    if reg is None:
+        return MF_REG_NONE, 0.0, 0.0
+    if isinstance(reg, GMW81):
+        return MF_REG_GMW81, float(reg.delta), float(reg.beta)
+    delta, eps = reg.initialize(np.finfo(np.float64).eps)
+    return MF_REG_DYNAMIC, delta, eps
 
 
 def _ptr(a):
@@ -317,10 +339,9 @@ def _factorize(F, diag, signs, reg, check):
         sp_ = _ptr(s)
     else:
         sp_ = None
-    kind, delta, eps = MF_REG_NONE, 0.0, 0.0
-    if reg is not None:
-        kind = MF_REG_DYNAMIC
-        delta, eps = reg.initialize(np.finfo(np.float64).eps)
+    kind, delta, eps = _reg_args(reg)
+    if kind == MF_REG_GMW81 and diag == MF_LDLT and sp_ is None:
+        raise ValueError("GMW81 needs signs (checksigns)")
     info = ctypes.c_int64(0)
     F._selinv = False
     _check(_native.hip().mf_factorize(F._f.handle, diag, sp_, kind, delta, eps, ctypes.byref(info)))
@@ -469,10 +490,7 @@ def chol_impl(S: ChordalSymbolic, Dval, Lval, diag=MF_CHOLESKY, signs=None, reg=
     if signs is not None:
         signs = np.ascontiguousarray(signs, dtype=np.float64)
         sp_ = _ptr(signs)
-    kind, delta, eps = MF_REG_NONE, 0.0, 0.0
-    if reg is not None:
-        kind = MF_REG_DYNAMIC
-        delta, eps = reg.initialize(np.finfo(np.float64).eps)
+    kind, delta, eps = _reg_args(reg)
     info = ctypes.c_int64(0)
     plan = device_plan(S)
     dp = None if diag == MF_CHOLESKY else _ptr(d)

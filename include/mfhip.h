@@ -40,7 +40,7 @@ enum {
 
 enum { MF_FP64 = 0, MF_FP32 = 1 };          /* precision of the fronts       */
 enum { MF_CHOLESKY = 0, MF_LDLT = 1 };      /* diag = :N / :U                */
-enum { MF_REG_NONE = 0, MF_REG_DYNAMIC = 1 };
+enum { MF_REG_NONE = 0, MF_REG_DYNAMIC = 1, MF_REG_GMW81 = 2 };
 enum { MF_UPLO_L = 0, MF_UPLO_U = 1 };     /* uplo = :L / :U                */
 
 /* Message of the last failing call on this thread. */
@@ -166,10 +166,15 @@ int mf_factor_refresh(mf_factor* f, const double* nzval, int device);
  * 181-203, 304-315), replacing chol_impl! (cholesky.jl:361-392).
  * diag: MF_CHOLESKY (:N) or MF_LDLT (:U).  signs: n pivot signs in permuted
  * order (NULL = Ones for Cholesky, Zeros for LDLt), host pointer.
- * reg_kind: MF_REG_NONE (NoRegularization, no_regularization.jl:27-42) or
+ * reg_kind: MF_REG_NONE (NoRegularization, no_regularization.jl:27-42),
  * MF_REG_DYNAMIC (DynamicRegularization(delta, eps),
- * dynamic_regularization.jl:75-90).  *info as chol_impl!.  The call
- * returns once the work is enqueued and *info is known (it synchronizes).
+ * dynamic_regularization.jl:75-90) or MF_REG_GMW81 (GMW81(beta, delta),
+ * regularization/gmw81.jl:37-155, with chol_factor!'s GMW81 form,
+ * cholesky.jl:739-875): there delta is the minimum pivot and eps is beta; a
+ * negative delta takes cbrt(eps(T)), a negative beta gmw81_beta of the
+ * assembled matrix (computed on the device).  GMW81 with LDLt needs signs
+ * (+-1, checksigns).  *info as chol_impl!.  The call returns once the work
+ * is enqueued and *info is known (it synchronizes).
  */
 int mf_factorize(mf_factor* f, int diag, const double* signs, int reg_kind, double delta,
                  double eps, int64_t* info);

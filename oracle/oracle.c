@@ -13,7 +13,9 @@
  *                  chol_factor!      cholesky.jl:694-737
  *                  chol_send!        cholesky.jl:950-983 (addscattertri!, utils.jl:555-579)
  *   potrf_rec      potrf! recursive  blas/potrf.jl:135-178, potrf2! blas/potrf.jl:3-41
- *   regularize     no_regularization.jl:27-42, dynamic_regularization.jl:75-90
+ *   regularize     no_regularization.jl:27-42, dynamic_regularization.jl:75-90,
+ *                  GMW81 regularization/gmw81.jl:109-155 (initialize :50-101)
+ *   chol_factor_gmw chol_factor! GMW81 specialization cholesky.jl:739-875
  *   trsm_rlt       trsx2_fwd!(:R)    blas/trsx.jl:11-31 (B <- B A^-T, A lower)
  *   or_div_fwd     div_impl! forward divide.jl:417-450, div_fwd_loop_snd!/nod! 493-821,
  *                  div_fwd_update!   divide.jl:1206-1248
@@ -67,11 +69,44 @@ void or_set_blas(void* potrf, void* trsm, void* syrk, void* gemm) {
 }
 
 /* ---- regularization ------------------------------------------------------ */
-enum { REG_NONE = 0, REG_DYNAMIC = 1 };
+enum { REG_NONE = 0, REG_DYNAMIC = 1, REG_GMW81 = 2 };
 typedef struct {
     int kind;
-    double delta, eps;
+    double delta, eps; /* GMW81: delta = minimum pivot, eps = beta */
 } reg_t;
+
+/* Julia's max for floats: NaN if either argument is NaN */
+static double jmax(double a, double b) {
+    if (a != a || b != b) return a + b;
+    return a > b ? a : b;
+}
+
+/* GMW81 regularize (gmw81.jl:109-155): s * max(|Djj|, Aimax^2 / beta, delta),
+ * Aimax the largest magnitude below the pivot in the pivot's front column */
+static double regularize_gmw(const reg_t* R, double s, double Djj, double Aimax) {
+    return s * jmax(jmax(fabs(Djj), Aimax * Aimax / R->eps), R->delta);
+}
+
+/* gmw81_beta (gmw81.jl:74-101) over the assembled factor storage:
+ * max(gamma, xi / sqrt(n^2 - 1), eps) with gamma the largest diagonal and xi
+ * the largest off-diagonal magnitude */
+double or_gmw81_beta(i64 nfr, const i64* resptr, const i64* sepptr, const i64* Dptr,
+                     const i64* Lptr, const double* Dval, const double* Lval) {
+    double gamma = 0.0, xi = 0.0;
+    i64 n = 0;
+    for (i64 j = 0; j < nfr; ++j) {
+        i64 nn = resptr[j + 1] - resptr[j], na = sepptr[j + 1] - sepptr[j];
+        const double* D = Dval + Dptr[j] - 1;
+        const double* L = Lval + Lptr[j] - 1;
+        n += nn;
+        for (i64 c = 0; c < nn; ++c) {
+            gamma = jmax(gamma, fabs(D[c * nn + c]));
+            for (i64 r = c + 1; r < nn; ++r) xi = jmax(xi, fabs(D[c * nn + r]));
+            for (i64 r = 0; r < na; ++r) xi = jmax(xi, fabs(L[c * na + r]));
+        }
+    }
+    return jmax(jmax(gamma, xi / sqrt((double)n * (double)n - 1.0)), 2.220446049250313e-16);
+}
 
 /* regularize(R, S, Djj, j): 0 signals a failed pivot. */
 static double regularize(const reg_t* R, double s, double Djj) {
@@ -205,6 +240,38 @@ static i64 potrf_rec(double* A, i64 lda, i64 n, double* d, const double* S, cons
     return 0;
 }
 
+/* chol_factor! for GMW81 (cholesky.jl:739-875): the stacked panel [D; L]
+ * (D nn x nn lower, L na x nn) factored column by column, each pivot
+ * regularized with the largest magnitude below it in the whole front column
+ * as it stands after the previous pivots.  The reference groups the columns
+ * in blocks of THRESHOLD = 64 and applies the updates of a block to the
+ * trailing columns with syrk!/gemm!; the products are the same. */
+static void chol_factor_gmw(double* D, i64 nn, double* L, i64 na, double* d, const double* S,
+                            const reg_t* R, int diag) {
+    for (i64 j = 0; j < nn; ++j) {
+        double amax = 0.0;
+        for (i64 i = j + 1; i < nn; ++i) amax = jmax(amax, fabs(A_(D, nn, i, j)));
+        for (i64 i = 0; i < na; ++i) amax = jmax(amax, fabs(A_(L, na, i, j)));
+        double Djj = regularize_gmw(R, S[j], A_(D, nn, j, j), amax);
+        if (diag == 0) {
+            Djj = sqrt(Djj);
+            A_(D, nn, j, j) = Djj;
+        } else {
+            d[j] = Djj;
+        }
+        double iD = 1.0 / Djj;
+        for (i64 i = j + 1; i < nn; ++i) A_(D, nn, i, j) *= iD;
+        for (i64 i = 0; i < na; ++i) A_(L, na, i, j) *= iD;
+        for (i64 k = j + 1; k < nn; ++k) {
+            double Dkj = A_(D, nn, k, j);
+            double f = diag == 0 ? Dkj : Djj * Dkj;
+            A_(D, nn, k, k) -= f * Dkj;
+            for (i64 i = k + 1; i < nn; ++i) A_(D, nn, i, k) -= A_(D, nn, i, j) * f;
+            for (i64 i = 0; i < na; ++i) A_(L, na, i, k) -= A_(L, na, i, j) * f;
+        }
+    }
+}
+
 /* ---- symbolic view ------------------------------------------------------- */
 typedef struct {
     i64 n, nfr;
@@ -251,7 +318,10 @@ i64 or_chol_range(i64 n, i64 nfr, const i64* resptr, const i64* sepptr, const i6
                   double* Fval, i64 j0, i64 j1) {
     reg_t R = {reg_kind, delta, eps};
     (void)n;
-    (void)nfr;
+    /* initialize(L, signs, GMW81()) (gmw81.jl:50-63): a negative beta takes
+     * gmw81_beta of the assembled matrix */
+    if (reg_kind == REG_GMW81 && eps < 0.0)
+        R.eps = or_gmw81_beta(nfr, resptr, sepptr, Dptr, Lptr, Dval, Lval);
     i64 ns = 0;
     Mptr[1] = 0; /* 1-based stack index, 0-based offsets */
     for (i64 j = j0; j < j1; ++j) {
@@ -274,7 +344,13 @@ i64 or_chol_range(i64 n, i64 nfr, const i64* resptr, const i64* sepptr, const i6
             /* chol_loop_nod! */
             double d11 = D11[0] + F[0];
             for (i64 r = 0; r < na; ++r) L21[r] += F[1 + r];
-            d11 = regularize(&R, signs[r0], d11);
+            if (R.kind == REG_GMW81) {
+                double amax = 0.0;
+                for (i64 r = 0; r < na; ++r) amax = jmax(amax, fabs(L21[r]));
+                d11 = regularize_gmw(&R, signs[r0], d11, amax);
+            } else {
+                d11 = regularize(&R, signs[r0], d11);
+            }
             if (d11 == 0.0) return 1 + r0;
             if (diag == 0) {
                 d11 = sqrt(d11);
@@ -311,11 +387,15 @@ i64 or_chol_range(i64 n, i64 nfr, const i64* resptr, const i64* sepptr, const i6
         /* chol_factor! */
         double* dj = diag ? d + r0 : NULL;
         double dummy_d[1];
-        i64 info = potrf_rec(D11, nn, nn, dj ? dj : dummy_d, signs + r0, &R, diag);
-        if (info) return info + r0;
+        if (R.kind == REG_GMW81) {
+            chol_factor_gmw(D11, nn, L21, na, dj, signs + r0, &R, diag);
+        } else {
+            i64 info = potrf_rec(D11, nn, nn, dj ? dj : dummy_d, signs + r0, &R, diag);
+            if (info) return info + r0;
+        }
         if (na > 0) {
-            trsm_rlt(D11, nn, L21, na, na, nn, diag);
-            if (diag) {
+            if (R.kind != REG_GMW81) trsm_rlt(D11, nn, L21, na, na, nn, diag);
+            if (diag && R.kind != REG_GMW81) {
                 for (i64 c = 0; c < nn; ++c) {
                     double idc = 1.0 / dj[c];
                     for (i64 r = 0; r < na; ++r) A_(L21, na, r, c) *= idc;

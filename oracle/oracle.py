@@ -55,6 +55,8 @@ def lib():
         L.or_selinv.restype = ctypes.c_int64
         L.or_complete.argtypes = [ctypes.c_int64] + [i64p] * 7 + [f64p] * 3 + [i64p, f64p]
         L.or_complete.restype = ctypes.c_int64
+        L.or_gmw81_beta.argtypes = [ctypes.c_int64] + [i64p] * 4 + [f64p] * 2
+        L.or_gmw81_beta.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -133,8 +135,19 @@ def assemble(S, P, A):
     return Dval, Lval
 
 
+def gmw81_beta(S, Dval, Lval):
+    """gmw81_beta (gmw81.jl:74-101) of the assembled factor storage."""
+    Dval = np.ascontiguousarray(Dval, dtype=np.float64)
+    Lval = np.ascontiguousarray(Lval, dtype=np.float64)
+    return float(lib().or_gmw81_beta(S.nfr, _p(S.resptr, i64p), _p(S.sepptr, i64p), _p(S.Dptr, i64p),
+                                     _p(S.Lptr, i64p), _p(Dval, f64p), _p(Lval, f64p)))
+
+
 def chol_impl(S, Dval, Lval, diag=0, signs=None, reg=None):
-    """In place on copies: returns (Dval, Lval, d, info); info as chol_impl!."""
+    """In place on copies: returns (Dval, Lval, d, info); info as chol_impl!.
+    reg: None, (delta, epsilon) for DynamicRegularization, or
+    ("gmw81", beta, delta) for GMW81 (a negative beta takes gmw81_beta of the
+    assembled matrix, a negative delta cbrt(eps))."""
     Dval = np.array(Dval, dtype=np.float64, copy=True)
     Lval = np.array(Lval, dtype=np.float64, copy=True)
     n = S.n
@@ -143,7 +156,10 @@ def chol_impl(S, Dval, Lval, diag=0, signs=None, reg=None):
         signs = np.ones(n) if diag == 0 else np.zeros(n)
     signs = np.ascontiguousarray(signs, dtype=np.float64)
     kind, delta, eps = 0, 0.0, 0.0
-    if reg is not None:
+    if reg is not None and reg[0] == "gmw81":
+        beta, dl = float(reg[1]), float(reg[2])
+        kind, delta, eps = 2, (dl if dl >= 0 else float(np.cbrt(np.finfo(np.float64).eps))), beta
+    elif reg is not None:
         kind, delta, eps = 1, float(reg[0]), float(reg[1])
     Mval = np.zeros(max(S.nMval, 1))
     Mptr = np.zeros(S.nMptr + 2, dtype=np.int64)
