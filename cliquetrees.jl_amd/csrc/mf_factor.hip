@@ -827,11 +827,37 @@ __device__ __forceinline__ void potrf_gmw(const DevFactor<T>& F, const Front& f,
         // (2) the largest magnitude below the pivot: tile rows from LDS, rows
         // below the tile from the block column
         double m = (r > j && r < kb && cg == 0) ? fabs((double)A[r + j * LDS]) : 0.0;
-        for (int i = kb + tid; i < nbelow; i += 256) {
-            const T* row = blk + i;
-            T v = ldc(row + (int64_t)j * nj);
-            for (int q = 0; q < j; ++q) v -= ldc(row + (int64_t)q * nj) * w[q];
-            m = jmax(m, fabs((double)v));
+        // RB rows per thread and QB columns per step: RB * QB independent
+        // loads in flight (the rows are device-coherent loads from far
+        // memory); out-of-range rows and columns read row kb / column 0 and
+        // are dropped
+        constexpr int RB = 4, QB = 4;
+        for (int i0 = kb; i0 < nbelow; i0 += 256 * RB) {
+            T acc[RB];
+            int64_t ro[RB];
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int i = i0 + tid + 256 * u;
+                ro[u] = i < nbelow ? i : kb;
+                acc[u] = ldc(blk + ro[u] + (int64_t)j * nj);
+            }
+            for (int q0 = 0; q0 < j; q0 += QB) {
+                T a[QB][RB], wq[QB];
+#pragma unroll
+                for (int qq = 0; qq < QB; ++qq) {
+                    const int q = q0 + qq < j ? q0 + qq : 0;
+                    wq[qq] = q0 + qq < j ? w[q] : (T)0;
+#pragma unroll
+                    for (int u = 0; u < RB; ++u) a[qq][u] = ldc(blk + ro[u] + (int64_t)q * nj);
+                }
+#pragma unroll
+                for (int qq = 0; qq < QB; ++qq)
+#pragma unroll
+                    for (int u = 0; u < RB; ++u) acc[u] -= a[qq][u] * wq[qq];
+            }
+#pragma unroll
+            for (int u = 0; u < RB; ++u)
+                if (i0 + tid + 256 * u < nbelow) m = jmax(m, fabs((double)acc[u]));
         }
         m = wave_max(m);
         if (lane == 0) s_red[wave] = m;
