@@ -9,7 +9,10 @@
 //      launch in ticket order; big region: levels by height;
 //   3. memory: panel pool (all fronts), U pool (small region: private slots;
 //      big region: slots reused after the parent's extend-add), W pool;
-//   4. static task lists of every launch of the factorization and the solves.
+//   4. static task lists of every launch of the factorization and the solves,
+//      the dataflow form of the tile-DAG, its tickets by list scheduling, and
+//      checks of every task's bounds and of both ticket orders.
+// build_plan runs these as the stages of class Planner, one method each.
 #include "mf_plan.h"
 
 #include <algorithm>
@@ -70,22 +73,53 @@ struct Allocator {
 
 inline int64_t rup(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
-}  // namespace
+// The planner's stages.  Each stage reads what the earlier ones left in the
+// HostPlan (and the few shared values below) and returns an empty string or
+// an error message; build_plan runs them in order.
+class Planner {
+  public:
+    Planner(HostPlan& plan, const PlanTuning& tune)
+        : P(plan), zero_frac(tune.zero_frac), UPD_GROUP(tune.upd_group), EA_ROWS(tune.ea_rows),
+          WIDE_K(tune.wide_k), SIM_WORKERS(tune.sim_workers), FUSE(tune.fuse != 0) {
+        P.tune = tune;
+    }
+    // the reference's arrays, 0-based, and their consistency
+    std::string read_input(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                           const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                           const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr);
+    std::string amalgamate();     // 1. internal fronts, children, rel maps, copy-out map
+    std::string classify();       // 2. small region, levels of the big region
+    std::string layout_memory();  // 3. panel, U (slot reuse), W pools, solve workspaces
+    std::string factor_tasks();   // 4. extend-add, diagonal and tile-DAG tasks
+    std::string solve_tasks();    // gather tables and persistent solve task lists
+    std::string check_bounds();   // every task's accesses inside its pools
+    std::string dataflow();       // predecessor counts and successor lists of the DAG
+    std::string schedule();       // tickets by list scheduling (and the GMW81 order)
+    std::string check_order();    // every wait of both orders targets an earlier ticket
+    std::string statistics();
 
-std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* resptr,
-                       const int64_t* sepptr, const int64_t* septgt, const int64_t* reltgt,
-                       const int64_t* chdptr, const int64_t* chdtgt, const int64_t* Dptr,
-                       const int64_t* Lptr, const PlanTuning& tune) {
+  private:
+    HostPlan& P;
+    const double zero_frac;
+    const int32_t UPD_GROUP, EA_ROWS, WIDE_K, SIM_WORKERS;
+    const bool FUSE;
+    int64_t n = 0, nfr = 0, nsep = 0;  // matrix order, reference fronts, separator entries
+    int32_t NJ = 0, nlev = 0;          // internal fronts, levels of the big region
+    std::vector<uint8_t> in_small_region;  // per internal front: whole subtree small
+    int64_t nnref(int64_t j) const { return P.resptr[j + 1] - P.resptr[j]; }
+    int64_t naref(int64_t j) const { return P.sepptr[j + 1] - P.sepptr[j]; }
+};
+
+std::string Planner::read_input(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t* sepptr,
+                                const int64_t* septgt, const int64_t* reltgt, const int64_t* chdptr,
+                                const int64_t* chdtgt, const int64_t* Dptr, const int64_t* Lptr) {
     (void)reltgt;
-    P.tune = tune;
-    const double zero_frac = tune.zero_frac;
-    const int32_t UPD_GROUP = tune.upd_group, EA_ROWS = tune.ea_rows, WIDE_K = tune.wide_k,
-                  SIM_WORKERS = tune.sim_workers;
-    const bool FUSE = tune.fuse != 0;
     if (n < 0 || nfr < 0) return "negative size";
     if (n > (int64_t)INT32_MAX / 2) return "n too large";
     P.n = n;
     P.nfr = nfr;
+    this->n = n;
+    this->nfr = nfr;
     P.resptr.resize(nfr + 1);
     P.sepptr.resize(nfr + 1);
     P.Dptr.resize(nfr + 1);
@@ -97,7 +131,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         P.Lptr[j] = Lptr[j] - 1;
     }
     if (P.resptr[0] != 0 || P.resptr[nfr] != n) return "resptr does not partition 1..n";
-    int64_t nsep = P.sepptr[nfr];
+    nsep = P.sepptr[nfr];
     P.septgt.resize(nsep);
     for (int64_t p = 0; p < nsep; ++p) {
         P.septgt[p] = septgt[p] - 1;
@@ -114,8 +148,6 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             P.pnt[c] = (int32_t)j;
         }
     }
-    auto nnref = [&](int64_t j) { return P.resptr[j + 1] - P.resptr[j]; };
-    auto naref = [&](int64_t j) { return P.sepptr[j + 1] - P.sepptr[j]; };
     for (int64_t j = 0; j < nfr; ++j) {
         if (nnref(j) <= 0) return "empty front";
         if (Dptr[j + 1] - Dptr[j] != nnref(j) * nnref(j)) return "Dptr inconsistent";
@@ -125,7 +157,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             if (p > P.sepptr[j] && P.septgt[p] <= P.septgt[p - 1]) return "separator not sorted";
         }
     }
+    return std::string();
+}
 
+std::string Planner::amalgamate() {
     // ---- 1. amalgamation --------------------------------------------------
     std::vector<int32_t> gowner(nfr);
     std::vector<int64_t> gcols(nfr), gzeros(nfr), gc0(nfr);
@@ -158,7 +193,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     for (int64_t j = 0; j < nfr; ++j)
         if (gowner[j] == j) groups.push_back((int32_t)j);
     std::sort(groups.begin(), groups.end(), [&](int32_t a, int32_t b) { return gtop[a] < gtop[b]; });
-    int32_t NJ = (int32_t)groups.size();
+    NJ = (int32_t)groups.size();
     std::vector<int32_t> gid(nfr, -1);
     for (int32_t J = 0; J < NJ; ++J) gid[groups[J]] = J;
     P.owner.resize(nfr);
@@ -232,10 +267,13 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             P.septpos[p] = (int32_t)(it - frow);
         }
     }
+    return std::string();
+}
 
+std::string Planner::classify() {
     // ---- 2. classification and levels --------------------------------------
     P.is_small.assign(NJ, 0);
-    std::vector<uint8_t> in_small_region(NJ, 0);
+    in_small_region.assign(NJ, 0);
     std::vector<int32_t> slev(NJ, 0), blev(NJ, 0);
     for (int32_t J = 0; J < NJ; ++J) {
         const Front& F = P.fronts[J];
@@ -253,7 +291,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         blev[J] = bl;
     }
     P.small_order.clear();
-    int32_t nlev = 0;
+    nlev = 0;
     for (int32_t J = 0; J < NJ; ++J) {
         if (in_small_region[J]) {
             P.small_order.push_back(J);
@@ -293,7 +331,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             P.n_big++;
         }
     }
+    return std::string();
+}
 
+std::string Planner::layout_memory() {
     // ---- 3. memory -----------------------------------------------------------
     int64_t poff = 0;
     for (int32_t J = 0; J < NJ; ++J) {
@@ -393,7 +434,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         mv += (int64_t)pk * nch * NB;
     }
     P.mv_elems1 = mv + 1;
+    return std::string();
+}
 
+std::string Planner::factor_tasks() {
     // ---- 4. task lists ---------------------------------------------------------
     P.level_small.clear();
     P.level_big.clear();
@@ -634,7 +678,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         }
         P.dag_counters = toff + 1;
     }
+    return std::string();
+}
 
+std::string Planner::solve_tasks() {
     // ---- solve gather tables ----------------------------------------------------
     // A solve task of a big front J covers one row block b of J (diagonal
     // blocks [64b, 64b + 64) for b < p, then update rows [nn + 64(b-p), ...)).
@@ -898,7 +945,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         }
         tl->swap(out);
     }
+    return std::string();
+}
 
+std::string Planner::check_bounds() {
     // ---- bounds validation of every task against the pools ----------------------
     {
         auto pool = [&](int8_t a) {
@@ -1067,7 +1117,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
             }
         }
     }
+    return std::string();
+}
 
+std::string Planner::dataflow() {
     // ---- dataflow form of the DAG ------------------------------------------------
     // Predecessor edges per tile: POTRF(K) <- last update of (K,K) (or every
     // extend-add task of the front for K = 0); TRSM(I,K) <- POTRF(K), last
@@ -1274,7 +1327,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         if (executed != NT) return "dataflow: not every task runs (cycle or missing release)";
         if ((int64_t)P.init_hi.size() > P.n_hi || (int64_t)P.init_lo.size() > P.n_lo) return "dataflow: queue sizes";
     }
+    return std::string();
+}
 
+std::string Planner::schedule() {
     // ---- ticket order by list scheduling ------------------------------------------
     // The persistent kernel hands out tasks in ticket order and a task waits
     // for its predecessors, so the order decides how much the workgroups wait.
@@ -1383,7 +1439,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         P.dag_gmw = P.dag;  // creation order: each block column complete before its diagonal tile
         P.dag.swap(dag2);
     }
+    return std::string();
+}
 
+std::string Planner::check_order() {
     // ---- DAG order validation: every wait targets an earlier ticket ------------
     // Run on the list-scheduled tickets and on the creation order that GMW81
     // factorizations use (gmw: a diagonal-tile task also waits until every
@@ -1497,7 +1556,10 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     };
     if (const char* e = validate_dag(P.dag, false)) return e;
     if (const char* e = validate_dag(P.dag_gmw, true)) return std::string("gmw81 order: ") + e;
+    return std::string();
+}
 
+std::string Planner::statistics() {
     // ---- statistics ------------------------------------------------------------
     for (int64_t j = 0; j < nfr; ++j) {
         double a = (double)nnref(j), b = (double)naref(j);
@@ -1508,6 +1570,27 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
         P.engine_flops += a * a * a / 3.0 + b * a * a + b * b * a;
     }
     return std::string();
+}
+
+}  // namespace
+
+std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* resptr,
+                       const int64_t* sepptr, const int64_t* septgt, const int64_t* reltgt,
+                       const int64_t* chdptr, const int64_t* chdtgt, const int64_t* Dptr,
+                       const int64_t* Lptr, const PlanTuning& tune) {
+    Planner pl(P, tune);
+    std::string e = pl.read_input(n, nfr, resptr, sepptr, septgt, reltgt, chdptr, chdtgt, Dptr, Lptr);
+    if (e.empty()) e = pl.amalgamate();
+    if (e.empty()) e = pl.classify();
+    if (e.empty()) e = pl.layout_memory();
+    if (e.empty()) e = pl.factor_tasks();
+    if (e.empty()) e = pl.solve_tasks();
+    if (e.empty()) e = pl.check_bounds();
+    if (e.empty()) e = pl.dataflow();
+    if (e.empty()) e = pl.schedule();
+    if (e.empty()) e = pl.check_order();
+    if (e.empty()) e = pl.statistics();
+    return e;
 }
 
 std::string tuning_from(const int64_t* v, int64_t nv, PlanTuning* t) {
