@@ -106,6 +106,9 @@ class Planner {
     int64_t n = 0, nfr = 0, nsep = 0;  // matrix order, reference fronts, separator entries
     int32_t NJ = 0, nlev = 0;          // internal fronts, levels of the big region
     std::vector<uint8_t> in_small_region;  // per internal front: whole subtree small
+    // GMW81: (diagonal-tile task, last update of a tile below it in its block
+    // column) edges, in creation-order task numbers
+    std::vector<std::pair<int32_t, int32_t>> gmw_edges;
     int64_t nnref(int64_t j) const { return P.resptr[j + 1] - P.resptr[j]; }
     int64_t naref(int64_t j) const { return P.sepptr[j + 1] - P.sepptr[j]; }
 };
@@ -1164,6 +1167,11 @@ std::string Planner::dataflow() {
                     } else if (!edge(lastw[tile(J, d.a, d.a)], t)) {
                         return "dataflow: potrf without its update";
                     }
+                    for (int32_t I = d.a + 1; d.a > 0 && I < P.dagf[J].R; ++I) {
+                        const int32_t pr = lastw[tile(J, I, d.a)];
+                        if (pr < 0 || pr >= t) return "dataflow: block column update after its diagonal tile";
+                        gmw_edges.push_back({t, pr});
+                    }
                     fin[tile(J, d.a, d.a)] = t;
                     if (d.b >= 0) {  // fused panel solve of (a+1, a)
                         if (d.a > 0 && !edge(lastw[tile(J, d.a + 1, d.a)], t))
@@ -1371,73 +1379,119 @@ std::string Planner::schedule() {
             }
             cost[t] = c;
         }
-        // longest path to the end, in reverse topological (static) order
-        for (int32_t t = NT - 1; t >= 0; --t) {
-            const DagTask& d = P.dag[t];
-            double m = 0;
-            for (int32_t k = P.succ_ptr[t]; k < P.succ_ptr[t + 1]; ++k) m = std::max(m, bl[P.succ[k]]);
-            const int32_t J = d.front;
-            if (d.kind == DT_EA && !d.c) m = std::max(m, bsyrk[J]);  // the front's update-matrix tiles
-            if (d.kind == DT_SYRK || d.kind == DT_SMALL) {
-                const int32_t par = P.fronts[J].parent;
-                if (par >= 0) m = std::max(m, bgate[par]);
+        // list-schedules the tasks under the successor lists (sp, sc) and
+        // predecessor counts d0; the start order goes to order
+        auto simulate = [&](const std::vector<int32_t>& sp, const std::vector<int32_t>& sc,
+                            const std::vector<int32_t>& d0, std::vector<int32_t>& order,
+                            double& makespan) -> const char* {
+            std::fill(bl.begin(), bl.end(), 0.0);
+            std::fill(bgate.begin(), bgate.end(), 0.0);
+            std::fill(bsyrk.begin(), bsyrk.end(), 0.0);
+            // longest path to the end, in reverse topological (static) order
+            for (int32_t t = NT - 1; t >= 0; --t) {
+                const DagTask& d = P.dag[t];
+                double m = 0;
+                for (int32_t k = sp[t]; k < sp[t + 1]; ++k) m = std::max(m, bl[sc[k]]);
+                const int32_t J = d.front;
+                if (d.kind == DT_EA && !d.c) m = std::max(m, bsyrk[J]);  // the front's update-matrix tiles
+                if (d.kind == DT_SYRK || d.kind == DT_SMALL) {
+                    const int32_t par = P.fronts[J].parent;
+                    if (par >= 0) m = std::max(m, bgate[par]);
+                }
+                if (d.kind == DT_EA || d.kind == DT_SMALL)
+                    for (int32_t q = P.uwaiters_ptr[J]; q < P.uwaiters_ptr[J + 1]; ++q)
+                        m = std::max(m, bgate[P.uwaiters[q]]);
+                bl[t] = cost[t] + m;
+                if (d.kind == DT_EA || d.kind == DT_SMALL) bgate[J] = std::max(bgate[J], bl[t]);
+                if (d.kind == DT_SYRK) bsyrk[J] = std::max(bsyrk[J], bl[t]);
             }
-            if (d.kind == DT_EA || d.kind == DT_SMALL)
-                for (int32_t q = P.uwaiters_ptr[J]; q < P.uwaiters_ptr[J + 1]; ++q)
-                    m = std::max(m, bgate[P.uwaiters[q]]);
-            bl[t] = cost[t] + m;
-            if (d.kind == DT_EA || d.kind == DT_SMALL) bgate[J] = std::max(bgate[J], bl[t]);
-            if (d.kind == DT_SYRK) bsyrk[J] = std::max(bsyrk[J], bl[t]);
-        }
-        typedef std::pair<double, int32_t> Item;
-        std::priority_queue<Item> ready;                                     // (bl, -t)
-        std::priority_queue<Item, std::vector<Item>, std::greater<Item>> run;  // (finish, t)
-        std::vector<int32_t> dep(P.dep0), gate(P.gate0), eacnt(NJ, 0), done(NJ, 0), eaucnt(NJ, 0);
-        auto push = [&](int32_t s) { ready.push({bl[s], -s}); };
-        for (int32_t t : P.init_hi) push(t);
-        for (int32_t t : P.init_lo) push(t);
-        auto gate_dec = [&](int32_t G) {
-            if (--gate[G] == 0)
-                for (int32_t e = P.gtask0[G]; e < P.gtask1[G]; ++e) push(e);
+            typedef std::pair<double, int32_t> Item;
+            std::priority_queue<Item> ready;                                     // (bl, -t)
+            std::priority_queue<Item, std::vector<Item>, std::greater<Item>> run;  // (finish, t)
+            std::vector<int32_t> dep(d0), gate(P.gate0), eacnt(NJ, 0), done(NJ, 0), eaucnt(NJ, 0);
+            auto push = [&](int32_t s) { ready.push({bl[s], -s}); };
+            for (int32_t t : P.init_hi) push(t);
+            for (int32_t t : P.init_lo) push(t);
+            auto gate_dec = [&](int32_t G) {
+                if (--gate[G] == 0)
+                    for (int32_t e = P.gtask0[G]; e < P.gtask1[G]; ++e) push(e);
+            };
+            order.clear();
+            order.reserve(NT);
+            double now = 0;
+            int32_t busy = 0;
+            while ((int32_t)order.size() < NT) {
+                while (busy < SIM_WORKERS && !ready.empty()) {
+                    const int32_t t = -ready.top().second;
+                    ready.pop();
+                    order.push_back(t);
+                    run.push({now + cost[t], t});
+                    ++busy;
+                }
+                if (run.empty()) return "list schedule: no runnable task";
+                const Item fin = run.top();
+                run.pop();
+                now = fin.first;
+                --busy;
+                const int32_t t = fin.second;
+                for (int32_t k = sp[t]; k < sp[t + 1]; ++k)
+                    if (--dep[sc[k]] == 0) push(sc[k]);
+                const DagTask& d = P.dag[t];
+                const int32_t J = d.front;
+                bool consumed = false, fdone = false;
+                if (d.kind == DT_EA && !d.c && ++eaucnt[J] == P.n_eau[J])
+                    for (int32_t q = P.eau_ptr[J]; q < P.eau_ptr[J + 1]; ++q)
+                        if (--dep[P.eau_wait[q]] == 0) push(P.eau_wait[q]);
+                if (d.kind == DT_EA) consumed = ++eacnt[J] == P.dagf[J].n_ea;
+                else if (d.kind == DT_SMALL) consumed = fdone = true;
+                else if (d.kind == DT_SYRK) fdone = ++done[J] == P.dagf[J].n_done;
+                if (consumed)
+                    for (int32_t q = P.uwaiters_ptr[J]; q < P.uwaiters_ptr[J + 1]; ++q) gate_dec(P.uwaiters[q]);
+                if (fdone && P.fronts[J].parent >= 0) gate_dec(P.fronts[J].parent);
+            }
+            makespan = now;
+            return nullptr;
         };
-        std::vector<int32_t> order;
-        order.reserve(NT);
-        double now = 0;
-        int32_t busy = 0;
-        while ((int32_t)order.size() < NT) {
-            while (busy < SIM_WORKERS && !ready.empty()) {
-                const int32_t t = -ready.top().second;
-                ready.pop();
-                order.push_back(t);
-                run.push({now + cost[t], t});
-                ++busy;
+        std::vector<int32_t> order, order_gmw;
+        if (const char* e = simulate(P.succ_ptr, P.succ, P.dep0, order, P.sim_makespan_us)) return e;
+        // GMW81 launches: each diagonal-tile task also waits for the last
+        // update of every tile below it in its block column, and takes about
+        // 200 us (it reads the block column once per pivot)
+        {
+            std::vector<std::pair<int32_t, int32_t>> ge(gmw_edges);  // (successor, predecessor)
+            for (int32_t t = 0; t < NT; ++t)
+                for (int32_t k = P.succ_ptr[t]; k < P.succ_ptr[t + 1]; ++k) ge.push_back({P.succ[k], t});
+            std::sort(ge.begin(), ge.end(), [](const std::pair<int32_t, int32_t>& x,
+                                               const std::pair<int32_t, int32_t>& y) {
+                return x.second != y.second ? x.second < y.second : x.first < y.first;
+            });
+            ge.erase(std::unique(ge.begin(), ge.end()), ge.end());
+            // predecessor counts: the block-column edges that are not already
+            // edges (the fused panel solve's update is one), plus the events
+            // that P.dep0 counts besides its edges
+            std::vector<int32_t> sp2(NT + 1, 0), sc2(ge.size()), d2(P.dep0);
+            for (int32_t t = 0; t < NT; ++t)
+                for (int32_t k = P.succ_ptr[t]; k < P.succ_ptr[t + 1]; ++k) d2[P.succ[k]]--;
+            for (auto& e : ge) {
+                sp2[e.second + 1]++;
+                d2[e.first]++;
             }
-            if (run.empty()) return "list schedule: no runnable task";
-            const Item fin = run.top();
-            run.pop();
-            now = fin.first;
-            --busy;
-            const int32_t t = fin.second;
-            for (int32_t k = P.succ_ptr[t]; k < P.succ_ptr[t + 1]; ++k)
-                if (--dep[P.succ[k]] == 0) push(P.succ[k]);
-            const DagTask& d = P.dag[t];
-            const int32_t J = d.front;
-            bool consumed = false, fdone = false;
-            if (d.kind == DT_EA && !d.c && ++eaucnt[J] == P.n_eau[J])
-                for (int32_t q = P.eau_ptr[J]; q < P.eau_ptr[J + 1]; ++q)
-                    if (--dep[P.eau_wait[q]] == 0) push(P.eau_wait[q]);
-            if (d.kind == DT_EA) consumed = ++eacnt[J] == P.dagf[J].n_ea;
-            else if (d.kind == DT_SMALL) consumed = fdone = true;
-            else if (d.kind == DT_SYRK) fdone = ++done[J] == P.dagf[J].n_done;
-            if (consumed)
-                for (int32_t q = P.uwaiters_ptr[J]; q < P.uwaiters_ptr[J + 1]; ++q) gate_dec(P.uwaiters[q]);
-            if (fdone && P.fronts[J].parent >= 0) gate_dec(P.fronts[J].parent);
+            for (int32_t t = 0; t < NT; ++t) sp2[t + 1] += sp2[t];
+            for (size_t k = 0; k < ge.size(); ++k) sc2[k] = ge[k].first;
+            const std::vector<double> cost0(cost);
+            for (int32_t t = 0; t < NT; ++t)
+                if (P.dag[t].kind == DT_POTRF) cost[t] += 175.0;
+            double ms = 0;
+            if (const char* e = simulate(sp2, sc2, d2, order_gmw, ms)) return std::string("gmw81: ") + e;
+            cost = cost0;
         }
-        P.sim_makespan_us = now;
-        std::vector<DagTask> dag2(NT);
-        for (int32_t i = 0; i < NT; ++i) dag2[i] = P.dag[order[i]];
-        P.dag_gmw = P.dag;  // creation order: each block column complete before its diagonal tile
+        std::vector<DagTask> dag2(NT), dag3(NT);
+        for (int32_t i = 0; i < NT; ++i) {
+            dag2[i] = P.dag[order[i]];
+            dag3[i] = P.dag[order_gmw[i]];
+        }
         P.dag.swap(dag2);
+        P.dag_gmw.swap(dag3);
     }
     return std::string();
 }
