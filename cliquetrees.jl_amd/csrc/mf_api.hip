@@ -335,7 +335,7 @@ void enqueue_factor(mf_factor* f) {
         const size_t nfr1 = H.fronts.size() + 1;
         launch_fill<int32_t>(f->eacnt, 2 * nfr1, 0, s);
         launch_fill<uint32_t>(f->ticket, 1, 0u, s);
-        // GMW81: creation order, where a diagonal tile follows every update of its block column
+        // GMW81: the ticket order where a diagonal tile follows every update of its block column
         DevDag D{f->reg_kind == MF_REG_GMW81 ? pl->d_dag_gmw : pl->d_dag, pl->d_dagf, pl->d_uwait_ptr, pl->d_uwait, f->cnt, f->eacnt, f->eacnt + nfr1,
                  f->prof, f->prof ? f->trace_buf : nullptr,
                  (int32_t)H.dag.size(), H.tune.prio, H.tune.wg_per_cu};

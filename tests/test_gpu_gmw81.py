@@ -155,3 +155,24 @@ def test_gmw81_needs_signs(ct, cases):
     info = ctypes.c_int64(0)
     rc = lib.mf_factorize(F._f.handle, ct.MF_LDLT, None, 2, -1.0, -1.0, ctypes.byref(info))
     assert rc == -1 and b"signs" in lib.mf_last_error()
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_gmw81_dropin_chol_impl(ct, oracle, cases, uplo):
+    """chol_impl! with GMW81 on host arrays (mf_chol_impl) in either layout,
+    Cholesky and LDLt with signs (permuted order, as chol_impl! receives them)."""
+    from densehelp import to_upper_layout
+    A, order, snd = cases["lap3d_16_nd"]
+    print(f"gmw81 drop-in uplo={uplo}", flush=True)
+    P, S = ct.symbolic(A, alg=order, snd=snd)
+    Dv, Lv = oracle.assemble(S, P, A)
+    for diag in (0, 1):
+        sp_ = np.ones(S.n) if diag else None
+        D0, L0, d0, _ = oracle.chol_impl(S, Dv, Lv, diag, signs=sp_, reg=("gmw81", -1.0, -1.0))
+        Din, Lin = (Dv, Lv) if uplo == "L" else to_upper_layout(S, Dv, Lv)
+        D1, L1, d1, info = ct.chol_impl(S, Din, Lin, diag=diag, signs=sp_, reg=ct.GMW81(), uplo=uplo)
+        if uplo == "U":
+            D0, L0 = to_upper_layout(S, D0, L0)
+        assert info == 0 and rel(D1, D0) <= RTOL64 and rel(L1, L0) <= RTOL64
+        if diag:
+            assert rel(d1, d0) <= RTOL64
