@@ -467,7 +467,7 @@ def run_single(args, ct, torch, name):
 
 def cpu_reference_from_profile(tag):
     """The whole-workload CPU reference measurement committed under profiles/
-    (``bench.py --workload cpu_reference`` on the GPU box's host), or None."""
+    (``bench.py --workload cpu_reference`` on the MI355X machine's host), or None."""
     path = os.path.join(ROOT, "profiles", f"cpu_reference_{tag}.md")
     if not os.path.exists(path):
         return None

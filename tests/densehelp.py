@@ -153,3 +153,26 @@ def to_upper_layout(S, Dval, Lval):
         D[p:p + k * k] = np.asarray(Dval[p:p + k * k]).reshape(k, k, order="F").T.ravel(order="F")
         L[q:q + a * k] = np.asarray(Lval[q:q + a * k]).reshape(a, k, order="F").T.ravel(order="F")
     return D, L
+
+
+def sparse_lower(S, Dval, Lval, unit=False):
+    """The factor stored in Dval/Lval as a scipy CSC lower-triangular matrix
+    (permuted order); ``unit`` puts ones on the diagonal."""
+    rows, cols, vals = [], [], []
+    for j, r0, k, a, srow in _blocks(S):
+        blk = np.asarray(Dval[S.Dptr[j] - 1: S.Dptr[j] - 1 + k * k]).reshape(k, k, order="F")
+        ii, jj = np.tril_indices(k)
+        v = blk[ii, jj].copy()
+        if unit:
+            v[ii == jj] = 1.0
+        rows.append(r0 + ii)
+        cols.append(r0 + jj)
+        vals.append(v)
+        if a:
+            Lb = np.asarray(Lval[S.Lptr[j] - 1: S.Lptr[j] - 1 + a * k]).reshape(a, k, order="F")
+            rr, cc = np.meshgrid(srow, np.arange(r0, r0 + k), indexing="ij")
+            rows.append(rr.ravel())
+            cols.append(cc.ravel())
+            vals.append(Lb.ravel())
+    return sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(S.n, S.n))

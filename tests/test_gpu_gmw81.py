@@ -176,3 +176,31 @@ def test_gmw81_dropin_chol_impl(ct, oracle, cases, uplo):
         assert info == 0 and rel(D1, D0) <= RTOL64 and rel(L1, L0) <= RTOL64
         if diag:
             assert rel(d1, d0) <= RTOL64
+
+
+def test_gmw81_perturbation_at_size(ct):
+    """32 x 32 x 30 7-point Laplacian shifted into indefiniteness, GMW81
+    Cholesky on the device: L L' - PAP' is diagonal (to rounding) with
+    nonnegative entries, and some pivots were raised (a property of the
+    method that needs no oracle, at a size with several levels of big
+    fronts)."""
+    import scipy.sparse as sp
+    from densehelp import sparse_lower
+    A = _shifted(ct.problems.laplacian3d_7pt(32, 32, 30, seed=9), 0.02)
+    print("gmw81 perturbation 32x32x30", flush=True)
+    P, S = ct.symbolic(A, alg=ct.grid_nd(32, 32, 30))
+    F = ct.ChordalCholesky(A, P, S)
+    ct.cholesky_(F, reg=ct.GMW81())
+    L = sparse_lower(S, F.Dval, F.Lval)
+    p = P.perm - 1
+    M = sp.csc_matrix(A)[p][:, p]
+    E = (L @ L.T - M).tocoo()
+    off = np.abs(E.data[E.row != E.col]).max()
+    # rounding of the products L L' (backward error of a Cholesky
+    # factorization: |E| <= c n eps |L| |L'|), entries of the regularized
+    # factor being larger than those of A
+    absL = abs(L)
+    scale = (absL @ absL.T).max()
+    d = E.diagonal()
+    assert off <= 64 * np.finfo(float).eps * scale
+    assert d.min() >= -64 * np.finfo(float).eps * scale and d.max() > 1e-3
