@@ -8,7 +8,7 @@ from .symbolic import ChordalSymbolic, Permutation, symbolic, flatindices, grid_
 from .factorization import (  # noqa: F401
     ChordalCholesky, ChordalLDLt, ChordalFactorization, DynamicRegularization, GMW81,
     PosDefException, SingularException, MFError, cholesky_, ldlt_, ldiv_, solve, rdiv_, rsolve, issuccess, refine,
-    chol_impl, div_impl, selinv_, selinv_impl, device_plan, DevicePlan, DeviceFactor, checkinfo, TUNING_NAMES,
+    chol_impl, div_impl, selinv_, selinv_impl, complete_, complete_impl, device_plan, DevicePlan, DeviceFactor, checkinfo, TUNING_NAMES,
     save_factor, load_factor, FACTOR_FORMAT,
     MF_FP64, MF_FP32, MF_CHOLESKY, MF_LDLT, MF_UPLO_L, MF_UPLO_U, STAT_NAMES, KC_NAMES,
 )

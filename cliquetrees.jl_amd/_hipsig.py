@@ -36,6 +36,7 @@ SIGNATURES = {
     "mf_selinv": ([vp], ci),
     "mf_selinv_get": ([vp, vp, vp, ci], ci),
     "mf_selinv_impl": ([vp, f64p, f64p, ci, i64p], ci),
+    "mf_complete_impl": ([vp, f64p, f64p, ci, i64p], ci),
     "mf_div_impl": ([vp, f64p, f64p, vp, f64p, i64, i64, ci, ci], ci),
     "mf_factor_set_uplo": ([vp, ci], ci),
     "mf_timing": ([vp, ci], ci),

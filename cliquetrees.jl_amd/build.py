@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 ARCH = os.environ.get("MF_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["mf_api.hip", "mf_factor.hip", "mf_solve.hip", "mf_refine.hip", "mf_selinv.hip",
-               "mf_plan.cpp", "mf_selinv_plan.cpp"]
+               "mf_complete.hip", "mf_plan.cpp", "mf_selinv_plan.cpp"]
 SYM_SOURCES = ["symbolic.cpp", "mmd.cpp", "mf_plan.cpp", "mf_selinv_plan.cpp"]
 
 

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -102,6 +103,8 @@ struct mf_plan {
     mf_factor* dropin = nullptr;   // FP64 factor object reused by mf_chol_impl / mf_div_impl
     // selected inversion: schedule built and uploaded on first use
     SelPlan* sel = nullptr;
+    struct CplState* cpl = nullptr;  // completion: its forest of front matrices (first use)
+    bool sel_x_only = false;         // a completion forest: selected inversion computes X only
     SelTask* d_sel = nullptr;
     int64_t* d_selx = nullptr;
     int64_t* d_selm = nullptr;
@@ -206,9 +209,47 @@ struct mf_factor {
     }
 };
 
+// Device memory handed out from one buffer, 256-byte aligned.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+};
+
+// The completion's forest (complete_impl!, mf_complete.hip): per reference
+// front j one root front holding its reordered front matrix K (nJ x nJ), in
+// the reference's order of the completion (root first), cut into chunks
+// whose K fit a memory budget.  A chunk keeps its plan, the original front
+// of each of its roots, and per root the panel offset of K (factored in
+// place) and the offset of X = chol(K)^-1 in the selected-inversion pool
+// (-1: small front, inverted by k_cpl_extract).
+struct CplChunk {
+    mf_plan* syn = nullptr;
+    int64_t t0 = 0, t1 = 0;       // root-first positions [t0, t1)
+    std::vector<int64_t> resptr;  // the chunk's column ranges (1-based), for info
+    int32_t* d_fr = nullptr;
+    int64_t* d_poff = nullptr;
+    int64_t* d_xoff = nullptr;    // after the chunk's first selected inversion
+    bool big = false;             // has fronts of the tile-DAG
+};
+
+struct CplState {
+    std::vector<CplChunk> chunks;
+    std::vector<void*> allocs;
+    Arena arena;  // one chunk's factor object and selected-inversion pool
+    int32_t* d_septgt = nullptr;
+    double* d_sig = nullptr;  // Sigma (ndz + nlz)
+    double* d_out = nullptr;  // the result (ndz + nlz)
+    ~CplState() {
+        for (CplChunk& c : chunks)
+            if (c.syn) (void)mf_plan_destroy(c.syn);
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
 mf_plan::~mf_plan() {
     delete dropin;
     delete sel;
+    delete cpl;
     for (void* p : allocs) (void)hipFree(p);
 }
 
@@ -590,7 +631,44 @@ int mf_plan_stats(const mf_plan* pl, int64_t* st, int64_t nst) {
     return MF_OK;
 }
 
-int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
+}  // extern "C"
+
+// The factor object's device buffers in allocation order; refs: the
+// reference-layout copies Dref/Lref (a factor carved from an arena, used
+// only inside the library, has none).
+static std::vector<std::pair<void**, size_t>> factor_buffers(mf_factor* f, const HostPlan& H, size_t es,
+                                                            bool refs) {
+    const size_t nf = H.fronts.size() + 1, n1 = (size_t)(H.n + 1);
+    std::vector<std::pair<void**, size_t>> v = {
+        {&f->panel, es * (size_t)H.panel_elems}, {&f->U, es * (size_t)H.u_elems},
+        {&f->W, es * (size_t)H.w_elems},         {&f->d, es * n1}};
+    if (refs) {
+        v.push_back({(void**)&f->Dref, 8 * (size_t)H.ndz});
+        v.push_back({(void**)&f->Lref, 8 * (size_t)H.nlz});
+    }
+    const std::pair<void**, size_t> rest[] = {
+        {(void**)&f->dref, 8 * n1},   {(void**)&f->adiag, 8 * n1},  {(void**)&f->signs, 8 * n1},
+        {(void**)&f->info, 64},       {(void**)&f->flags, 4 * nf},  {(void**)&f->prog, 4 * nf},
+        {(void**)&f->ticket, 64},     {(void**)&f->hang, 64},       {(void**)&f->gmw, 64},
+        {(void**)&f->cnt, 4 * (size_t)(H.dag_counters + 1)},        {(void**)&f->eacnt, 8 * nf},
+        {(void**)&f->perm_dev, 8 * n1}};
+    v.insert(v.end(), std::begin(rest), std::end(rest));
+    return v;
+}
+
+static size_t arena_round(size_t b) { return (std::max<size_t>(b, 16) + 255) & ~(size_t)255; }
+
+// Bytes of an arena factor of plan H (FP64).
+static size_t arena_factor_bytes(const HostPlan& H) {
+    mf_factor probe;
+    size_t b = 0;
+    for (auto& x : factor_buffers(&probe, H, 8, false)) b += arena_round(x.second);
+    return b;
+}
+
+// Creates a factor object; arena != NULL: its buffers are carved from the
+// arena (from arena->used on; nothing is freed with the object).
+static int factor_create(mf_plan* pl, int precision, Arena* arena, mf_factor** out) {
     if (!pl || !out || (precision != MF_FP64 && precision != MF_FP32))
         return fail(MF_EARG, "bad argument");
     *out = nullptr;
@@ -609,23 +687,20 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
     const HostPlan& H = pl->H;
     size_t es = precision == MF_FP32 ? 4 : 8;
     auto alloc = [&](void** p, size_t bytes) -> bool {
+        if (arena) {
+            const size_t b = arena_round(bytes);
+            if (arena->used + b > arena->size) return false;
+            *p = arena->base + arena->used;
+            arena->used += b;
+            return true;
+        }
         if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
         f->allocs.push_back(*p);
         f->alloc_bytes += std::max<size_t>(bytes, 16);
         return true;
     };
-    bool ok = alloc(&f->panel, es * (size_t)H.panel_elems) && alloc(&f->U, es * (size_t)H.u_elems) &&
-              alloc(&f->W, es * (size_t)H.w_elems) && alloc(&f->d, es * (size_t)(H.n + 1)) &&
-              alloc((void**)&f->Dref, 8 * (size_t)H.ndz) && alloc((void**)&f->Lref, 8 * (size_t)H.nlz) &&
-              alloc((void**)&f->dref, 8 * (size_t)(H.n + 1)) &&
-              alloc((void**)&f->adiag, 8 * (size_t)(H.n + 1)) &&
-              alloc((void**)&f->signs, 8 * (size_t)(H.n + 1)) &&
-              alloc((void**)&f->info, 64) && alloc((void**)&f->flags, 4 * (H.fronts.size() + 1)) &&
-              alloc((void**)&f->prog, 4 * (H.fronts.size() + 1)) &&
-              alloc((void**)&f->ticket, 64) && alloc((void**)&f->hang, 64) && alloc((void**)&f->gmw, 64) &&
-              alloc((void**)&f->cnt, 4 * (size_t)(H.dag_counters + 1)) &&
-              alloc((void**)&f->eacnt, 8 * (H.fronts.size() + 1)) &&
-              alloc((void**)&f->perm_dev, 8 * (size_t)(H.n + 1));
+    bool ok = true;
+    for (auto& x : factor_buffers(f, H, es, arena == nullptr)) ok = ok && alloc(x.first, x.second);
     if (!ok) {
         delete f;
         return fail(MF_ENOMEM, "hipMalloc failed (factor)");
@@ -647,6 +722,12 @@ int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
     }
     *out = f;
     return MF_OK;
+}
+
+extern "C" {
+
+int mf_factor_create(mf_plan* pl, int precision, mf_factor** out) {
+    return factor_create(pl, precision, nullptr, out);
 }
 
 int mf_factor_destroy(mf_factor* f) {
@@ -1383,7 +1464,7 @@ namespace {
 int ensure_sel_plan(mf_plan* pl) {
     if (pl->sel) return MF_OK;
     SelPlan* S = new SelPlan();
-    std::string err = build_selinv(pl->H, *S);
+    std::string err = build_selinv(pl->H, *S, pl->sel_x_only);
     if (!err.empty()) {
         delete S;
         return fail(MF_EARG, "selected-inversion schedule: " + err);
@@ -1481,6 +1562,218 @@ int mf_selinv_get(mf_factor* f, double* Dval, double* Lval, int device) {
     if (Lval && H.nlz) HIPCHK(hipMemcpyAsync(Lval, f->Lref, 8 * H.nlz, k, f->stream));
     HIPCHK(hipStreamSynchronize(f->stream));
     return read_info(f, nullptr);
+}
+
+// Elements of K per chunk of the completion's forest (a chunk's factor
+// object and selected-inversion pool hold about four times that many
+// doubles: the panels, X, the recursion's scratch and Sigma);
+// MF_COMPLETE_CHUNK_ELEMS overrides.
+static int64_t cpl_budget() {
+    if (const char* e = std::getenv("MF_COMPLETE_CHUNK_ELEMS")) {
+        const long long v = std::atoll(e);
+        if (v > 0) return (int64_t)v;
+    }
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)1 << 33;
+    return std::max<int64_t>(std::min<int64_t>((int64_t)(fr / 64), (int64_t)1 << 29), (int64_t)1 << 16);
+}
+
+// Builds the completion's forest of a plan (on first use).
+static int ensure_cpl(mf_plan* pl) {
+    if (pl->cpl) return MF_OK;
+    const HostPlan& H = pl->H;
+    const int64_t nfr = H.nfr;
+    CplState* C = new CplState();
+    auto nj_of = [&](int64_t j) { return (H.resptr[j + 1] - H.resptr[j]) + (H.sepptr[j + 1] - H.sepptr[j]); };
+    const int64_t budget = cpl_budget();
+    int rc = MF_OK;
+    for (int64_t t0 = 0; t0 < nfr && !rc;) {
+        int64_t t1 = t0, elems = 0;
+        while (t1 < nfr && (t1 == t0 || elems + nj_of(nfr - 1 - t1) * nj_of(nfr - 1 - t1) <= budget)) {
+            elems += nj_of(nfr - 1 - t1) * nj_of(nfr - 1 - t1);
+            ++t1;
+        }
+        const int64_t m = t1 - t0;
+        CplChunk c;
+        c.t0 = t0;
+        c.t1 = t1;
+        std::vector<int64_t> resptr(m + 1, 1), ones(m + 2, 1), Dptr(m + 1, 1), dummy(1, 1);
+        std::vector<int32_t> fr(m);
+        for (int64_t u = 0; u < m; ++u) {
+            const int64_t j = nfr - 1 - (t0 + u), nj = nj_of(j);
+            fr[u] = (int32_t)j;
+            resptr[u + 1] = resptr[u] + nj;
+            Dptr[u + 1] = Dptr[u] + nj * nj;
+        }
+        rc = mf_plan_create(resptr[m] - 1, m, resptr.data(), ones.data(), dummy.data(), dummy.data(), ones.data(),
+                            dummy.data(), Dptr.data(), ones.data(), 0, &c.syn);
+        if (rc) break;
+        const HostPlan& SH = c.syn->H;
+        std::vector<int64_t> poff(m);
+        for (int64_t u = 0; u < m; ++u) {
+            const int32_t J = SH.owner[u];
+            const Front& F = SH.fronts[J];
+            // a root of the forest is its own internal front: nn = nJ, na = 0
+            if (F.nn != resptr[u + 1] - resptr[u] || F.na != 0 || F.c0 != resptr[u] - 1) {
+                rc = fail(MF_EARG, "completion forest: a root is not one internal front");
+                break;
+            }
+            poff[u] = F.poff;
+            c.big = c.big || !SH.is_small[J];
+        }
+        if (!rc && ((rc = upload(fr, &c.d_fr, C->allocs)) || (rc = upload(poff, &c.d_poff, C->allocs)))) break;
+        if (rc) break;
+        // the arena holds the largest chunk's factor and selected-inversion pool
+        size_t need = arena_factor_bytes(SH);
+        if (c.big) {
+            c.syn->sel_x_only = true;
+            if ((rc = ensure_sel_plan(c.syn))) break;
+            const SelPlan& SP = *c.syn->sel;
+            need += arena_round(8 * (size_t)(2 * SP.x_elems + SP.lt_elems + SP.m_elems + SH.panel_elems));
+        }
+        C->arena.size = std::max(C->arena.size, need);
+        c.resptr = std::move(resptr);
+        C->chunks.push_back(std::move(c));
+        t0 = t1;
+    }
+    if (!rc) {
+        std::vector<int32_t> septgt(H.septgt.begin(), H.septgt.end());
+        rc = upload(septgt, &C->d_septgt, C->allocs);
+    }
+    auto dalloc = [&](double** p, size_t n) {
+        if (hipMalloc((void**)p, 8 * std::max<size_t>(n, 1)) != hipSuccess) return false;
+        C->allocs.push_back(*p);
+        return true;
+    };
+    if (!rc && (!dalloc(&C->d_sig, (size_t)(H.ndz + H.nlz)) || !dalloc(&C->d_out, (size_t)(H.ndz + H.nlz)) ||
+                !dalloc((double**)&C->arena.base, C->arena.size / 8 + 1)))
+        rc = fail(MF_ENOMEM, "hipMalloc failed (completion)");
+    if (rc) {
+        delete C;
+        return rc;
+    }
+    pl->cpl = C;
+    return MF_OK;
+}
+
+// Dval/Lval blocks transposed per front, between the :U and :L layouts
+// (either direction: to_upper = false reads :U into :L, true writes :L as :U).
+static void cpl_transpose(const HostPlan& H, const double* Din, const double* Lin, double* Dout, double* Lout,
+                          bool to_upper) {
+    for (int64_t j = 0; j < H.nfr; ++j) {
+        const int64_t nn = H.resptr[j + 1] - H.resptr[j], na = H.sepptr[j + 1] - H.sepptr[j];
+        const double* Di = Din + H.Dptr[j];
+        double* Do = Dout + H.Dptr[j];
+        for (int64_t c = 0; c < nn; ++c)
+            for (int64_t r = 0; r < nn; ++r) Do[r + c * nn] = Di[c + r * nn];
+        const double* Li = Lin + H.Lptr[j];
+        double* Lo = Lout + H.Lptr[j];
+        for (int64_t c = 0; c < nn; ++c)
+            for (int64_t r = 0; r < na; ++r) {
+                if (to_upper) Lo[c + r * nn] = Li[r + c * na];  // nn x na <- na x nn
+                else Lo[r + c * na] = Li[c + r * nn];           // na x nn <- nn x na
+            }
+    }
+}
+
+// One chunk of the completion: K into the chunk factor's panel, Cholesky in
+// place, X of the big roots by the selected-inversion schedule, the outputs
+// of the chunk's fronts into d_out.  *inf: the forest's failing column
+// (1-based in the chunk), 0 if none.
+static int cpl_chunk(mf_plan* plan, CplChunk& c, hipStream_t caller, int64_t* inf) {
+    CplState& C = *plan->cpl;
+    const HostPlan& H = plan->H;
+    mf_factor* sf = nullptr;
+    C.arena.used = 0;
+    int rc = factor_create(c.syn, MF_FP64, &C.arena, &sf);
+    if (rc) return rc;
+    if (c.big) {
+        sf->sel_pool = C.arena.base + C.arena.used;  // returned to the arena below
+        sf->sel_bytes = C.arena.size - C.arena.used;
+    }
+    const int m = (int)(c.t1 - c.t0);
+    // Sigma was copied on the caller's stream
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, caller));
+    HIPCHK(hipStreamWaitEvent(sf->stream, ev, 0));
+    HIPCHK(hipEventDestroy(ev));
+    launch_cpl_gather(plan->D, m, c.d_fr, C.d_sig, C.d_sig + H.ndz, C.d_septgt, c.d_poff, (double*)sf->panel,
+                      sf->stream);
+    HIPCHK(hipGetLastError());
+    sf->factored = false;
+    sf->sel_valid = false;
+    sf->sh_valid = false;
+    sf->ref_stale = false;
+    rc = mf_factorize(sf, MF_CHOLESKY, nullptr, MF_REG_NONE, 0.0, 0.0, inf);
+    if (!rc && !*inf) {
+        if (c.big) rc = mf_selinv(sf);
+        if (!rc && c.big && !c.d_xoff) {
+            const HostPlan& SH = c.syn->H;
+            std::vector<int64_t> xoff(m, -1);
+            for (int u = 0; u < m; ++u) {
+                const int32_t J = SH.owner[u];
+                if (!SH.is_small[J]) xoff[u] = c.syn->sel->xoff[J];
+            }
+            rc = upload(xoff, &c.d_xoff, C.allocs);
+        }
+        if (!rc) {
+            launch_cpl_extract(plan->D, m, c.d_fr, (const double*)sf->sel_pool, c.d_xoff, (const double*)sf->panel,
+                               c.d_poff, C.d_out, C.d_out + H.ndz, sf->stream);
+            if (hipGetLastError() != hipSuccess) rc = fail(MF_EHIP, "completion: launch failed");
+        }
+    }
+    if (hipStreamSynchronize(sf->stream) != hipSuccess && !rc) rc = fail(MF_EHIP, "completion: stream failed");
+    sf->sel_pool = nullptr;
+    (void)mf_factor_destroy(sf);
+    return rc;
+}
+
+int mf_complete_impl(mf_plan* plan, double* Dval, double* Lval, int uplo, int64_t* info) {
+    if (!plan || (!Dval && plan->H.ndz) || (!Lval && plan->H.nlz)) return fail(MF_EARG, "null argument");
+    if (uplo != MF_UPLO_L && uplo != MF_UPLO_U) return fail(MF_EARG, "bad uplo");
+    const HostPlan& H = plan->H;
+    int rc = ensure_cpl(plan);
+    if (rc) return rc;
+    CplState& C = *plan->cpl;
+    const hipStream_t s = nullptr;  // the null stream: transfers of Sigma and the result
+    std::vector<double> tD, tL;
+    const double *Din = Dval, *Lin = Lval;
+    if (uplo == MF_UPLO_U) {
+        tD.resize(H.ndz);
+        tL.resize(H.nlz);
+        cpl_transpose(H, Dval, Lval, tD.data(), tL.data(), false);
+        Din = tD.data();
+        Lin = tL.data();
+    }
+    if (H.ndz) HIPCHK(hipMemcpyAsync(C.d_sig, Din, 8 * H.ndz, hipMemcpyHostToDevice, s));
+    if (H.nlz) HIPCHK(hipMemcpyAsync(C.d_sig + H.ndz, Lin, 8 * H.nlz, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (CplChunk& c : C.chunks) {
+        int64_t inf = 0;
+        if ((rc = cpl_chunk(plan, c, s, &inf))) return rc;
+        if (inf) {
+            // the first failing column of the chunk (its roots in the
+            // reference's order): the local column of chol(Sigma(sep, sep)),
+            // or of the reordered Schur complement
+            const int64_t col = inf - 1;
+            const int64_t u = std::upper_bound(c.resptr.begin(), c.resptr.end(), col + 1) - c.resptr.begin() - 1;
+            const int64_t j = H.nfr - 1 - (c.t0 + u), na = H.sepptr[j + 1] - H.sepptr[j];
+            const int64_t k = col - (c.resptr[u] - 1);
+            if (info) *info = k < na ? k + 1 : k - na + 1;
+            return MF_OK;
+        }
+    }
+    if (uplo == MF_UPLO_U) {
+        if (H.ndz) HIPCHK(hipMemcpy(tD.data(), C.d_out, 8 * H.ndz, hipMemcpyDeviceToHost));
+        if (H.nlz) HIPCHK(hipMemcpy(tL.data(), C.d_out + H.ndz, 8 * H.nlz, hipMemcpyDeviceToHost));
+        cpl_transpose(H, tD.data(), tL.data(), Dval, Lval, true);
+    } else {
+        if (H.ndz) HIPCHK(hipMemcpy(Dval, C.d_out, 8 * H.ndz, hipMemcpyDeviceToHost));
+        if (H.nlz) HIPCHK(hipMemcpy(Lval, C.d_out + H.ndz, 8 * H.nlz, hipMemcpyDeviceToHost));
+    }
+    if (info) *info = 0;
+    return MF_OK;
 }
 
 int mf_selinv_impl(mf_plan* plan, double* Dval, double* Lval, int uplo, int64_t* info) {

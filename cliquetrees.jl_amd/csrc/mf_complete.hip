@@ -1,0 +1,140 @@
+// mf_complete.hip — gfx950 kernels of the completion (complete_impl!,
+// src/Multifrontal.jl/src/complete.jl:46-451): Dval/Lval hold a matrix Sigma
+// on the factor's pattern; the result is the Cholesky factor of the inverse
+// of Sigma's maximum-determinant positive definite completion.
+//
+// Per front j (residual nn, separator na, nJ = na + nn) the reference forms
+// M = Sigma(sep, sep), C = chol(M), Y = C^-1 Sigma21, the Schur complement
+// S = Sigma11 - Y'Y = R'R (R lower, by reversing the order: the lower
+// Cholesky of J S J is J R' J), and returns L21 = -C^-T Y R^-1, D11 = R^-1.
+// All of it is one Cholesky of the reordered front matrix
+//     K = [ M        Sigma21 J ]
+//         [ J Sigma21'  J Sigma11 J ]
+// (J reverses the nn residual columns): chol(K) = [C 0; J Y' Ltil] with
+// Ltil = chol(J S J), and with X = chol(K)^-1 the outputs are its last nn
+// rows: D11(r, c) = X(na + nn-1-c, na + nn-1-r), L21(r, c) = X(na + nn-1-c, r).
+// The fronts are independent given Sigma, so the library builds forests of
+// these K (plans whose fronts are all roots, in chunks that fit a memory
+// budget), factors them in place (the big ones on the tile-DAG), takes X of
+// the big fronts from the selected-inversion schedule (k_selinv's L11^-1)
+// and inverts the small ones here.  k_cpl_gather writes K into the forest's
+// panels, k_cpl_extract the outputs; both run over one chunk, whose root u
+// is reference front fr[u].
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mf_kernels.h"
+
+namespace mf {
+
+namespace {
+
+// Sigma(a, b), a >= b global (permuted) columns, from the reference layout:
+// column b's front holds it in its diagonal block or its separator rows.
+__device__ __forceinline__ double sigma_at(const DevPlan& P, const double* Dval, const double* Lval,
+                                           const int32_t* septgt, int a, int b) {
+    const OutFront o = P.out[P.col2ref[b]];
+    const int lb = b - o.r0;
+    if (a < o.r0 + o.nn) return Dval[o.Dp + (a - o.r0) + (int64_t)lb * o.nn];
+    // a among the front's separator rows (sorted): binary search
+    const int32_t* s = septgt + o.sepoff;
+    int lo = 0, hi = o.na;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s[mid] < a) lo = mid + 1;
+        else hi = mid;
+    }
+    return Lval[o.Lp + lo + (int64_t)lb * o.na];
+}
+
+// Workgroups (u, y) for root u (reference front fr[u]): columns c = y,
+// y + gridDim.y, ... of its K (nJ x nJ column-major, both triangles, at
+// K + koff[u]).
+__global__ __launch_bounds__(256) void k_cpl_gather(DevPlan P, const int32_t* fr, const double* Dval,
+                                                    const double* Lval, const int32_t* septgt,
+                                                    const int64_t* koff, double* K) {
+    const int u = blockIdx.x;
+    const OutFront o = P.out[fr[u]];
+    const int nn = o.nn, na = o.na, nj = nn + na;
+    double* Kj = K + koff[u];
+    const double* D11 = Dval + o.Dp;
+    const double* L21 = Lval + o.Lp;
+    const int32_t* sep = septgt + o.sepoff;
+    for (int c = blockIdx.y; c < nj; c += gridDim.y)
+        for (int r = c + threadIdx.x; r < nj; r += blockDim.x) {
+            double v;
+            if (r < na) v = sigma_at(P, Dval, Lval, septgt, sep[r], sep[c]);          // M
+            else if (c < na) v = L21[c + (int64_t)(nn - 1 - (r - na)) * na];           // J Sigma21'
+            else v = D11[(nn - 1 - (c - na)) + (int64_t)(nn - 1 - (r - na)) * nn];     // J Sigma11 J
+            Kj[r + (int64_t)c * nj] = v;
+            Kj[c + (int64_t)r * nj] = v;  // both triangles: K is read as a full block
+        }
+}
+
+// One workgroup per root u (reference front fr[u]): D11 and L21 of the
+// completion from the last nn rows of X = chol(K)^-1.  xoff (NULL: none) with
+// xoff[u] >= 0: X of a big front, lower triangle, ld nJ, at X + xoff[u];
+// else the factor of K (a small front, nJ <= 64) at L + loff[u] (ld nJ),
+// inverted here in LDS.
+__global__ __launch_bounds__(256) void k_cpl_extract(DevPlan P, const int32_t* fr, const double* X,
+                                                     const int64_t* xoff, const double* L,
+                                                     const int64_t* loff, double* Dval, double* Lval) {
+    __shared__ double Ls[64 * 65], Xs[64 * 65];
+    const int u = blockIdx.x, tid = threadIdx.x;
+    const OutFront o = P.out[fr[u]];
+    const int nn = o.nn, na = o.na, nj = nn + na;
+    const double* Xj;
+    int64_t ldx;
+    if (xoff && xoff[u] >= 0) {
+        Xj = X + xoff[u];
+        ldx = nj;
+    } else {
+        // small front: X = L^-1 by forward substitution, column c by thread c
+        const double* Lj = L + loff[u];
+        for (int e = tid; e < nj * nj; e += blockDim.x) {
+            const int c = e / nj, r = e - c * nj;
+            Ls[r + c * 65] = r >= c ? Lj[r + (int64_t)c * nj] : 0.0;
+        }
+        __syncthreads();
+        if (tid < nj) {
+            const int c = tid;
+            for (int r = 0; r < c; ++r) Xs[r + c * 65] = 0.0;
+            Xs[c + c * 65] = 1.0 / Ls[c + c * 65];
+            for (int r = c + 1; r < nj; ++r) {
+                double s = 0.0;
+                for (int k = c; k < r; ++k) s += Ls[r + k * 65] * Xs[k + c * 65];
+                Xs[r + c * 65] = -s / Ls[r + r * 65];
+            }
+        }
+        __syncthreads();
+        Xj = Xs;
+        ldx = 65;
+    }
+    double* D11 = Dval + o.Dp;
+    double* L21 = Lval + o.Lp;
+    for (int e = tid; e < nn * nn; e += blockDim.x) {
+        const int c = e / nn, r = e - c * nn;
+        D11[e] = r >= c ? Xj[(na + nn - 1 - c) + (int64_t)(na + nn - 1 - r) * ldx] : 0.0;
+    }
+    for (int e = tid; e < na * nn; e += blockDim.x) {
+        const int c = e / na, r = e - c * na;
+        L21[e] = Xj[(na + nn - 1 - c) + (int64_t)r * ldx];
+    }
+}
+
+}  // namespace
+
+void launch_cpl_gather(const DevPlan& P, int m, const int32_t* fr, const double* Dval, const double* Lval,
+                       const int32_t* septgt, const int64_t* koff, double* K, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_cpl_gather, dim3(m, 16), dim3(256), 0, s, P, fr, Dval, Lval, septgt, koff, K);
+}
+
+void launch_cpl_extract(const DevPlan& P, int m, const int32_t* fr, const double* X, const int64_t* xoff,
+                        const double* L, const int64_t* loff, double* Dval, double* Lval, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_cpl_extract, dim3(m), dim3(256), 0, s, P, fr, X, xoff, L, loff, Dval, Lval);
+}
+
+}  // namespace mf

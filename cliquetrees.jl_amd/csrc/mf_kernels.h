@@ -150,6 +150,14 @@ struct DevSel {
 template <typename T>
 void launch_selinv(const DevPlan& P, const DevSel<T>& S, hipStream_t s);
 
+// ---- completion (mf_complete.hip), FP64 ----
+// K of every reference front (the reordered front matrix whose Cholesky
+// factor carries the completion) from Sigma in the reference layout.
+void launch_cpl_gather(const DevPlan& P, int m, const int32_t* fr, const double* Dval, const double* Lval,
+                       const int32_t* septgt, const int64_t* koff, double* K, hipStream_t s);
+void launch_cpl_extract(const DevPlan& P, int m, const int32_t* fr, const double* X, const int64_t* xoff,
+                        const double* L, const int64_t* loff, double* Dval, double* Lval, hipStream_t s);
+
 // ---- iterative refinement (mf_refine.hip), FP64 ----
 void launch_residual(int64_t n, const int64_t* colptr, const int64_t* rowval, const double* val,
                      const double* X, int64_t ldx, const double* B, int64_t ldb, double* R,

@@ -119,7 +119,9 @@ struct SelPlan {
 };
 
 // Builds the selected-inversion schedule of a plan (Cholesky factors).
-std::string build_selinv(const HostPlan& H, SelPlan& S);
+// x_only: only X = L11^-1 of the big fronts (no Sigma), for a forest whose
+// fronts are all roots (the completion's).
+std::string build_selinv(const HostPlan& H, SelPlan& S, bool x_only = false);
 
 // Builds the plan; returns an empty string on success, else an error message.
 std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* resptr,

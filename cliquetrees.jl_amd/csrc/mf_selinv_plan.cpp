@@ -86,7 +86,7 @@ SelTask plain_task(int kind, int J, int a) {
 
 }  // namespace
 
-std::string build_selinv(const HostPlan& H, SelPlan& S) {
+std::string build_selinv(const HostPlan& H, SelPlan& S, bool x_only) {
     const int32_t NJ = (int32_t)H.fronts.size();
     S = SelPlan();
     S.total.assign(NJ, 0);
@@ -130,9 +130,11 @@ std::string build_selinv(const HostPlan& H, SelPlan& S) {
             done += (int32_t)(v.size() - first);
         };
         if (H.is_small[J]) {
-            SelTask t = plain_task(ST_SMALL, J, 0);
-            B[J].push_back(t);
-            close_phase(B[J], 0);
+            if (!x_only) {
+                SelTask t = plain_task(ST_SMALL, J, 0);
+                B[J].push_back(t);
+                close_phase(B[J], 0);
+            }
         } else {
             const int nk = ntiles(nn), nr = ntiles(na);
             const int64_t po = F.poff, xo = S.xoff[J], lo = S.ltoff[J], mo = S.moff[J];
@@ -215,7 +217,7 @@ std::string build_selinv(const HostPlan& H, SelPlan& S) {
             }
             // Sigma11(i, j) = sum_{k >= 64 i} X(k, i) X(k, j) - sum_k Lt(k, i) Sigma21(k, j), i >= j
             f0 = B[J].size();
-            for (int i = 0; i < nk; ++i)
+            for (int i = 0; i < nk && !x_only; ++i)
                 for (int j = 0; j <= i; ++j) {
                     SelTask t = gemm_task(J, SP_SIG, po + (int64_t)NB * i + (int64_t)NB * j * nJ, nJ,
                                           tile_len(nn, i), tile_len(nn, j), i == j);

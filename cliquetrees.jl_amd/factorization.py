@@ -395,6 +395,43 @@ def selinv_impl(S: ChordalSymbolic, Dval, Lval, uplo="L"):
     return Dval, Lval
 
 
+def complete_impl(S: ChordalSymbolic, Dval, Lval, uplo="L"):
+    """complete_impl! on host arrays through mf_complete_impl
+    (complete.jl:46-72): Dval/Lval hold a matrix Sigma on the factor's
+    pattern in the reference layout (uplo); returns (Dval, Lval, info) with
+    the Cholesky factor of the inverse of Sigma's maximum-determinant
+    completion.  info > 0: the local column of the first failing potrf! in
+    the reference's order of fronts; the arrays are then returned as given."""
+    Dval = np.array(Dval, dtype=np.float64, copy=True)
+    Lval = np.array(Lval, dtype=np.float64, copy=True)
+    if Dval.shape != (S.ndz,) or Lval.shape != (S.nlz,):
+        raise ValueError("Dval/Lval do not match the symbolic factorization")
+    info = ctypes.c_int64(0)
+    plan = device_plan(S)
+    _check(_native.hip().mf_complete_impl(plan.handle, Dval.ctypes.data_as(_native.f64p),
+                                          Lval.ctypes.data_as(_native.f64p), _UPLO[uplo],
+                                          ctypes.byref(info)))
+    return Dval, Lval, int(info.value)
+
+
+def complete_(F: ChordalFactorization, check=True):
+    """complete!(F; check) (complete.jl:1-23): F holds a matrix on its
+    pattern (after selinv_, its selected inverse); afterwards F is the
+    Cholesky factor of the inverse of that matrix's maximum-determinant
+    completion, ready for ldiv_/solve (mf_factor_load), and F.info is set;
+    check raises PosDefException on a failed potrf! (checkinfo)."""
+    if F.DIAG != MF_CHOLESKY:
+        raise TypeError("complete! is defined for ChordalCholesky factorizations")
+    D, L, info = complete_impl(F.S, F.Dval, F.Lval, F.uplo)
+    F.info = info
+    if check:
+        checkinfo(info, MF_CHOLESKY)
+    if info == 0:
+        _check(_native.hip().mf_factor_load(F._f.handle, _ptr(D), _ptr(L), None, MF_CHOLESKY, 0))
+        F._selinv = False
+    return F
+
+
 def issuccess(F):
     return F.info == 0
 

@@ -285,6 +285,23 @@ int mf_selinv(mf_factor* f);
 int mf_selinv_get(mf_factor* f, double* Dval, double* Lval, int device);
 int mf_selinv_impl(mf_plan* plan, double* Dval, double* Lval, int uplo, int64_t* info);
 
+/* Completion: Dval/Lval hold a matrix Sigma on the factor's pattern in the
+ * reference layout (uplo; e.g. a selected inverse); they are overwritten
+ * with the Cholesky factor of the inverse of Sigma's maximum-determinant
+ * positive definite completion.  Replaces complete_impl!(Mptr, Mval, Dptr,
+ * Dval, Lptr, Lval, Fval, res, rel, chd, uplo)
+ * (src/Multifrontal.jl/src/complete.jl:46-72), which complete!(F)
+ * (complete.jl:1-23) calls.  The fronts are independent given Sigma: each
+ * front's reordered matrix [Sigma(sep, sep), Sigma21 J; J Sigma21', J
+ * Sigma11 J] (J reverses the residual columns) is factored on the device as
+ * one root of a forest, the big ones on the tile-DAG; the result is read
+ * from the inverse of that factor.  info > 0: the first front in the
+ * reference's order (root first) whose potrf! fails, with the reference's
+ * local column (of Sigma(sep, sep), else of the reordered Schur complement);
+ * Dval/Lval are then left unchanged.  Work buffers are kept in the plan
+ * after the first call; calls on one plan must not overlap. */
+int mf_complete_impl(mf_plan* plan, double* Dval, double* Lval, int uplo, int64_t* info);
+
 /* Kernel timing (HIP events around each launch, on the factor's stream).
  * times[k] = total milliseconds, counts[k] = launches, flops[k] = algorithmic
  * work of kernel class k since the last reset (classes: MF_KC_*). */
