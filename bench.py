@@ -39,6 +39,12 @@ Other workloads (--workload):
   kkt_serving    Poisson stream of KKT LDLt requests, patterns reused over 20
                  value updates, continuous batching; p50/p99 latency, solves/s
                  against one-at-a-time serving
+  lap7_64_selinv selected inversion (selinv!) of the lap7_64 FP64 factor
+  lap7_64_complete
+                 completion (complete!) of that selected inverse through the
+                 drop-in mf_complete_impl (host arrays, PCIe inside the call);
+                 not among the default run's workloads (its first call builds
+                 the completion's forest plans, about 20 s)
   copy           a 1 GiB device copy (calibration of the HBM traffic counters)
   cpu_reference  the CPU reference path (oracle + host OpenBLAS) on the whole
                  lap7_64 factorization and 16-RHS solve, median / p99 of
@@ -685,6 +691,55 @@ def run_selinv(args, ct, torch, name="lap7_64"):
             "root_column_vs_solve": err}
 
 
+def complete_ref_flops(S):
+    """Flops of complete_impl! on the reference's own fronts
+    (complete.jl:140-260): potrf of Sigma(sep, sep) na^3 / 3, two trsm with
+    it 2 na^2 nn, syrk na nn^2 / 2, potrf of the Schur complement nn^3 / 3,
+    trsm na nn^2, trtri nn^3 / 3 multiply-adds, two flops each."""
+    nn = S.nn().astype(np.float64)
+    na = S.na().astype(np.float64)
+    return float(2 * (na ** 3 / 3 + 2 * na ** 2 * nn + 1.5 * na * nn ** 2 + 2.0 / 3.0 * nn ** 3).sum())
+
+
+def run_complete(args, ct, torch, name="lap7_64"):
+    """Completion (complete!) of the selected inverse of the workload's FP64
+    Cholesky factor through the drop-in mf_complete_impl: host arrays in and
+    out (the PCIe transfers are inside the timed call), the first call
+    (forest plans, X schedules and the device arena) untimed.  Checked at
+    size: the result is the factor again."""
+    W = workload(ct, name)
+    A = W["A"]
+    P, S = ct.symbolic(A, alg=W["order"])
+    F = ct.ChordalCholesky(A, P, S)
+    ct.cholesky_(F)
+    D0, L0 = F.Dval, F.Lval
+    ct.selinv_(F)
+    SD, SL = F.Dval, F.Lval
+    t0 = time.perf_counter()
+    ct.complete_impl(S, SD, SL)
+    first = time.perf_counter() - t0
+    log(f"complete {name}: first call {first:.2f} s")
+    for _ in range(max(0, args.warmup - 1)):
+        ct.complete_impl(S, SD, SL)
+    ts = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        CD, CL, info = ct.complete_impl(S, SD, SL)
+        ts.append((time.perf_counter() - t0) * 1e3)
+        log(f"complete {name}: {ts[-1]:.1f} ms")
+    err = max(float(np.abs(CD - D0).max() / np.abs(D0).max()), float(np.abs(CL - L0).max() / np.abs(L0).max()))
+    ms = float(np.median(ts))
+    flops = complete_ref_flops(S)
+    return {"metric": "completion TFLOP/s (reference complete_impl! flops, host arrays in and out)",
+            "value": round(flops / (ms * 1e-3) / 1e12, 3), "unit": "TFLOP/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 1),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": W["desc"].split(", 16 RHS")[0] + ", complete! of its selected inverse",
+                       "ref_flops": flops, "first_call_s": round(first, 2)},
+            "info": int(info), "max_rel_diff_vs_factor": err}
+
+
 def run_copy(args, ct, torch):
     """A plain device copy of 1 GiB (read 1 GiB, write 1 GiB): the reference
     kernel whose FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic
@@ -736,7 +791,8 @@ def main():
         out = run_serving(args, ct, torch)
     elif args.workload == "lap7_64_selinv":
         out = run_selinv(args, ct, torch, "lap7_64")
-
+    elif args.workload == "lap7_64_complete":
+        out = run_complete(args, ct, torch, "lap7_64")
     else:
         out = run_single(args, ct, torch, args.workload)
         if args.workload == "lap7_64" and not args.headline_only:
