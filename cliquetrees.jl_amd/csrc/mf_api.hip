@@ -104,7 +104,6 @@ struct mf_plan {
     // selected inversion: schedule built and uploaded on first use
     SelPlan* sel = nullptr;
     struct CplState* cpl = nullptr;  // completion: its forest of front matrices (first use)
-    bool sel_x_only = false;         // a completion forest: selected inversion computes X only
     SelTask* d_sel = nullptr;
     int64_t* d_selx = nullptr;
     int64_t* d_selm = nullptr;
@@ -216,26 +215,32 @@ struct Arena {
 };
 
 // The completion's forest (complete_impl!, mf_complete.hip): per reference
-// front j one root front holding its reordered front matrix K (nJ x nJ), in
+// front j one root front holding its reordered front matrix K (nJ' x nJ',
+// the separator part of a big front padded to a multiple of 64 rows), in
 // the reference's order of the completion (root first), cut into chunks
 // whose K fit a memory budget.  A chunk keeps its plan, the original front
-// of each of its roots, and per root the panel offset of K (factored in
-// place) and the offset of X = chol(K)^-1 in the selected-inversion pool
-// (-1: small front, inverted by k_cpl_extract).
+// of each of its roots, per root the panel offset of K (factored in place)
+// and the offset of its backward solves Y (-1: small front, inverted by
+// k_cpl_extract), and the schedule of those solves.
 struct CplChunk {
     mf_plan* syn = nullptr;
     int64_t t0 = 0, t1 = 0;       // root-first positions [t0, t1)
     std::vector<int64_t> resptr;  // the chunk's column ranges (1-based), for info
+    std::vector<int64_t> sep;     // per root: na and the padded na'
+    std::vector<int64_t> sepp;
     int32_t* d_fr = nullptr;
     int64_t* d_poff = nullptr;
-    int64_t* d_xoff = nullptr;    // after the chunk's first selected inversion
-    bool big = false;             // has fronts of the tile-DAG
+    int64_t* d_xoff = nullptr;
+    SelTask* d_task = nullptr;    // the backward solves (k_selinv tasks)
+    int32_t ntask = 0;
+    int64_t y_elems = 0;          // Y (and the same again for U)
 };
 
 struct CplState {
     std::vector<CplChunk> chunks;
     std::vector<void*> allocs;
-    Arena arena;  // one chunk's factor object and selected-inversion pool
+    Arena arena;  // one chunk's factor object, Y, U and the solve counters
+    double* d_eye = nullptr;  // a 64 x 64 identity
     int32_t* d_septgt = nullptr;
     double* d_sig = nullptr;  // Sigma (ndz + nlz)
     double* d_out = nullptr;  // the result (ndz + nlz)
@@ -1464,7 +1469,7 @@ namespace {
 int ensure_sel_plan(mf_plan* pl) {
     if (pl->sel) return MF_OK;
     SelPlan* S = new SelPlan();
-    std::string err = build_selinv(pl->H, *S, pl->sel_x_only);
+    std::string err = build_selinv(pl->H, *S);
     if (!err.empty()) {
         delete S;
         return fail(MF_EARG, "selected-inversion schedule: " + err);
@@ -1584,7 +1589,12 @@ static int ensure_cpl(mf_plan* pl) {
     const HostPlan& H = pl->H;
     const int64_t nfr = H.nfr;
     CplState* C = new CplState();
-    auto nj_of = [&](int64_t j) { return (H.resptr[j + 1] - H.resptr[j]) + (H.sepptr[j + 1] - H.sepptr[j]); };
+    // K's separator rows: na, padded to a multiple of 64 in a big front
+    auto sep_of = [&](int64_t j) {
+        const int64_t nn = H.resptr[j + 1] - H.resptr[j], na = H.sepptr[j + 1] - H.sepptr[j];
+        return nn + na > SMALL_NJ ? (na + 63) / 64 * 64 : na;
+    };
+    auto nj_of = [&](int64_t j) { return (H.resptr[j + 1] - H.resptr[j]) + sep_of(j); };
     const int64_t budget = cpl_budget();
     int rc = MF_OK;
     for (int64_t t0 = 0; t0 < nfr && !rc;) {
@@ -1599,9 +1609,13 @@ static int ensure_cpl(mf_plan* pl) {
         c.t1 = t1;
         std::vector<int64_t> resptr(m + 1, 1), ones(m + 2, 1), Dptr(m + 1, 1), dummy(1, 1);
         std::vector<int32_t> fr(m);
+        c.sep.resize(m);
+        c.sepp.resize(m);
         for (int64_t u = 0; u < m; ++u) {
             const int64_t j = nfr - 1 - (t0 + u), nj = nj_of(j);
             fr[u] = (int32_t)j;
+            c.sep[u] = H.sepptr[j + 1] - H.sepptr[j];
+            c.sepp[u] = sep_of(j);
             resptr[u + 1] = resptr[u] + nj;
             Dptr[u + 1] = Dptr[u] + nj * nj;
         }
@@ -1609,7 +1623,9 @@ static int ensure_cpl(mf_plan* pl) {
                             dummy.data(), Dptr.data(), ones.data(), 0, &c.syn);
         if (rc) break;
         const HostPlan& SH = c.syn->H;
+        const int32_t NJ = (int32_t)SH.fronts.size();
         std::vector<int64_t> poff(m);
+        std::vector<int32_t> pad(NJ, 0), nnr(NJ, 0);
         for (int64_t u = 0; u < m; ++u) {
             const int32_t J = SH.owner[u];
             const Front& F = SH.fronts[J];
@@ -1619,18 +1635,26 @@ static int ensure_cpl(mf_plan* pl) {
                 break;
             }
             poff[u] = F.poff;
-            c.big = c.big || !SH.is_small[J];
+            pad[J] = (int32_t)c.sepp[u];
+            nnr[J] = (int32_t)(F.nn - c.sepp[u]);
         }
-        if (!rc && ((rc = upload(fr, &c.d_fr, C->allocs)) || (rc = upload(poff, &c.d_poff, C->allocs)))) break;
         if (rc) break;
-        // the arena holds the largest chunk's factor and selected-inversion pool
-        size_t need = arena_factor_bytes(SH);
-        if (c.big) {
-            c.syn->sel_x_only = true;
-            if ((rc = ensure_sel_plan(c.syn))) break;
-            const SelPlan& SP = *c.syn->sel;
-            need += arena_round(8 * (size_t)(2 * SP.x_elems + SP.lt_elems + SP.m_elems + SH.panel_elems));
+        SelPlan SP;
+        const std::string err = build_cpl_solve(SH, pad, nnr, SP);
+        if (!err.empty()) {
+            rc = fail(MF_EARG, err);
+            break;
         }
+        std::vector<int64_t> xoff(m);
+        for (int64_t u = 0; u < m; ++u) xoff[u] = SP.xoff[SH.owner[u]];
+        c.ntask = (int32_t)SP.task.size();
+        c.y_elems = SP.x_elems;
+        if ((rc = upload(fr, &c.d_fr, C->allocs)) || (rc = upload(poff, &c.d_poff, C->allocs)) ||
+            (rc = upload(xoff, &c.d_xoff, C->allocs)) || (c.ntask && (rc = upload(SP.task, &c.d_task, C->allocs))))
+            break;
+        // the arena holds the largest chunk's factor object, Y, U and counters
+        const size_t need = arena_factor_bytes(SH) + 2 * arena_round(8 * (size_t)c.y_elems) +
+                            arena_round(4 * (size_t)(NJ + 1));
         C->arena.size = std::max(C->arena.size, need);
         c.resptr = std::move(resptr);
         C->chunks.push_back(std::move(c));
@@ -1638,7 +1662,9 @@ static int ensure_cpl(mf_plan* pl) {
     }
     if (!rc) {
         std::vector<int32_t> septgt(H.septgt.begin(), H.septgt.end());
-        rc = upload(septgt, &C->d_septgt, C->allocs);
+        std::vector<double> eye(64 * 64, 0.0);
+        for (int i = 0; i < 64; ++i) eye[i + 64 * i] = 1.0;
+        if (!(rc = upload(septgt, &C->d_septgt, C->allocs))) rc = upload(eye, &C->d_eye, C->allocs);
     }
     auto dalloc = [&](double** p, size_t n) {
         if (hipMalloc((void**)p, 8 * std::max<size_t>(n, 1)) != hipSuccess) return false;
@@ -1676,8 +1702,8 @@ static void cpl_transpose(const HostPlan& H, const double* Din, const double* Li
     }
 }
 
-// One chunk of the completion: K into the chunk factor's panel, Cholesky in
-// place, X of the big roots by the selected-inversion schedule, the outputs
+// One chunk of the completion: K into the chunk factor's panels, Cholesky
+// in place, the backward solves Y of the big roots on k_selinv, the outputs
 // of the chunk's fronts into d_out.  *inf: the forest's failing column
 // (1-based in the chunk), 0 if none.
 static int cpl_chunk(mf_plan* plan, CplChunk& c, hipStream_t caller, int64_t* inf) {
@@ -1687,10 +1713,6 @@ static int cpl_chunk(mf_plan* plan, CplChunk& c, hipStream_t caller, int64_t* in
     C.arena.used = 0;
     int rc = factor_create(c.syn, MF_FP64, &C.arena, &sf);
     if (rc) return rc;
-    if (c.big) {
-        sf->sel_pool = C.arena.base + C.arena.used;  // returned to the arena below
-        sf->sel_bytes = C.arena.size - C.arena.used;
-    }
     const int m = (int)(c.t1 - c.t0);
     // Sigma was copied on the caller's stream
     hipEvent_t ev;
@@ -1706,25 +1728,36 @@ static int cpl_chunk(mf_plan* plan, CplChunk& c, hipStream_t caller, int64_t* in
     sf->sh_valid = false;
     sf->ref_stale = false;
     rc = mf_factorize(sf, MF_CHOLESKY, nullptr, MF_REG_NONE, 0.0, 0.0, inf);
+    double* Y = (double*)(C.arena.base + C.arena.used);
+    if (!rc && !*inf && c.ntask) {
+        const HostPlan& SH = c.syn->H;
+        double* U = (double*)((char*)Y + arena_round(8 * (size_t)c.y_elems));
+        int32_t* cnt = (int32_t*)((char*)U + arena_round(8 * (size_t)c.y_elems));
+        DevSel<double> D{};
+        D.base[SP_PANEL] = (double*)sf->panel;
+        D.base[SP_W] = (double*)sf->W;
+        D.base[SP_X] = Y;
+        D.base[SP_T] = U;
+        D.base[SP_LT] = C.d_eye;
+        D.task = c.d_task;
+        D.xoff = c.d_xoff;
+        D.cnt = cnt;
+        D.ticket = sf->ticket;
+        D.hang = sf->hang;
+        D.stall = sf->stall;
+        D.ntask = c.ntask;
+        launch_fill<int32_t>(cnt, (int64_t)SH.fronts.size() + 1, 0, sf->stream);
+        launch_fill<uint32_t>(sf->ticket, 1, 0u, sf->stream);
+        launch_selinv<double>(c.syn->D, D, sf->stream);
+        if (hipGetLastError() != hipSuccess) rc = fail(MF_EHIP, "completion: launch failed");
+        if (!rc) rc = read_info(sf, nullptr);  // the solves' watchdog
+    }
     if (!rc && !*inf) {
-        if (c.big) rc = mf_selinv(sf);
-        if (!rc && c.big && !c.d_xoff) {
-            const HostPlan& SH = c.syn->H;
-            std::vector<int64_t> xoff(m, -1);
-            for (int u = 0; u < m; ++u) {
-                const int32_t J = SH.owner[u];
-                if (!SH.is_small[J]) xoff[u] = c.syn->sel->xoff[J];
-            }
-            rc = upload(xoff, &c.d_xoff, C.allocs);
-        }
-        if (!rc) {
-            launch_cpl_extract(plan->D, m, c.d_fr, (const double*)sf->sel_pool, c.d_xoff, (const double*)sf->panel,
-                               c.d_poff, C.d_out, C.d_out + H.ndz, sf->stream);
-            if (hipGetLastError() != hipSuccess) rc = fail(MF_EHIP, "completion: launch failed");
-        }
+        launch_cpl_extract(plan->D, m, c.d_fr, Y, c.ntask ? c.d_xoff : nullptr, (const double*)sf->panel, c.d_poff,
+                           C.d_out, C.d_out + H.ndz, sf->stream);
+        if (hipGetLastError() != hipSuccess) rc = fail(MF_EHIP, "completion: launch failed");
     }
     if (hipStreamSynchronize(sf->stream) != hipSuccess && !rc) rc = fail(MF_EHIP, "completion: stream failed");
-    sf->sel_pool = nullptr;
     (void)mf_factor_destroy(sf);
     return rc;
 }
@@ -1758,9 +1791,8 @@ int mf_complete_impl(mf_plan* plan, double* Dval, double* Lval, int uplo, int64_
             // or of the reordered Schur complement
             const int64_t col = inf - 1;
             const int64_t u = std::upper_bound(c.resptr.begin(), c.resptr.end(), col + 1) - c.resptr.begin() - 1;
-            const int64_t j = H.nfr - 1 - (c.t0 + u), na = H.sepptr[j + 1] - H.sepptr[j];
-            const int64_t k = col - (c.resptr[u] - 1);
-            if (info) *info = k < na ? k + 1 : k - na + 1;
+            const int64_t k = col - (c.resptr[u] - 1);  // padding columns (na <= k < na') cannot fail
+            if (info) *info = k < c.sep[u] ? k + 1 : k - c.sepp[u] + 1;
             return MF_OK;
         }
     }

@@ -15,11 +15,15 @@
 // rows: D11(r, c) = X(na + nn-1-c, na + nn-1-r), L21(r, c) = X(na + nn-1-c, r).
 // The fronts are independent given Sigma, so the library builds forests of
 // these K (plans whose fronts are all roots, in chunks that fit a memory
-// budget), factors them in place (the big ones on the tile-DAG), takes X of
-// the big fronts from the selected-inversion schedule (k_selinv's L11^-1)
-// and inverts the small ones here.  k_cpl_gather writes K into the forest's
-// panels, k_cpl_extract the outputs; both run over one chunk, whose root u
-// is reference front fr[u].
+// budget) and factors them in place (the big ones on the tile-DAG).  A big
+// front (nJ > 64) is padded: its separator part grows to na' = 64 ceil(na /
+// 64) rows with identity rows and columns between M and the residual part,
+// which leaves the Schur complement and hence the outputs unchanged and
+// puts the residual part on a tile boundary.  The last nn rows of its
+// inverse factor come from backward solves Y = L^-T [0; I_nn] run as tile
+// tasks of k_selinv; a small front's factor is inverted here in LDS.
+// k_cpl_gather writes K into the forest's panels, k_cpl_extract the
+// outputs; both run over one chunk, whose root u is reference front fr[u].
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -48,15 +52,19 @@ __device__ __forceinline__ double sigma_at(const DevPlan& P, const double* Dval,
     return Lval[o.Lp + lo + (int64_t)lb * o.na];
 }
 
+// Separator rows of a front's K: na, padded to a multiple of 64 for a big
+// front (nn + na > 64).
+__device__ __forceinline__ int cpl_sep_rows(int nn, int na) { return nn + na > 64 ? (na + 63) & ~63 : na; }
+
 // Workgroups (u, y) for root u (reference front fr[u]): columns c = y,
-// y + gridDim.y, ... of its K (nJ x nJ column-major, both triangles, at
-// K + koff[u]).
+// y + gridDim.y, ... of its K (nJ' x nJ' column-major, both triangles, at
+// K + koff[u]; nJ' = na' + nn).
 __global__ __launch_bounds__(256) void k_cpl_gather(DevPlan P, const int32_t* fr, const double* Dval,
                                                     const double* Lval, const int32_t* septgt,
                                                     const int64_t* koff, double* K) {
     const int u = blockIdx.x;
     const OutFront o = P.out[fr[u]];
-    const int nn = o.nn, na = o.na, nj = nn + na;
+    const int nn = o.nn, na = o.na, np = cpl_sep_rows(nn, na), nj = nn + np;
     double* Kj = K + koff[u];
     const double* D11 = Dval + o.Dp;
     const double* L21 = Lval + o.Lp;
@@ -64,9 +72,11 @@ __global__ __launch_bounds__(256) void k_cpl_gather(DevPlan P, const int32_t* fr
     for (int c = blockIdx.y; c < nj; c += gridDim.y)
         for (int r = c + threadIdx.x; r < nj; r += blockDim.x) {
             double v;
-            if (r < na) v = sigma_at(P, Dval, Lval, septgt, sep[r], sep[c]);          // M
-            else if (c < na) v = L21[c + (int64_t)(nn - 1 - (r - na)) * na];           // J Sigma21'
-            else v = D11[(nn - 1 - (c - na)) + (int64_t)(nn - 1 - (r - na)) * nn];     // J Sigma11 J
+            if (r < na) v = sigma_at(P, Dval, Lval, septgt, sep[r], sep[c]);           // M
+            else if (r < np) v = r == c ? 1.0 : 0.0;                                    // padding
+            else if (c < na) v = L21[c + (int64_t)(nn - 1 - (r - np)) * na];            // J Sigma21'
+            else if (c < np) v = 0.0;
+            else v = D11[(nn - 1 - (c - np)) + (int64_t)(nn - 1 - (r - np)) * nn];      // J Sigma11 J
             Kj[r + (int64_t)c * nj] = v;
             Kj[c + (int64_t)r * nj] = v;  // both triangles: K is read as a full block
         }
@@ -74,9 +84,9 @@ __global__ __launch_bounds__(256) void k_cpl_gather(DevPlan P, const int32_t* fr
 
 // One workgroup per root u (reference front fr[u]): D11 and L21 of the
 // completion from the last nn rows of X = chol(K)^-1.  xoff (NULL: none) with
-// xoff[u] >= 0: X of a big front, lower triangle, ld nJ, at X + xoff[u];
-// else the factor of K (a small front, nJ <= 64) at L + loff[u] (ld nJ),
-// inverted here in LDS.
+// xoff[u] >= 0: a big front, Y = (those rows)' (nJ' x nn, ld nJ') at
+// X + xoff[u]; else the factor of K (a small front, nJ <= 64) at L + loff[u]
+// (ld nJ), inverted here in LDS.
 __global__ __launch_bounds__(256) void k_cpl_extract(DevPlan P, const int32_t* fr, const double* X,
                                                      const int64_t* xoff, const double* L,
                                                      const int64_t* loff, double* Dval, double* Lval) {
@@ -84,12 +94,25 @@ __global__ __launch_bounds__(256) void k_cpl_extract(DevPlan P, const int32_t* f
     const int u = blockIdx.x, tid = threadIdx.x;
     const OutFront o = P.out[fr[u]];
     const int nn = o.nn, na = o.na, nj = nn + na;
+    double* D11 = Dval + o.Dp;
+    double* L21 = Lval + o.Lp;
+    if (xoff && xoff[u] >= 0) {
+        // X(na + q, col) = Y(col', q) with the padded column col' of col
+        const int np = cpl_sep_rows(nn, na), ldy = nn + np;
+        const double* Y = X + xoff[u];
+        for (int e = tid; e < nn * nn; e += blockDim.x) {
+            const int c = e / nn, r = e - c * nn;
+            D11[e] = r >= c ? Y[(np + nn - 1 - r) + (int64_t)(nn - 1 - c) * ldy] : 0.0;
+        }
+        for (int e = tid; e < na * nn; e += blockDim.x) {
+            const int c = e / na, r = e - c * na;
+            L21[e] = Y[r + (int64_t)(nn - 1 - c) * ldy];
+        }
+        return;
+    }
     const double* Xj;
     int64_t ldx;
-    if (xoff && xoff[u] >= 0) {
-        Xj = X + xoff[u];
-        ldx = nj;
-    } else {
+    {
         // small front: X = L^-1 by forward substitution, column c by thread c
         const double* Lj = L + loff[u];
         for (int e = tid; e < nj * nj; e += blockDim.x) {
@@ -111,8 +134,6 @@ __global__ __launch_bounds__(256) void k_cpl_extract(DevPlan P, const int32_t* f
         Xj = Xs;
         ldx = 65;
     }
-    double* D11 = Dval + o.Dp;
-    double* L21 = Lval + o.Lp;
     for (int e = tid; e < nn * nn; e += blockDim.x) {
         const int c = e / nn, r = e - c * nn;
         D11[e] = r >= c ? Xj[(na + nn - 1 - c) + (int64_t)(na + nn - 1 - r) * ldx] : 0.0;

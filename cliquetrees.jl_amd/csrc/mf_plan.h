@@ -119,9 +119,13 @@ struct SelPlan {
 };
 
 // Builds the selected-inversion schedule of a plan (Cholesky factors).
-// x_only: only X = L11^-1 of the big fronts (no Sigma), for a forest whose
-// fronts are all roots (the completion's).
-std::string build_selinv(const HostPlan& H, SelPlan& S, bool x_only = false);
+std::string build_selinv(const HostPlan& H, SelPlan& S);
+
+// Builds the completion's backward-solve schedule on a forest of roots:
+// per big root J, pad_na[J] (a multiple of 64) padded separator rows and
+// nnr[J] residual columns; the tasks run on k_selinv.
+std::string build_cpl_solve(const HostPlan& H, const std::vector<int32_t>& pad_na,
+                            const std::vector<int32_t>& nnr, SelPlan& S);
 
 // Builds the plan; returns an empty string on success, else an error message.
 std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* resptr,

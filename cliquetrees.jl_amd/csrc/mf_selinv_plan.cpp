@@ -86,7 +86,7 @@ SelTask plain_task(int kind, int J, int a) {
 
 }  // namespace
 
-std::string build_selinv(const HostPlan& H, SelPlan& S, bool x_only) {
+std::string build_selinv(const HostPlan& H, SelPlan& S) {
     const int32_t NJ = (int32_t)H.fronts.size();
     S = SelPlan();
     S.total.assign(NJ, 0);
@@ -130,11 +130,9 @@ std::string build_selinv(const HostPlan& H, SelPlan& S, bool x_only) {
             done += (int32_t)(v.size() - first);
         };
         if (H.is_small[J]) {
-            if (!x_only) {
-                SelTask t = plain_task(ST_SMALL, J, 0);
-                B[J].push_back(t);
-                close_phase(B[J], 0);
-            }
+            SelTask t = plain_task(ST_SMALL, J, 0);
+            B[J].push_back(t);
+            close_phase(B[J], 0);
         } else {
             const int nk = ntiles(nn), nr = ntiles(na);
             const int64_t po = F.poff, xo = S.xoff[J], lo = S.ltoff[J], mo = S.moff[J];
@@ -217,7 +215,7 @@ std::string build_selinv(const HostPlan& H, SelPlan& S, bool x_only) {
             }
             // Sigma11(i, j) = sum_{k >= 64 i} X(k, i) X(k, j) - sum_k Lt(k, i) Sigma21(k, j), i >= j
             f0 = B[J].size();
-            for (int i = 0; i < nk && !x_only; ++i)
+            for (int i = 0; i < nk; ++i)
                 for (int j = 0; j <= i; ++j) {
                     SelTask t = gemm_task(J, SP_SIG, po + (int64_t)NB * i + (int64_t)NB * j * nJ, nJ,
                                           tile_len(nn, i), tile_len(nn, j), i == j);
@@ -289,6 +287,108 @@ std::string build_selinv(const HostPlan& H, SelPlan& S, bool x_only) {
             if (p.b_off < 0 || span(p.b_off, t.n, p.k, p.ldb, p.tb) >= size[p.arrB])
                 return "selinv operand B out of bounds";
         }
+    }
+    return std::string();
+}
+
+// ---- the completion's backward solves ---------------------------------------
+//
+// Per big root J of a completion forest (nn_f = nJ' columns, factor
+// L = chol(K') in the panel, inverted 64 x 64 diagonal tiles in W), with
+// the separator part padded to na' = 64 s rows and nn = nnr[J] residual
+// columns after it: Y = L^-T [0; I_nn] (nJ' x nn, ld nJ', in SP_X at
+// xoff[J]), whose rows are the last nn rows of L^-1.  RHS column tile c
+// has its identity block in row tile s + c, so its chain starts there:
+//   U(I, c) = -sum_{K = I+1}^{s+c} L(K, I)' Y(K, c)     (SP_T, same layout)
+//   Y(I, c) = W_I' U(I, c),   Y(s+c, c) = W_{s+c}' I    (SP_LT: a 64 x 64 identity)
+// for I = s + c - 1 down to 0.  Phases per front: U(I) then Y(I) for I
+// descending, each holding the RHS tiles whose chain has reached I; the
+// fronts' phases are interleaved round-robin (the longest chains first).
+std::string build_cpl_solve(const HostPlan& H, const std::vector<int32_t>& pad_na,
+                            const std::vector<int32_t>& nnr, SelPlan& S) {
+    const int32_t NJ = (int32_t)H.fronts.size();
+    S = SelPlan();
+    S.total.assign(NJ, 0);
+    S.xoff.assign(NJ, -1);
+    S.ltoff.assign(NJ, -1);
+    S.moff.assign(NJ, -1);
+    std::vector<std::vector<std::vector<SelTask>>> ph(NJ);
+    for (int32_t J = 0; J < NJ; ++J) {
+        if (H.is_small[J]) continue;
+        const Front& F = H.fronts[J];
+        const int nJ = F.nn, s = pad_na[J] / NB, nn = nnr[J], nrt = ntiles(nn);
+        if (F.na != 0 || F.woff < 0 || pad_na[J] % NB || pad_na[J] + nn != nJ || nn <= 0)
+            return "completion solve: front " + std::to_string(J) + " is not a padded forest root";
+        const int64_t po = F.poff, wo = F.woff, yo = S.x_elems;
+        S.xoff[J] = yo;
+        S.x_elems += (int64_t)nJ * NB * nrt;
+        for (int I = s + nrt - 1; I >= 0; --I) {
+            const int m = tile_len(nJ, I);
+            std::vector<SelTask> U, Y;
+            for (int c = 0; c < nrt; ++c) {
+                const int top = s + c, mc = tile_len(nn, c);
+                if (I > top) continue;
+                const int64_t cy = (int64_t)NB * I + (int64_t)NB * c * nJ;  // tile (I, c) in Y / U
+                SelTask y = gemm_task(J, SP_X, yo + cy, nJ, m, mc, false);
+                if (I < top) {
+                    const int k = (int)(std::min<int64_t>((int64_t)NB * (top + 1), nJ) - (int64_t)NB * (I + 1));
+                    SelTask u = gemm_task(J, SP_T, yo + cy, nJ, m, mc, false);
+                    add_part(u, SP_PANEL, po + (int64_t)NB * (I + 1) + (int64_t)NB * I * nJ, nJ, 1, SP_X,
+                             yo + (int64_t)NB * (I + 1) + (int64_t)NB * c * nJ, nJ, 1, k, -1);
+                    U.push_back(u);
+                    add_part(y, SP_W, wo + (int64_t)NB * I, nJ, 1, SP_T, yo + cy, nJ, 1, m, 1);
+                } else {
+                    add_part(y, SP_W, wo + (int64_t)NB * I, nJ, 1, SP_LT, 0, NB, 1, m, 1);
+                }
+                Y.push_back(y);
+            }
+            if (!U.empty()) ph[J].push_back(std::move(U));
+            ph[J].push_back(std::move(Y));
+        }
+    }
+    // ticket order: phase p of every front, for p = 0, 1, ...; fronts with
+    // more phases first; a task waits for its front's earlier phases
+    std::vector<int32_t> order;
+    for (int32_t J = 0; J < NJ; ++J)
+        if (!ph[J].empty()) order.push_back(J);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return ph[a].size() > ph[b].size(); });
+    std::vector<int32_t> done(NJ, 0);
+    for (size_t p = 0; !order.empty() && p < ph[order[0]].size(); ++p)
+        for (int32_t J : order) {
+            if (p >= ph[J].size()) break;  // sorted: the rest have fewer phases
+            for (SelTask t : ph[J][p]) {
+                t.wait[0] = done[J] > 0 ? J : -1;
+                t.target[0] = done[J];
+                S.task.push_back(t);
+            }
+            done[J] += (int32_t)ph[J][p].size();
+        }
+    for (int32_t J = 0; J < NJ; ++J) S.total[J] = done[J];
+    for (const SelTask& t : S.task) {
+        ++S.n_gemm;
+        for (int q = 0; q < t.np; ++q) S.flops += 2.0 * t.m * t.n * t.p[q].k;
+    }
+    // bounds of every product against its buffers, and every wait against
+    // the tasks of smaller tickets
+    const int64_t size[SP_N] = {H.panel_elems, H.w_elems, S.x_elems, S.x_elems, (int64_t)NB * NB, 0, 0};
+    auto last = [](int64_t off, int rows, int cols, int ld, int trans) {
+        return trans ? off + (cols - 1) + (int64_t)(rows - 1) * ld : off + (rows - 1) + (int64_t)(cols - 1) * ld;
+    };
+    std::vector<int32_t> seen(NJ, 0);
+    for (const SelTask& t : S.task) {
+        if (t.m <= 0 || t.n <= 0 || t.m > NB || t.n > NB || t.np < 1) return "completion solve: bad tile";
+        if (t.c_off < 0 || last(t.c_off, t.m, t.n, t.ldc, 0) >= size[t.arrC]) return "completion solve: output out of bounds";
+        for (int q = 0; q < t.np; ++q) {
+            const SelPart& pp = t.p[q];
+            if (pp.k <= 0) return "completion solve: empty product";
+            if (pp.a_off < 0 || last(pp.a_off, t.m, pp.k, pp.lda, pp.ta) >= size[pp.arrA])
+                return "completion solve: operand A out of bounds";
+            if (pp.b_off < 0 || last(pp.b_off, t.n, pp.k, pp.ldb, pp.tb) >= size[pp.arrB])
+                return "completion solve: operand B out of bounds";
+        }
+        if (t.wait[0] >= 0 && seen[t.wait[0]] < t.target[0]) return "completion solve: wait on a later ticket";
+        ++seen[t.front];
     }
     return std::string();
 }
