@@ -65,6 +65,10 @@ def sym():
         lib.mf_selinv_check.argtypes = ([ctypes.c_int64] * 2 + [i64p] * 8 + [f64p] * 4 +
                                         [i64p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64])
         lib.mf_selinv_check.restype = ctypes.c_int
+        lib.mf_cpl_solve_check.argtypes = [ctypes.c_int64, i64p, i64p, ctypes.c_uint64,
+                                           ctypes.POINTER(ctypes.c_double), i64p, ctypes.c_int64,
+                                           ctypes.c_char_p, ctypes.c_int64]
+        lib.mf_cpl_solve_check.restype = ctypes.c_int
         _sym = lib
     return _sym
 

@@ -617,3 +617,145 @@ extern "C" int mf_selinv_check(int64_t n, int64_t nfr, const int64_t* resptr, co
     for (int64_t i = 0; stats && i < nstats && i < (int64_t)(sizeof(v) / sizeof(v[0])); ++i) stats[i] = v[i];
     return err.empty() ? 0 : -1;
 }
+
+// Host-only entry (built into libmfsym.so): the completion's backward-solve
+// schedule on a forest of nfr roots (separator na[u], residual nn[u]; a
+// big root padded as the library pads it), run on the host in ticket order
+// from the Cholesky factor of a random positive definite K per root (seeded)
+// with its inverted 64 x 64 diagonal tiles, every wait checked and every
+// value read checked to be written; *worst = the largest relative
+// difference of the rows the completion reads against L^-T e by plain
+// backward substitution.  stats: tasks, flops, Y elements.
+extern "C" int mf_cpl_solve_check(int64_t nfr, const int64_t* na, const int64_t* nn, uint64_t seed, double* worst,
+                                  int64_t* stats, int64_t nstats, char* msg, int64_t msglen) {
+    using namespace mf;
+    auto fail_msg = [&](const std::string& m) {
+        if (msg && msglen > 0) {
+            std::strncpy(msg, m.c_str(), (size_t)msglen - 1);
+            msg[msglen - 1] = 0;
+        }
+        return -1;
+    };
+    if (nfr <= 0 || !na || !nn || !worst) return fail_msg("bad argument");
+    auto padded = [&](int64_t u) { return na[u] + nn[u] > SMALL_NJ ? (na[u] + 63) / 64 * 64 : na[u]; };
+    std::vector<int64_t> resptr(nfr + 1, 1), ones(nfr + 2, 1), Dptr(nfr + 1, 1), dummy(1, 1);
+    for (int64_t u = 0; u < nfr; ++u) {
+        if (na[u] < 0 || nn[u] <= 0) return fail_msg("bad front size");
+        const int64_t nj = nn[u] + padded(u);
+        resptr[u + 1] = resptr[u] + nj;
+        Dptr[u + 1] = Dptr[u] + nj * nj;
+    }
+    HostPlan H;
+    std::string err = build_plan(H, resptr[nfr] - 1, nfr, resptr.data(), ones.data(), dummy.data(), dummy.data(),
+                                 ones.data(), dummy.data(), Dptr.data(), ones.data(), PlanTuning());
+    if (!err.empty()) return fail_msg(err);
+    const int32_t NJ = (int32_t)H.fronts.size();
+    std::vector<int32_t> pad(NJ, 0), nnr(NJ, 0);
+    for (int64_t u = 0; u < nfr; ++u) {
+        const int32_t J = H.owner[u];
+        pad[J] = (int32_t)padded(u);
+        nnr[J] = (int32_t)nn[u];
+    }
+    SelPlan S;
+    err = build_cpl_solve(H, pad, nnr, S);
+    if (!err.empty()) return fail_msg(err);
+    const double unset = std::nan("");
+    std::vector<double> panel(H.panel_elems, 0.0), W(H.w_elems, 0.0), Y(S.x_elems, unset), U(S.x_elems, unset),
+        eye(NB * NB, 0.0);
+    for (int i = 0; i < NB; ++i) eye[i + NB * i] = 1.0;
+    uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    };
+    std::vector<std::vector<double>> Ls(NJ);
+    for (int32_t J = 0; J < NJ; ++J) {
+        const Front& F = H.fronts[J];
+        const int n = F.nn;
+        std::vector<double> G((size_t)n * n), L((size_t)n * n, 0.0);
+        for (double& g : G) g = rnd();
+        for (int j = 0; j < n; ++j)
+            for (int i = j; i < n; ++i) {
+                double s = i == j ? (double)n : 0.0;
+                for (int k = 0; k < n; ++k) s += G[i + (size_t)k * n] * G[j + (size_t)k * n];
+                L[i + (size_t)j * n] = s;  // K, lower triangle
+            }
+        for (int j = 0; j < n; ++j) {  // Cholesky in place
+            double d = L[j + (size_t)j * n];
+            for (int k = 0; k < j; ++k) d -= L[j + (size_t)k * n] * L[j + (size_t)k * n];
+            L[j + (size_t)j * n] = std::sqrt(d);
+            for (int i = j + 1; i < n; ++i) {
+                double s = L[i + (size_t)j * n];
+                for (int k = 0; k < j; ++k) s -= L[i + (size_t)k * n] * L[j + (size_t)k * n];
+                L[i + (size_t)j * n] = s / L[j + (size_t)j * n];
+            }
+        }
+        for (int j = 0; j < n; ++j)
+            for (int i = j; i < n; ++i) panel[F.poff + i + (size_t)j * n] = L[i + (size_t)j * n];
+        if (!H.is_small[J])
+            for (int k0 = 0; k0 < n; k0 += NB) {  // W: (row k0 + r, column c) at woff + k0 + r + c n
+                const int kb = std::min(NB, n - k0);
+                for (int c = 0; c < kb; ++c) {
+                    std::vector<double> x(kb, 0.0);
+                    x[c] = 1.0;
+                    for (int r = 0; r < kb; ++r) {
+                        double s = x[r];
+                        for (int k = 0; k < r; ++k) s -= L[k0 + r + (size_t)(k0 + k) * n] * x[k];
+                        x[r] = s / L[k0 + r + (size_t)(k0 + r) * n];
+                    }
+                    for (int r = 0; r < kb; ++r) W[F.woff + k0 + r + (size_t)c * n] = x[r];
+                }
+            }
+        Ls[J] = std::move(L);
+    }
+    double* base[SP_N] = {panel.data(), W.data(), Y.data(), U.data(), eye.data(), nullptr, nullptr};
+    std::vector<int32_t> cnt(NJ, 0);
+    std::vector<double> acc((size_t)NB * NB);
+    for (const SelTask& t : S.task) {
+        if (t.wait[0] >= 0 && cnt[t.wait[0]] < t.target[0]) return fail_msg("completion solve: wait not met");
+        std::fill(acc.begin(), acc.end(), 0.0);
+        for (int q = 0; q < t.np; ++q) {
+            const SelPart& p = t.p[q];
+            const double* A = base[p.arrA] + p.a_off;
+            const double* B = base[p.arrB] + p.b_off;
+            for (int c = 0; c < t.n; ++c)
+                for (int r = 0; r < t.m; ++r) {
+                    double s = 0.0;
+                    for (int k = 0; k < p.k; ++k) {
+                        const double a = p.ta ? A[k + (size_t)r * p.lda] : A[r + (size_t)k * p.lda];
+                        const double b = p.tb ? B[k + (size_t)c * p.ldb] : B[c + (size_t)k * p.ldb];
+                        if (std::isnan(a) || std::isnan(b)) return fail_msg("completion solve: reads an unwritten value");
+                        s += a * b;
+                    }
+                    acc[r + (size_t)NB * c] += p.sgn * s;
+                }
+        }
+        double* C = base[t.arrC] + t.c_off;
+        for (int c = 0; c < t.n; ++c)
+            for (int r = 0; r < t.m; ++r) C[r + (size_t)c * t.ldc] = acc[r + (size_t)NB * c];
+        ++cnt[t.front];
+    }
+    double w = 0.0;
+    for (int32_t J = 0; J < NJ; ++J) {
+        if (H.is_small[J]) continue;
+        const int n = H.fronts[J].nn, p = pad[J];
+        const std::vector<double>& L = Ls[J];
+        for (int q = 0; q < nnr[J]; ++q) {
+            std::vector<double> x(n, 0.0);
+            x[p + q] = 1.0;
+            for (int i = n - 1; i >= 0; --i) {
+                double s = x[i];
+                for (int k = i + 1; k < n; ++k) s -= L[k + (size_t)i * n] * x[k];
+                x[i] = s / L[i + (size_t)i * n];
+            }
+            double xm = 0.0;
+            for (int i = 0; i <= p + q; ++i) xm = std::max(xm, std::fabs(x[i]));
+            for (int i = 0; i <= p + q; ++i)
+                w = std::max(w, std::fabs(Y[S.xoff[J] + i + (size_t)q * n] - x[i]) / xm);
+        }
+    }
+    *worst = w;
+    const int64_t v[] = {(int64_t)S.task.size(), (int64_t)std::llround(S.flops), S.x_elems};
+    for (int64_t i = 0; stats && i < nstats && i < 3; ++i) stats[i] = v[i];
+    return 0;
+}

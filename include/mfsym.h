@@ -122,6 +122,18 @@ int mf_selinv_check(int64_t n, int64_t nfr, const int64_t* resptr, const int64_t
                     const double* Dval, const double* Lval, double* SDval, double* SLval,
                     int64_t* stats, int64_t nstats, char* msg, int64_t msglen);
 
+/* Host-only check of the completion's backward-solve schedule
+ * (libmfhip.so's mf_complete_impl; no counterpart in the reference): a
+ * forest of nfr roots with separator na[u] and residual nn[u] (big roots
+ * padded as the library pads them), the schedule run on the host in ticket
+ * order from the Cholesky factor of a random positive definite matrix per
+ * root (seed), each wait and each value read checked; *worst = the largest
+ * difference of the rows the completion reads from backward substitution,
+ * relative to each column's largest entry.  stats: tasks, flops, Y
+ * elements.  Returns 0 or -1 (msg). */
+int mf_cpl_solve_check(int64_t nfr, const int64_t* na, const int64_t* nn, uint64_t seed, double* worst,
+                       int64_t* stats, int64_t nstats, char* msg, int64_t msglen);
+
 #ifdef __cplusplus
 }
 #endif
