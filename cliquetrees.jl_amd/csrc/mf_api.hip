@@ -927,6 +927,8 @@ int mf_factorize_async(mf_factor* f, int diag, const double* signs_dev, int reg_
     if (reg_kind == MF_REG_GMW81 && diag == MF_LDLT && !signs_dev)
         return fail(MF_EARG, "GMW81 with LDLt needs signs");
     const HostPlan& H = f->plan->H;
+    if (reg_kind == MF_REG_GMW81 && H.dag_gmw.size() != H.dag.size())
+        return fail(MF_EARG, "this plan has no GMW81 order");
     f->diag = diag;
     f->reg_kind = reg_kind;
     f->delta = delta;
@@ -986,6 +988,7 @@ int mf_factor_get(mf_factor* f, double* Dval, double* Lval, double* d, int devic
 int mf_solve_natural(mf_factor* f, double* C, int64_t nrhs, int64_t ldc, int device) {
     if (!f || nrhs < 0 || (nrhs > 0 && !C)) return fail(MF_EARG, "bad argument");
     const HostPlan& H = f->plan->H;
+    if (H.tune.factor_only) return fail(MF_EARG, "this plan has no solve tasks");
     if (ldc < H.n) return fail(MF_EARG, "ldc < n");
     if (nrhs == 0 || H.n == 0) return MF_OK;
     if (solves64(f) && !f->sh_valid) return fail(MF_ESTATE, "FP64 solves of an FP32 factor: factorize after enabling them");
@@ -1040,6 +1043,7 @@ int mf_solve_natural(mf_factor* f, double* C, int64_t nrhs, int64_t ldc, int dev
 int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t ldb, int device) {
     if (!f || !perm || nrhs < 0 || (nrhs > 0 && !B)) return fail(MF_EARG, "bad argument");
     const HostPlan& H = f->plan->H;
+    if (H.tune.factor_only) return fail(MF_EARG, "this plan has no solve tasks");
     if (ldb < H.n) return fail(MF_EARG, "ldb < n");
     if (nrhs == 0 || H.n == 0) return MF_OK;
     if (solves64(f) && !f->sh_valid) return fail(MF_ESTATE, "FP64 solves of an FP32 factor: factorize after enabling them");
@@ -1087,6 +1091,7 @@ int mf_solve(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t
 int mf_solve_noalloc(mf_factor* f, const int64_t* perm, double* B, int64_t nrhs, int64_t ldb) {
     if (!f || !perm || nrhs < 0 || (nrhs > 0 && !B)) return fail(MF_EARG, "bad argument");
     const HostPlan& H = f->plan->H;
+    if (H.tune.factor_only) return fail(MF_EARG, "this plan has no solve tasks");
     if (ldb < H.n) return fail(MF_EARG, "ldb < n");
     if (nrhs == 0 || H.n == 0) return MF_OK;
     if (solves64(f) && !f->sh_valid) return fail(MF_ESTATE, "FP64 solves of an FP32 factor: factorize after enabling them");
@@ -1619,8 +1624,10 @@ static int ensure_cpl(mf_plan* pl) {
             resptr[u + 1] = resptr[u] + nj;
             Dptr[u + 1] = Dptr[u] + nj * nj;
         }
-        rc = mf_plan_create(resptr[m] - 1, m, resptr.data(), ones.data(), dummy.data(), dummy.data(), ones.data(),
-                            dummy.data(), Dptr.data(), ones.data(), 0, &c.syn);
+        PlanTuning tune;
+        tune.factor_only = true;
+        rc = plan_create(resptr[m] - 1, m, resptr.data(), ones.data(), dummy.data(), dummy.data(), ones.data(),
+                         dummy.data(), Dptr.data(), ones.data(), tune, &c.syn);
         if (rc) break;
         const HostPlan& SH = c.syn->H;
         const int32_t NJ = (int32_t)SH.fronts.size();

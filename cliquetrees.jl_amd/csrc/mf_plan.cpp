@@ -1024,6 +1024,7 @@ std::string Planner::check_bounds() {
             }
         }
         for (const std::vector<PTask>* tl : {&P.ftask, &P.btask}) {
+            if (P.tune.factor_only) break;  // no solve task lists
             int64_t nsmall = (int64_t)P.small_order.size();
             for (const Level& L : P.levels) nsmall += (int64_t)L.small_list.size();
             int64_t seen = 0;
@@ -1058,7 +1059,7 @@ std::string Planner::check_bounds() {
             }
         }
         // every dependency points to an earlier ticket
-        {
+        if (!P.tune.factor_only) {
             // positions 4 i + slot (slot = wavefront inside a tiny ticket)
             std::vector<int64_t> first(NJ, -1), last(NJ, -1);
             auto mark = [&](const std::vector<PTask>& tl) {
@@ -1457,7 +1458,7 @@ std::string Planner::schedule() {
         // GMW81 launches: each diagonal-tile task also waits for the last
         // update of every tile below it in its block column, and takes about
         // 200 us (it reads the block column once per pivot)
-        {
+        if (!P.tune.factor_only) {
             std::vector<std::pair<int32_t, int32_t>> ge(gmw_edges);  // (successor, predecessor)
             for (int32_t t = 0; t < NT; ++t)
                 for (int32_t k = P.succ_ptr[t]; k < P.succ_ptr[t + 1]; ++k) ge.push_back({P.succ[k], t});
@@ -1485,11 +1486,9 @@ std::string Planner::schedule() {
             if (const char* e = simulate(sp2, sc2, d2, order_gmw, ms)) return std::string("gmw81: ") + e;
             cost = cost0;
         }
-        std::vector<DagTask> dag2(NT), dag3(NT);
-        for (int32_t i = 0; i < NT; ++i) {
-            dag2[i] = P.dag[order[i]];
-            dag3[i] = P.dag[order_gmw[i]];
-        }
+        std::vector<DagTask> dag2(NT), dag3(order_gmw.size());
+        for (int32_t i = 0; i < NT; ++i) dag2[i] = P.dag[order[i]];
+        for (size_t i = 0; i < order_gmw.size(); ++i) dag3[i] = P.dag[order_gmw[i]];
         P.dag.swap(dag2);
         P.dag_gmw.swap(dag3);
     }
@@ -1609,7 +1608,8 @@ std::string Planner::check_order() {
         return nullptr;
     };
     if (const char* e = validate_dag(P.dag, false)) return e;
-    if (const char* e = validate_dag(P.dag_gmw, true)) return std::string("gmw81 order: ") + e;
+    if (!P.tune.factor_only)
+        if (const char* e = validate_dag(P.dag_gmw, true)) return std::string("gmw81 order: ") + e;
     return std::string();
 }
 
@@ -1638,7 +1638,7 @@ std::string build_plan(HostPlan& P, int64_t n, int64_t nfr, const int64_t* respt
     if (e.empty()) e = pl.classify();
     if (e.empty()) e = pl.layout_memory();
     if (e.empty()) e = pl.factor_tasks();
-    if (e.empty()) e = pl.solve_tasks();
+    if (e.empty() && !tune.factor_only) e = pl.solve_tasks();
     if (e.empty()) e = pl.check_bounds();
     if (e.empty()) e = pl.dataflow();
     if (e.empty()) e = pl.schedule();

@@ -22,6 +22,8 @@ struct PlanTuning {
     int32_t prio = 0;                   // tile-DAG task kinds at raised wave priority (bit mask)
     int32_t fuse = 0;                   // diagonal tasks run the next panel solve and look-ahead update
     int32_t wg_per_cu = 0;              // resident tile-DAG workgroups per CU (0: as many as fit)
+    bool factor_only = false;           // a plan for Cholesky/LDLt factorizations only (the
+                                        // completion's forests): no solve task lists, no GMW81 order
 };
 
 // Task of the persistent triangular solves (one launch per direction).

@@ -646,8 +646,10 @@ extern "C" int mf_cpl_solve_check(int64_t nfr, const int64_t* na, const int64_t*
         Dptr[u + 1] = Dptr[u] + nj * nj;
     }
     HostPlan H;
+    PlanTuning tune;
+    tune.factor_only = true;  // as the library plans the completion's forests
     std::string err = build_plan(H, resptr[nfr] - 1, nfr, resptr.data(), ones.data(), dummy.data(), dummy.data(),
-                                 ones.data(), dummy.data(), Dptr.data(), ones.data(), PlanTuning());
+                                 ones.data(), dummy.data(), Dptr.data(), ones.data(), tune);
     if (!err.empty()) return fail_msg(err);
     const int32_t NJ = (int32_t)H.fronts.size();
     std::vector<int32_t> pad(NJ, 0), nnr(NJ, 0);
