@@ -1574,10 +1574,9 @@ int mf_selinv_get(mf_factor* f, double* Dval, double* Lval, int device) {
     return read_info(f, nullptr);
 }
 
-// Elements of K per chunk of the completion's forest (a chunk's factor
-// object and selected-inversion pool hold about four times that many
-// doubles: the panels, X, the recursion's scratch and Sigma);
-// MF_COMPLETE_CHUNK_ELEMS overrides.
+// Elements of K per chunk of the completion's forest (a chunk's arena holds
+// at most about four times that many doubles: the panels, the inverted
+// diagonal tiles and the solves' Y and U); MF_COMPLETE_CHUNK_ELEMS overrides.
 static int64_t cpl_budget() {
     if (const char* e = std::getenv("MF_COMPLETE_CHUNK_ELEMS")) {
         const long long v = std::atoll(e);
@@ -1585,7 +1584,7 @@ static int64_t cpl_budget() {
     }
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)1 << 33;
-    return std::max<int64_t>(std::min<int64_t>((int64_t)(fr / 64), (int64_t)1 << 29), (int64_t)1 << 16);
+    return std::max<int64_t>(std::min<int64_t>((int64_t)(fr / 64), (int64_t)1 << 30), (int64_t)1 << 16);
 }
 
 // Builds the completion's forest of a plan (on first use).

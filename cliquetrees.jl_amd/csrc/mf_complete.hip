@@ -34,52 +34,75 @@ namespace mf {
 
 namespace {
 
-// Sigma(a, b), a >= b global (permuted) columns, from the reference layout:
-// column b's front holds it in its diagonal block or its separator rows.
-__device__ __forceinline__ double sigma_at(const DevPlan& P, const double* Dval, const double* Lval,
-                                           const int32_t* septgt, int a, int b) {
-    const OutFront o = P.out[P.col2ref[b]];
-    const int lb = b - o.r0;
-    if (a < o.r0 + o.nn) return Dval[o.Dp + (a - o.r0) + (int64_t)lb * o.nn];
-    // a among the front's separator rows (sorted): binary search
-    const int32_t* s = septgt + o.sepoff;
-    int lo = 0, hi = o.na;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (s[mid] < a) lo = mid + 1;
-        else hi = mid;
-    }
-    return Lval[o.Lp + lo + (int64_t)lb * o.na];
-}
-
 // Separator rows of a front's K: na, padded to a multiple of 64 for a big
 // front (nn + na > 64).
 __device__ __forceinline__ int cpl_sep_rows(int nn, int na) { return nn + na > 64 ? (na + 63) & ~63 : na; }
 
+constexpr int CPL_SEP_LDS = 12288;  // separator rows staged in LDS (ints)
+
 // Workgroups (u, y) for root u (reference front fr[u]): columns c = y,
-// y + gridDim.y, ... of its K (nJ' x nJ' column-major, both triangles, at
-// K + koff[u]; nJ' = na' + nn).
+// y + gridDim.y, ... of its K (nJ' x nJ' column-major at K + koff[u]; the
+// lower triangle, which is all the factorization reads).  M(r, c) =
+// Sigma(a, b) with a = sep[r] >= b = sep[c]: column b's front o holds it in
+// its diagonal block (a in o's residual) or at a's position among o's
+// separator rows (sorted), found by binary search in LDS, where o's rows
+// are staged once per run of columns owned by o (in global memory when
+// there are more than CPL_SEP_LDS).
 __global__ __launch_bounds__(256) void k_cpl_gather(DevPlan P, const int32_t* fr, const double* Dval,
                                                     const double* Lval, const int32_t* septgt,
                                                     const int64_t* koff, double* K) {
-    const int u = blockIdx.x;
+    __shared__ int32_t srow[CPL_SEP_LDS];
+    __shared__ int s_owner;
+    const int u = blockIdx.x, tid = threadIdx.x;
     const OutFront o = P.out[fr[u]];
     const int nn = o.nn, na = o.na, np = cpl_sep_rows(nn, na), nj = nn + np;
     double* Kj = K + koff[u];
     const double* D11 = Dval + o.Dp;
     const double* L21 = Lval + o.Lp;
     const int32_t* sep = septgt + o.sepoff;
-    for (int c = blockIdx.y; c < nj; c += gridDim.y)
-        for (int r = c + threadIdx.x; r < nj; r += blockDim.x) {
-            double v;
-            if (r < na) v = sigma_at(P, Dval, Lval, septgt, sep[r], sep[c]);           // M
-            else if (r < np) v = r == c ? 1.0 : 0.0;                                    // padding
-            else if (c < na) v = L21[c + (int64_t)(nn - 1 - (r - np)) * na];            // J Sigma21'
-            else if (c < np) v = 0.0;
-            else v = D11[(nn - 1 - (c - np)) + (int64_t)(nn - 1 - (r - np)) * nn];      // J Sigma11 J
-            Kj[r + (int64_t)c * nj] = v;
-            Kj[c + (int64_t)r * nj] = v;  // both triangles: K is read as a full block
+    if (tid == 0) s_owner = -1;
+    __syncthreads();
+    for (int c = blockIdx.y; c < nj; c += gridDim.y) {
+        double* col = Kj + (int64_t)c * nj;
+        if (c < na) {
+            // M's column c, rows r >= c (uniform per workgroup)
+            const int b = sep[c];
+            const int ob = P.col2ref[b];
+            const OutFront q = P.out[ob];
+            const int lb = b - q.r0;
+            const bool staged = q.na <= CPL_SEP_LDS;
+            if (staged && ob != s_owner) {
+                __syncthreads();  // the previous owner's rows are no longer read
+                for (int i = tid; i < q.na; i += blockDim.x) srow[i] = septgt[q.sepoff + i];
+                if (tid == 0) s_owner = ob;
+                __syncthreads();
+            }
+            const int32_t* qs = staged ? srow : septgt + q.sepoff;
+            for (int r = c + tid; r < na; r += blockDim.x) {
+                const int a = sep[r];
+                double v;
+                if (a < q.r0 + q.nn) {
+                    v = Dval[q.Dp + (a - q.r0) + (int64_t)lb * q.nn];
+                } else {
+                    int lo = 0, hi = q.na;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (qs[mid] < a) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    v = Lval[q.Lp + lo + (int64_t)lb * q.na];
+                }
+                col[r] = v;
+            }
+            for (int r = na + tid; r < nj; r += blockDim.x)  // padding rows (0), then J Sigma21'
+                col[r] = r < np ? 0.0 : L21[c + (int64_t)(nn - 1 - (r - np)) * na];
+        } else if (c < np) {
+            for (int r = c + tid; r < nj; r += blockDim.x) col[r] = r == c ? 1.0 : 0.0;  // padding
+        } else {
+            for (int r = c + tid; r < nj; r += blockDim.x)  // J Sigma11 J
+                col[r] = D11[(nn - 1 - (c - np)) + (int64_t)(nn - 1 - (r - np)) * nn];
         }
+    }
 }
 
 // One workgroup per root u (reference front fr[u]): D11 and L21 of the
