@@ -134,6 +134,23 @@ def test_complete_factor_object(ct, oracle, cases):
     assert G.info > 0
 
 
+def test_complete_factor_object_upper(ct, cases):
+    """complete!(selinv!(cholesky!(F))) on a ChordalCholesky{:U} factor
+    object: its Dval/Lval hold the transposed blocks throughout, and F is
+    its own factor again."""
+    A, order, snd = cases["lap3d_20_nd"]
+    print("complete factor object uplo=U", flush=True)
+    F = ct.ChordalCholesky(A, alg=order, snd=snd, uplo="U")
+    ct.cholesky_(F)
+    D0, L0 = F.Dval, F.Lval
+    ct.selinv_(F)
+    ct.complete_(F)
+    assert F.info == 0
+    assert rel(F.Dval, D0) <= RTOL and rel(F.Lval, L0) <= RTOL
+    b = np.random.default_rng(5).standard_normal(A.shape[0])
+    assert np.abs(A @ ct.solve(F, b) - b).max() <= 1e-9 * np.abs(b).max()
+
+
 def test_complete_in_chunks(ct, oracle, cases):
     """The forest cut into about ten chunks (a small memory budget through
     MF_COMPLETE_CHUNK_ELEMS, read when a plan's completion is first built):
