@@ -233,13 +233,14 @@ struct CplChunk {
     int64_t* d_xoff = nullptr;
     SelTask* d_task = nullptr;    // the backward solves (k_selinv tasks)
     int32_t ntask = 0;
-    int64_t y_elems = 0;          // Y (and the same again for U)
+    int64_t nctr = 0;             // their counters
+    int64_t y_elems = 0;          // Y (and the same again for U and for R)
 };
 
 struct CplState {
     std::vector<CplChunk> chunks;
     std::vector<void*> allocs;
-    Arena arena;  // one chunk's factor object, Y, U and the solve counters
+    Arena arena;  // one chunk's factor object, Y, U, R and the solve counters
     double* d_eye = nullptr;  // a 64 x 64 identity
     int32_t* d_septgt = nullptr;
     double* d_sig = nullptr;  // Sigma (ndz + nlz)
@@ -1654,13 +1655,14 @@ static int ensure_cpl(mf_plan* pl) {
         std::vector<int64_t> xoff(m);
         for (int64_t u = 0; u < m; ++u) xoff[u] = SP.xoff[SH.owner[u]];
         c.ntask = (int32_t)SP.task.size();
+        c.nctr = std::max<int64_t>(SP.ncounters, NJ) + 1;
         c.y_elems = SP.x_elems;
         if ((rc = upload(fr, &c.d_fr, C->allocs)) || (rc = upload(poff, &c.d_poff, C->allocs)) ||
             (rc = upload(xoff, &c.d_xoff, C->allocs)) || (c.ntask && (rc = upload(SP.task, &c.d_task, C->allocs))))
             break;
-        // the arena holds the largest chunk's factor object, Y, U and counters
-        const size_t need = arena_factor_bytes(SH) + 2 * arena_round(8 * (size_t)c.y_elems) +
-                            arena_round(4 * (size_t)(NJ + 1));
+        // the arena holds the largest chunk's factor object, Y, U, R and counters
+        const size_t need = arena_factor_bytes(SH) + 3 * arena_round(8 * (size_t)c.y_elems) +
+                            arena_round(4 * (size_t)c.nctr);
         C->arena.size = std::max(C->arena.size, need);
         c.resptr = std::move(resptr);
         C->chunks.push_back(std::move(c));
@@ -1736,15 +1738,16 @@ static int cpl_chunk(mf_plan* plan, CplChunk& c, hipStream_t caller, int64_t* in
     rc = mf_factorize(sf, MF_CHOLESKY, nullptr, MF_REG_NONE, 0.0, 0.0, inf);
     double* Y = (double*)(C.arena.base + C.arena.used);
     if (!rc && !*inf && c.ntask) {
-        const HostPlan& SH = c.syn->H;
         double* U = (double*)((char*)Y + arena_round(8 * (size_t)c.y_elems));
-        int32_t* cnt = (int32_t*)((char*)U + arena_round(8 * (size_t)c.y_elems));
+        double* R = (double*)((char*)U + arena_round(8 * (size_t)c.y_elems));
+        int32_t* cnt = (int32_t*)((char*)R + arena_round(8 * (size_t)c.y_elems));
         DevSel<double> D{};
         D.base[SP_PANEL] = (double*)sf->panel;
         D.base[SP_W] = (double*)sf->W;
         D.base[SP_X] = Y;
         D.base[SP_T] = U;
         D.base[SP_LT] = C.d_eye;
+        D.base[SP_M] = R;
         D.task = c.d_task;
         D.xoff = c.d_xoff;
         D.cnt = cnt;
@@ -1752,7 +1755,7 @@ static int cpl_chunk(mf_plan* plan, CplChunk& c, hipStream_t caller, int64_t* in
         D.hang = sf->hang;
         D.stall = sf->stall;
         D.ntask = c.ntask;
-        launch_fill<int32_t>(cnt, (int64_t)SH.fronts.size() + 1, 0, sf->stream);
+        launch_fill<int32_t>(cnt, c.nctr, 0, sf->stream);
         launch_fill<uint32_t>(sf->ticket, 1, 0u, sf->stream);
         launch_selinv<double>(c.syn->D, D, sf->stream);
         if (hipGetLastError() != hipSuccess) rc = fail(MF_EHIP, "completion: launch failed");

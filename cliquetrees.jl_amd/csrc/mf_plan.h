@@ -118,6 +118,7 @@ struct SelPlan {
     double flops = 0;                      // multiply-add flops of the task list (2 per FMA)
     int32_t max_depth = 0;
     int64_t n_gemm = 0, n_small = 0, n_gather = 0;
+    int64_t ncounters = 0;                 // task counters the schedule uses (0: one per front)
 };
 
 // Builds the selected-inversion schedule of a plan (Cholesky factors).

@@ -298,21 +298,31 @@ std::string build_selinv(const HostPlan& H, SelPlan& S) {
 // the separator part padded to na' = 64 s rows and nn = nnr[J] residual
 // columns after it: Y = L^-T [0; I_nn] (nJ' x nn, ld nJ', in SP_X at
 // xoff[J]), whose rows are the last nn rows of L^-1.  RHS column tile c
-// has its identity block in row tile s + c, so its chain starts there:
-//   U(I, c) = -sum_{K = I+1}^{s+c} L(K, I)' Y(K, c)     (SP_T, same layout)
-//   Y(I, c) = W_I' U(I, c),   Y(s+c, c) = W_{s+c}' I    (SP_LT: a 64 x 64 identity)
-// for I = s + c - 1 down to 0.  Phases per front: U(I) then Y(I) for I
-// descending, each holding the RHS tiles whose chain has reached I; the
-// fronts' phases are interleaved round-robin (the longest chains first).
+// has its identity block in row tile s + c (= top), so its chain starts
+// there and runs up, I = top - 1 down to 0:
+//   U(I, c) = -sum_{K = I+1}^{min(I+H-1, top)} L(K, I)' Y(K, c)   (SP_T)
+//   R(I, c) = -sum_{K = I+H}^{top} L(K, I)' Y(K, c)                (SP_M)
+//   Y(I, c) = W_I' U(I, c) + W_I' R(I, c),  Y(top, c) = W_top' I   (SP_LT: identity)
+// The chain's own tasks (U and Y, at most H - 1 tiles of reduction) are
+// phases of the front's counter; R(I, c), the long rest of the reduction,
+// can start once Y(I + H, c) is done and runs beside the chain's next
+// H - 1 steps, on a counter of its own that Y(I, c) waits for.  The fronts'
+// phases are interleaved round-robin (the longest chains first), each R
+// task right after the phase that completes its inputs.
 std::string build_cpl_solve(const HostPlan& H, const std::vector<int32_t>& pad_na,
                             const std::vector<int32_t>& nnr, SelPlan& S) {
+    constexpr int LOOK = 8;  // H: tiles of the reduction on the chain
     const int32_t NJ = (int32_t)H.fronts.size();
     S = SelPlan();
     S.total.assign(NJ, 0);
     S.xoff.assign(NJ, -1);
     S.ltoff.assign(NJ, -1);
     S.moff.assign(NJ, -1);
-    std::vector<std::vector<std::vector<SelTask>>> ph(NJ);
+    struct Phase {
+        std::vector<SelTask> main, extra;  // extra: R tasks (own counters), after main
+    };
+    std::vector<std::vector<Phase>> ph(NJ);
+    int32_t nctr = NJ;
     for (int32_t J = 0; J < NJ; ++J) {
         if (H.is_small[J]) continue;
         const Front& F = H.fronts[J];
@@ -322,32 +332,52 @@ std::string build_cpl_solve(const HostPlan& H, const std::vector<int32_t>& pad_n
         const int64_t po = F.poff, wo = F.woff, yo = S.x_elems;
         S.xoff[J] = yo;
         S.x_elems += (int64_t)nJ * NB * nrt;
-        for (int I = s + nrt - 1; I >= 0; --I) {
-            const int m = tile_len(nJ, I);
-            std::vector<SelTask> U, Y;
-            for (int c = 0; c < nrt; ++c) {
-                const int top = s + c, mc = tile_len(nn, c);
-                if (I > top) continue;
-                const int64_t cy = (int64_t)NB * I + (int64_t)NB * c * nJ;  // tile (I, c) in Y / U
+        const int top_max = s + nrt - 1;
+        // phase index of Y(I): the phases run U(I), Y(I) for I = top_max .. 0
+        // (no U phase at I = top_max), so Y(I) is phase 2 (top_max - I)
+        std::vector<Phase> P(2 * (top_max + 1) - 1);
+        auto yph = [&](int I) { return 2 * (top_max - I); };
+        auto rows_to = [&](int K) { return std::min<int64_t>((int64_t)NB * (K + 1), nJ); };  // end of tile K
+        for (int c = 0; c < nrt; ++c) {
+            const int top = s + c, mc = tile_len(nn, c);
+            for (int I = top; I >= 0; --I) {
+                const int m = tile_len(nJ, I);
+                const int64_t cy = (int64_t)NB * I + (int64_t)NB * c * nJ;  // tile (I, c)
                 SelTask y = gemm_task(J, SP_X, yo + cy, nJ, m, mc, false);
-                if (I < top) {
-                    const int k = (int)(std::min<int64_t>((int64_t)NB * (top + 1), nJ) - (int64_t)NB * (I + 1));
-                    SelTask u = gemm_task(J, SP_T, yo + cy, nJ, m, mc, false);
-                    add_part(u, SP_PANEL, po + (int64_t)NB * (I + 1) + (int64_t)NB * I * nJ, nJ, 1, SP_X,
-                             yo + (int64_t)NB * (I + 1) + (int64_t)NB * c * nJ, nJ, 1, k, -1);
-                    U.push_back(u);
-                    add_part(y, SP_W, wo + (int64_t)NB * I, nJ, 1, SP_T, yo + cy, nJ, 1, m, 1);
-                } else {
+                if (I == top) {
                     add_part(y, SP_W, wo + (int64_t)NB * I, nJ, 1, SP_LT, 0, NB, 1, m, 1);
+                    P[yph(I)].main.push_back(y);
+                    continue;
                 }
-                Y.push_back(y);
+                const int knear = std::min(I + LOOK - 1, top);
+                SelTask u = gemm_task(J, SP_T, yo + cy, nJ, m, mc, false);
+                add_part(u, SP_PANEL, po + (int64_t)NB * (I + 1) + (int64_t)NB * I * nJ, nJ, 1, SP_X,
+                         yo + (int64_t)NB * (I + 1) + (int64_t)NB * c * nJ, nJ, 1,
+                         (int)(rows_to(knear) - (int64_t)NB * (I + 1)), -1);
+                P[yph(I) - 1].main.push_back(u);
+                add_part(y, SP_W, wo + (int64_t)NB * I, nJ, 1, SP_T, yo + cy, nJ, 1, m, 1);
+                if (I + LOOK <= top) {
+                    SelTask r = gemm_task(nctr, SP_M, yo + cy, nJ, m, mc, false);
+                    add_part(r, SP_PANEL, po + (int64_t)NB * (I + LOOK) + (int64_t)NB * I * nJ, nJ, 1, SP_X,
+                             yo + (int64_t)NB * (I + LOOK) + (int64_t)NB * c * nJ, nJ, 1,
+                             (int)(rows_to(top) - (int64_t)NB * (I + LOOK)), -1);
+                    r.b = J;  // the front whose chain it waits for
+                    P[yph(I + LOOK)].extra.push_back(r);
+                    add_part(y, SP_W, wo + (int64_t)NB * I, nJ, 1, SP_M, yo + cy, nJ, 1, m, 1);
+                    y.wait[1] = nctr;
+                    y.target[1] = 1;
+                    ++nctr;
+                }
+                P[yph(I)].main.push_back(y);
             }
-            if (!U.empty()) ph[J].push_back(std::move(U));
-            ph[J].push_back(std::move(Y));
         }
+        for (Phase& q : P)
+            if (!q.main.empty() || !q.extra.empty()) ph[J].push_back(std::move(q));
     }
+    S.ncounters = nctr;
     // ticket order: phase p of every front, for p = 0, 1, ...; fronts with
-    // more phases first; a task waits for its front's earlier phases
+    // more phases first; a chain task waits for its front's earlier phases,
+    // an R task for its front's phases through its own
     std::vector<int32_t> order;
     for (int32_t J = 0; J < NJ; ++J)
         if (!ph[J].empty()) order.push_back(J);
@@ -357,12 +387,17 @@ std::string build_cpl_solve(const HostPlan& H, const std::vector<int32_t>& pad_n
     for (size_t p = 0; !order.empty() && p < ph[order[0]].size(); ++p)
         for (int32_t J : order) {
             if (p >= ph[J].size()) break;  // sorted: the rest have fewer phases
-            for (SelTask t : ph[J][p]) {
+            for (SelTask t : ph[J][p].main) {
                 t.wait[0] = done[J] > 0 ? J : -1;
                 t.target[0] = done[J];
                 S.task.push_back(t);
             }
-            done[J] += (int32_t)ph[J][p].size();
+            done[J] += (int32_t)ph[J][p].main.size();
+            for (SelTask t : ph[J][p].extra) {
+                t.wait[0] = J;
+                t.target[0] = done[J];
+                S.task.push_back(t);
+            }
         }
     for (int32_t J = 0; J < NJ; ++J) S.total[J] = done[J];
     for (const SelTask& t : S.task) {
@@ -371,11 +406,11 @@ std::string build_cpl_solve(const HostPlan& H, const std::vector<int32_t>& pad_n
     }
     // bounds of every product against its buffers, and every wait against
     // the tasks of smaller tickets
-    const int64_t size[SP_N] = {H.panel_elems, H.w_elems, S.x_elems, S.x_elems, (int64_t)NB * NB, 0, 0};
+    const int64_t size[SP_N] = {H.panel_elems, H.w_elems, S.x_elems, S.x_elems, (int64_t)NB * NB, S.x_elems, 0};
     auto last = [](int64_t off, int rows, int cols, int ld, int trans) {
         return trans ? off + (cols - 1) + (int64_t)(rows - 1) * ld : off + (rows - 1) + (int64_t)(cols - 1) * ld;
     };
-    std::vector<int32_t> seen(NJ, 0);
+    std::vector<int32_t> seen(nctr, 0);
     for (const SelTask& t : S.task) {
         if (t.m <= 0 || t.n <= 0 || t.m > NB || t.n > NB || t.np < 1) return "completion solve: bad tile";
         if (t.c_off < 0 || last(t.c_off, t.m, t.n, t.ldc, 0) >= size[t.arrC]) return "completion solve: output out of bounds";
@@ -387,7 +422,10 @@ std::string build_cpl_solve(const HostPlan& H, const std::vector<int32_t>& pad_n
             if (pp.b_off < 0 || last(pp.b_off, t.n, pp.k, pp.ldb, pp.tb) >= size[pp.arrB])
                 return "completion solve: operand B out of bounds";
         }
-        if (t.wait[0] >= 0 && seen[t.wait[0]] < t.target[0]) return "completion solve: wait on a later ticket";
+        for (int w = 0; w < 2; ++w)
+            if (t.wait[w] >= 0 && (t.wait[w] >= nctr || seen[t.wait[w]] < t.target[w]))
+                return "completion solve: wait on a later ticket";
+        if (t.front < 0 || t.front >= nctr) return "completion solve: counter out of range";
         ++seen[t.front];
     }
     return std::string();
@@ -663,7 +701,7 @@ extern "C" int mf_cpl_solve_check(int64_t nfr, const int64_t* na, const int64_t*
     if (!err.empty()) return fail_msg(err);
     const double unset = std::nan("");
     std::vector<double> panel(H.panel_elems, 0.0), W(H.w_elems, 0.0), Y(S.x_elems, unset), U(S.x_elems, unset),
-        eye(NB * NB, 0.0);
+        R(S.x_elems, unset), eye(NB * NB, 0.0);
     for (int i = 0; i < NB; ++i) eye[i + NB * i] = 1.0;
     uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
     auto rnd = [&]() {
@@ -710,11 +748,12 @@ extern "C" int mf_cpl_solve_check(int64_t nfr, const int64_t* na, const int64_t*
             }
         Ls[J] = std::move(L);
     }
-    double* base[SP_N] = {panel.data(), W.data(), Y.data(), U.data(), eye.data(), nullptr, nullptr};
-    std::vector<int32_t> cnt(NJ, 0);
+    double* base[SP_N] = {panel.data(), W.data(), Y.data(), U.data(), eye.data(), R.data(), nullptr};
+    std::vector<int32_t> cnt(std::max<int64_t>(S.ncounters, NJ), 0);
     std::vector<double> acc((size_t)NB * NB);
     for (const SelTask& t : S.task) {
-        if (t.wait[0] >= 0 && cnt[t.wait[0]] < t.target[0]) return fail_msg("completion solve: wait not met");
+        for (int w = 0; w < 2; ++w)
+            if (t.wait[w] >= 0 && cnt[t.wait[w]] < t.target[w]) return fail_msg("completion solve: wait not met");
         std::fill(acc.begin(), acc.end(), 0.0);
         for (int q = 0; q < t.np; ++q) {
             const SelPart& p = t.p[q];

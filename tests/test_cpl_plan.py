@@ -34,6 +34,14 @@ def test_cpl_solve_schedule_mixed_fronts(ct, seed):
     assert st[0] > 0 and st[2] > 0
 
 
+def test_cpl_solve_schedule_long_chains(ct):
+    """Chains longer than the look-ahead (the reduction split between the
+    chain and a task of its own): a wide separator, a wide residual, both."""
+    rc, msg, worst, st = cpl_check([700, 0, 530, 1000], [70, 600, 300, 1])
+    assert rc == 0, msg
+    assert worst <= 1e-12
+
+
 def test_cpl_solve_schedule_of_a_grid(ct):
     """The fronts of a 12^3 nested-dissection Laplacian (the sizes the
     completion of that factor builds)."""
